@@ -1,0 +1,182 @@
+/* mtl4gpu.h -- C ABI of the MI355X (gfx950) sparse iterative-solve library.
+ *
+ * The library replaces MTL4/ITL's CPU path for CSR SpMV on compressed2D,
+ * IC(0)/ILU(0) factorisation and their triangular solves, and the vector
+ * kernels inside itl::cg and itl::bicgstab.  Every entry point below names the
+ * reference interface it stands in for (paths relative to the reference
+ * checkout; mtl/ = boost/numeric/mtl, itl/ = boost/numeric/itl).
+ *
+ * Conventions
+ *   - Every function returns an int status (M4G_OK = 0).  Status values that
+ *     mirror the reference: 1 = "Too many iterations." (basic_iteration.hpp:43-48);
+ *     M4G_ERR_NOT_SQUARE, M4G_ERR_MISSING_DIAGONAL, M4G_ERR_ORTHOGONALITY and
+ *     M4G_ERR_LOGIC mirror mtl::matrix_not_square, mtl::missing_diagonal,
+ *     itl::unexpected_orthogonality and mtl::logic_error.  Negative values are
+ *     runtime failures.  m4g_last_error() returns a message for the calling
+ *     thread's last failure.
+ *   - Handles own their device memory.  Host pointers are borrowed for the
+ *     duration of the call only.  m4g_vector_wrap borrows device memory.
+ *   - All work is queued on the library stream of the current device (or on
+ *     the stream given to m4g_set_stream).  Calls returning host scalars, and
+ *     downloads, synchronise that stream.
+ *   - One preconditioner handle must not be applied from two streams at once
+ *     (its sweep exchange buffers are per handle, as the reference's PC
+ *     solves share a static temporary, ic_0.hpp:72).
+ */
+#ifndef MTL4GPU_H
+#define MTL4GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+    M4G_OK = 0,
+    M4G_MAX_ITER = 1,               /* basic_iteration::check_max, basic_iteration.hpp:43-48 */
+    M4G_ERR_NOT_SQUARE = 2,         /* mtl::matrix_not_square (ic_0.hpp:147, ilu_0.hpp:49) */
+    M4G_ERR_MISSING_DIAGONAL = 3,   /* mtl::missing_diagonal (lower_trisolve.hpp:198-200) */
+    M4G_ERR_ORTHOGONALITY = 4,      /* itl::unexpected_orthogonality (bicgstab.hpp:46,51,59) */
+    M4G_ERR_LOGIC = 5,              /* mtl::logic_error (unsorted rows, dense input) */
+    M4G_ERR_SIZE = 6,               /* mtl::incompatible_size */
+    M4G_ERR_INVALID = 7,            /* null handle, bad enum, bad argument */
+    M4G_ERR_INTERRUPTED = 8,        /* interruptible_iteration flag set (interruptible_iteration.hpp:37-40) */
+    M4G_ERR_HIP = -1,               /* HIP runtime error */
+    M4G_ERR_NO_DEVICE = -2,         /* no GPU visible */
+    M4G_ERR_TIMEOUT = -3            /* a bounded device-side wait ran out (schedule fault) */
+};
+
+/* value types (Collection<T>::value_type) */
+enum { M4G_F64 = 0, M4G_F32 = 1 };
+/* index types of host CSR arrays: int32, int64 (= std::size_t, parameter.hpp:33), uint32 (TIM/ timing programs) */
+enum { M4G_IDX_I32 = 0, M4G_IDX_I64 = 1, M4G_IDX_U32 = 2 };
+/* assign modes of w = A*v, w += A*v, w -= A*v (mtl/operation/assign_mode.hpp:22-110) */
+enum { M4G_ASSIGN = 0, M4G_PLUS = 1, M4G_MINUS = 2 };
+/* preconditioner kinds: itl::pc::identity, diagonal, ic_0, ilu_0 */
+enum { M4G_PC_IDENTITY = 0, M4G_PC_DIAGONAL = 1, M4G_PC_IC0 = 2, M4G_PC_ILU0 = 3 };
+/* Krylov methods for the refinement / batched entry points */
+enum { M4G_METHOD_CG = 0, M4G_METHOD_BICGSTAB = 1 };
+
+typedef struct m4g_matrix_s* m4g_matrix;   /* device-resident mtl::compressed2D */
+typedef struct m4g_vector_s* m4g_vector;   /* device-resident mtl::dense_vector */
+typedef struct m4g_pc_s*     m4g_pc;       /* device-resident itl::pc::* */
+
+/* ---- library state -------------------------------------------------------- */
+const char* m4g_last_error(void);
+/* Selects the HIP device for this thread's subsequent calls (default 0). */
+int m4g_set_device(int device);
+/* Queues subsequent work of this thread on the given hipStream_t (NULL = library stream). */
+int m4g_set_stream(void* hip_stream);
+int m4g_synchronize(void);
+/* The hipStream_t the calling thread's work is queued on (for timing events). */
+void* m4g_get_stream(void);
+/* Library build string (arch, flags). */
+const char* m4g_version(void);
+
+/* ---- matrices: mtl::compressed2D (mtl/matrix/compressed2D.hpp:258-653) ------ */
+/* compressed2D(m, n, nnz, starts, indices, data): copies the three host arrays
+ * (compressed2D.hpp:355-364).  Rows must be sorted by column without
+ * duplicates, as the inserter leaves them (compressed2D.hpp:920-986); an
+ * unsorted row returns M4G_ERR_LOGIC.  values: double or float (value_type). */
+int m4g_matrix_create(m4g_matrix* A, int64_t nrows, int64_t ncols, int64_t nnz,
+                      const void* starts, const void* indices, const void* values,
+                      int index_type, int value_type);
+int m4g_matrix_destroy(m4g_matrix A);
+/* num_rows, num_cols, nnz */
+int m4g_matrix_dims(m4g_matrix A, int64_t* nrows, int64_t* ncols, int64_t* nnz);
+/* ref_major / ref_minor / data copies (compressed2D.hpp:610-625, 648), int64 indices. */
+int m4g_matrix_download(m4g_matrix A, int64_t* starts, int64_t* indices, double* values);
+/* Replaces the values (same pattern, CSR order); values_device != 0 means a device pointer. */
+int m4g_matrix_set_values(m4g_matrix A, const double* values, int values_device);
+
+/* ---- vectors: mtl::dense_vector (mtl/vector/dense_vector.hpp:51-270) -------- */
+int m4g_vector_create(m4g_vector* v, int64_t n, int value_type);
+/* Borrows n elements of device memory at device_ptr (not freed by destroy). */
+int m4g_vector_wrap(m4g_vector* v, void* device_ptr, int64_t n, int value_type);
+int m4g_vector_destroy(m4g_vector v);
+int m4g_vector_size(m4g_vector v, int64_t* n);
+void* m4g_vector_device_ptr(m4g_vector v);
+int m4g_vector_upload(m4g_vector v, const void* host);     /* host array of the vector's type */
+int m4g_vector_download(m4g_vector v, void* host);
+int m4g_vector_fill(m4g_vector v, double value);          /* v = value */
+int m4g_vector_copy(m4g_vector dst, m4g_vector src);      /* dst = src, converting precision */
+
+/* ---- level-1 kernels ------------------------------------------------------ */
+/* dot(x, y) = sum x_i * y_i (mtl/operation/dot.hpp:62-256) */
+int m4g_dot(m4g_vector x, m4g_vector y, double* result);
+/* two_norm(x) (mtl/operation/two_norm.hpp:33-61) */
+int m4g_two_norm(m4g_vector x, double* result);
+/* unary_dot(x) = sum x_i^2 (mtl/operation/unary_dot.hpp:29-65) */
+int m4g_unary_dot(m4g_vector x, double* result);
+/* y += alpha * x (vec_vec_aop_expr.hpp, e.g. cg.hpp:107) */
+int m4g_axpy(double alpha, m4g_vector x, m4g_vector y);
+/* y = x + beta * y (cg.hpp:102) */
+int m4g_xpay(m4g_vector x, double beta, m4g_vector y);
+/* (lazy(r) -= alpha*q) || (lazy(rr) = lazy_unary_dot(r)) (cg.hpp:109) */
+int m4g_axpy_unary_dot(double alpha, m4g_vector q, m4g_vector r, double* rr);
+
+/* ---- SpMV: smat_cvec_mult (mtl/operation/mat_vec_mult.hpp:619-689) ---------- */
+/* w = A*v, w += A*v, w -= A*v (crtp_base_vector.hpp:93-104, 192-198, 269-275) */
+int m4g_spmv(m4g_matrix A, m4g_vector v, m4g_vector w, int mode);
+/* r = b - A*x (compute_summand.hpp:42-60 then element-wise minus; cg.hpp:93) */
+int m4g_residual(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_vector r);
+/* (lazy(q) = A*p) || (lazy(pq) = lazy_dot(p, q)) (cg.hpp:104; fused_expr.hpp:42-91) */
+int m4g_spmv_dot(m4g_matrix A, m4g_vector p, m4g_vector q, double* pq);
+
+/* ---- preconditioners (itl/pc/) ---------------------------------------------- */
+/* ic_0<Matrix, Value>(A) (ic_0.hpp:39-194), ilu_0<Matrix, Value>(A) (ilu_0.hpp:85-92,
+ * ilu.hpp:33-133), diagonal<Matrix> (diagonal.hpp:28-103), identity (identity.hpp:24-71).
+ * factor_value_type is Value: M4G_F64, or M4G_F32 to store the factors in float
+ * after factorising in double (ic_0.hpp:122-130; ilu_0.hpp:80-81). */
+int m4g_pc_create(m4g_pc* P, m4g_matrix A, int kind, int factor_value_type);
+int m4g_pc_destroy(m4g_pc P);
+/* P.solve(x, y) / solve(P, x) (ic_0.hpp:78-87, 257-262; ilu.hpp:83-91, 189-195) */
+int m4g_pc_solve(m4g_pc P, m4g_vector x, m4g_vector y);
+/* P.adjoint_solve(x, y) (ic_0.hpp:97-103; ilu.hpp:105-115) */
+int m4g_pc_adjoint_solve(m4g_pc P, m4g_vector x, m4g_vector y);
+/* Queues P.solve(x, y) without waiting or checking the sweeps' wait
+ * counters (for timing loops); the next m4g_pc_solve reports any fault. */
+int m4g_pc_apply_async(m4g_pc P, m4g_vector x, m4g_vector y);
+/* get_L() / get_U() (ic_0.hpp:106-107; ilu.hpp:118-119): which = 0 for L, 1 for U.
+ * IC(0): L = trans(U), inverted diagonal last in each row; U: inverted diagonal first.
+ * ILU(0): L strict lower (unit diagonal implied); U with inverted diagonal first. */
+int m4g_pc_factor_nnz(m4g_pc P, int which, int64_t* nnz);
+int m4g_pc_download_factor(m4g_pc P, int which, int64_t* starts, int64_t* indices, double* values);
+
+/* Schedule statistics of a factor's sweeps (no reference counterpart; diagnostics). */
+typedef struct {
+    int64_t levels_lower, levels_upper;   /* natural-order DAG depth of each sweep */
+    int64_t tiles_lower, tiles_upper;     /* workgroup tiles of the pipelined sweeps */
+    int64_t steps_lower, steps_upper;     /* max in-tile levels over tiles */
+    int64_t grid_nx, grid_ny, grid_nz;    /* inferred grid (0 if none) */
+    int64_t tile_nx, tile_ny, tile_nz;    /* tile box (0 for row-block tiles) */
+} m4g_pc_stats;
+int m4g_pc_stats_get(m4g_pc P, m4g_pc_stats* stats);
+
+/* ---- iteration control: itl::basic_iteration (basic_iteration.hpp:24-165) ---- */
+typedef struct {
+    int max_iter;          /* in */
+    double rtol;           /* in: relative tolerance */
+    double atol;           /* in: absolute tolerance */
+    double norm_r0;        /* in: basic_iteration(r0, ...) computes |two_norm(r0)|; the tests pass b */
+    int iterations;        /* in/out: counter i (kept across calls, like the iteration object) */
+    int error;             /* in/out: error code; like basic_iteration's member it is only ever set */
+    double resid;          /* out: last residual given to finished() */
+    double* history;       /* optional host array: history[i] = resid at the check of iteration i */
+    int history_capacity;
+    const volatile int* interrupt; /* optional host flag polled between iteration batches */
+} m4g_iteration;
+
+/* itl::cg(A, x, b, P, iter) (cg.hpp:75-112; identity PC -> cg.hpp:39-72).
+ * x is in/out.  Returns the iteration's error code (0 / 1) or an error status. */
+int m4g_cg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* iter);
+/* itl::bicgstab(A, x, b, M, iter) (bicgstab.hpp:27-78). */
+int m4g_bicgstab(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* iter);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MTL4GPU_H */

@@ -1,0 +1,156 @@
+// Vector kernels of the CG / BiCGStab loops: dot, squared norm, axpy, xpay,
+// fused axpy + squared norm, fill and precision conversion.  All are
+// HBM-bound streams; reductions are deterministic two-stage sums (a fixed
+// partition into blocks, a fixed butterfly inside each block, and the last
+// block adding the block partials in order).
+#include "m4g_internal.hpp"
+#include "m4g_device.hpp"
+
+namespace m4g {
+
+constexpr int kNT = 256;
+
+static int reduce_grid(int64_t n)
+{
+    int64_t g = (n + kNT * 4 - 1) / (kNT * 4);
+    if (g < 1) g = 1;
+    if (g > kReduceBlocks) g = kReduceBlocks;
+    return int(g);
+}
+
+__global__ __launch_bounds__(kNT) void k_dot(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                             double* partials, unsigned* counter, double* out)
+{
+    __shared__ double scratch[kNT / kWave];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
+        acc += x[i] * y[i];
+    acc = block_sum<kNT>(acc, scratch);
+    double total;
+    if (last_block_total<kNT>(acc, partials, counter, &total, scratch)) *out = total;
+}
+
+__global__ __launch_bounds__(kNT) void k_sumsq(int64_t n, const double* __restrict__ x, double* partials,
+                                               unsigned* counter, double* out)
+{
+    __shared__ double scratch[kNT / kWave];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
+        acc += x[i] * x[i];
+    acc = block_sum<kNT>(acc, scratch);
+    double total;
+    if (last_block_total<kNT>(acc, partials, counter, &total, scratch)) *out = total;
+}
+
+__global__ __launch_bounds__(kNT) void k_axpy_sumsq(int64_t n, double alpha, const double* __restrict__ q,
+                                                    double* __restrict__ r, double* partials, unsigned* counter,
+                                                    double* out)
+{
+    __shared__ double scratch[kNT / kWave];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT) {
+        double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    acc = block_sum<kNT>(acc, scratch);
+    double total;
+    if (last_block_total<kNT>(acc, partials, counter, &total, scratch)) *out = total;
+}
+
+__global__ __launch_bounds__(kNT) void k_axpy(int64_t n, double alpha, const double* __restrict__ x,
+                                              double* __restrict__ y)
+{
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
+        y[i] = y[i] + alpha * x[i];
+}
+
+__global__ __launch_bounds__(kNT) void k_xpay(int64_t n, const double* __restrict__ x, double beta,
+                                              double* __restrict__ y)
+{
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
+        y[i] = x[i] + beta * y[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_fill(int64_t n, T value, T* __restrict__ x)
+{
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT) x[i] = value;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(kNT) void k_convert(int64_t n, const S* __restrict__ src, D* __restrict__ dst)
+{
+    for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
+        dst[i] = static_cast<D>(src[i]);
+}
+
+static int stream_grid(int64_t n)
+{
+    int64_t g = (n + kNT - 1) / kNT;
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return int(g);
+}
+
+void launch_dot(int64_t n, const double* x, const double* y, double* out, hipStream_t s)
+{
+    ReduceWs& ws = reduce_ws();
+    hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, y, ws.partials, ws.counters, out);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s)
+{
+    ReduceWs& ws = reduce_ws();
+    hipLaunchKernelGGL(k_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, ws.partials, ws.counters, out);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_axpy_sumsq(int64_t n, double alpha, const double* q, double* r, double* out, hipStream_t s)
+{
+    ReduceWs& ws = reduce_ws();
+    hipLaunchKernelGGL(k_axpy_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, alpha, q, r, ws.partials,
+                       ws.counters, out);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_axpy, dim3(stream_grid(n)), dim3(kNT), 0, s, n, alpha, x, y);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_xpay(int64_t n, const double* x, double beta, double* y, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_xpay, dim3(stream_grid(n)), dim3(kNT), 0, s, n, x, beta, y);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_fill(int64_t n, double value, void* x, int type, hipStream_t s)
+{
+    if (n == 0) return;
+    if (type == M4G_F64)
+        hipLaunchKernelGGL(k_fill<double>, dim3(stream_grid(n)), dim3(kNT), 0, s, n, value, (double*)x);
+    else
+        hipLaunchKernelGGL(k_fill<float>, dim3(stream_grid(n)), dim3(kNT), 0, s, n, float(value), (float*)x);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_convert(int64_t n, const void* src, int st, void* dst, int dt, hipStream_t s)
+{
+    if (n == 0) return;
+    if (st == M4G_F64 && dt == M4G_F32)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_grid(n)), dim3(kNT), 0, s, n,
+                           (const double*)src, (float*)dst);
+    else if (st == M4G_F32 && dt == M4G_F64)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(stream_grid(n)), dim3(kNT), 0, s, n,
+                           (const float*)src, (double*)dst);
+    else
+        throw Error(M4G_ERR_INVALID, "launch_convert: unsupported types");
+    M4G_HIP(hipGetLastError());
+}
+
+}  // namespace m4g

@@ -1,0 +1,692 @@
+// Device-resident CG and BiCGStab (itl/krylov/cg.hpp:39-112,
+// itl/krylov/bicgstab.hpp:27-78) with the iteration control of
+// itl::basic_iteration (basic_iteration.hpp:24-165) evaluated on the GPU.
+//
+// Every scalar of the method (rho, alpha, omega, ...) lives in a device state
+// block.  Each reduction kernel finishes in its last workgroup, which applies
+// the method's scalar update and, where the reference tests convergence, the
+// test itself.  Kernels of an iteration that starts after the state says
+// "finished" return at once, so the host can queue iterations in batches and
+// only polls the finished flag between batches; results and iteration counts
+// are those of the sequential method.
+#include "m4g_internal.hpp"
+#include "m4g_device.hpp"
+#include "spmv_dev.hpp"
+#include "pc.hpp"
+#include "krylov.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+namespace m4g {
+
+constexpr int kNT = 256;
+constexpr int kRedBlocks = 1024;
+
+// ---------------------------------------------------------------------------
+// iteration control on the device
+// ---------------------------------------------------------------------------
+__device__ inline void record(KState* st, double resid)
+{
+    st->resid = resid;                                  // basic_iteration.hpp:83
+    if (st->history && st->i < st->history_cap) st->history[st->i] = resid;
+}
+
+__device__ inline bool converged(const KState* st, double r)
+{
+    if (st->norm_r0 == 0) return r <= st->atol;        // :90-91
+    return r <= st->rtol * st->norm_r0 || r <= st->atol;   // :92
+}
+
+// finished(r): converged, or check_max (:43-48).  Returns true when finished.
+__device__ inline bool finished(KState* st, double resid)
+{
+    record(st, resid);
+    if (converged(st, resid)) {
+        st->done = 1;
+        return true;
+    }
+    if (st->i >= st->max_iter) {
+        st->error = M4G_MAX_ITER;
+        st->done = 1;
+        return true;
+    }
+    return false;
+}
+
+// while (!finished(r)) { ++iter; ... }
+__device__ inline bool top_check(KState* st, double resid)
+{
+    if (finished(st, resid)) return true;
+    st->i += 1;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// reductions: K sums per thread -> per-block partials -> last block
+// ---------------------------------------------------------------------------
+template <int K>
+struct Sums {
+    double v[K];
+};
+
+template <int NT, int K>
+__device__ inline bool reduce_last(Sums<K>& s, double* partials, unsigned* counter, Sums<K>& tot)
+{
+    __shared__ double scratch[NT / kWave];
+    __shared__ bool am_last;
+#pragma unroll
+    for (int k = 0; k < K; k++) s.v[k] = block_sum<NT>(s.v[k], scratch);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[k * gridDim.x + blockIdx.x] = s.v[k];
+        __threadfence();
+        am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double acc = 0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += NT)
+            acc += __hip_atomic_load(partials + k * gridDim.x + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tot.v[k] = block_sum<NT>(acc, scratch);
+    }
+    if (threadIdx.x == 0) *counter = 0;
+    return threadIdx.x == 0;
+}
+
+#define GRID_STRIDE(i, n) for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < (n); i += int64_t(gridDim.x) * kNT)
+
+// ---------------------------------------------------------------------------
+// CG kernels
+// ---------------------------------------------------------------------------
+// rr = r.r; initial finished() test.  For the identity PC rho = rr (cg.hpp:54).
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_start(int64_t n, const T* __restrict__ r, KState* st, double* partials,
+                                                   unsigned* counter)
+{
+    Sums<1> s{{0}};
+    GRID_STRIDE(i, n) s.v[0] += double(r[i]) * double(r[i]);
+    Sums<1> t;
+    if (reduce_last<kNT, 1>(s, partials, counter, t)) {
+        st->rr = t.v[0];
+        st->rho = t.v[0];
+        top_check(st, sqrt(fabs(t.v[0])));
+    }
+}
+
+template <typename T>
+__device__ inline double prec_mul(T a, T b)
+{
+    return double(a) * double(b);
+}
+
+// rho = dot(r, z) (cg.hpp:97).  T=float accumulates products in float per
+// thread (FP32 accumulation of the inner solve), partials in double.
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_rho(int64_t n, const T* __restrict__ r, const T* __restrict__ z,
+                                                 KState* st, double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    Sums<1> s{{0}};
+    T acc = 0;
+    GRID_STRIDE(i, n) acc += r[i] * z[i];
+    s.v[0] = double(acc);
+    Sums<1> t;
+    if (reduce_last<kNT, 1>(s, partials, counter, t)) st->rho = t.v[0];
+}
+
+// diagonal PC fused with the dot: z = inv_diag .* r; rho = r.z
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_diag_rho(int64_t n, const double* __restrict__ inv, const T* __restrict__ r,
+                                                      T* __restrict__ z, KState* st, double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    Sums<1> s{{0}};
+    T acc = 0;
+    GRID_STRIDE(i, n)
+    {
+        T zi = T(inv[i]) * r[i];
+        z[i] = zi;
+        acc += r[i] * zi;
+    }
+    s.v[0] = double(acc);
+    Sums<1> t;
+    if (reduce_last<kNT, 1>(s, partials, counter, t)) st->rho = t.v[0];
+}
+
+// p = z (first) or p = z + (rho/rho_1) * p (cg.hpp:99-102; no-PC: :57-60)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_p(int64_t n, const T* __restrict__ z, T* __restrict__ p, const KState* st)
+{
+    if (st->done) return;
+    if (st->i <= 1) {                                    // iter.first()
+        GRID_STRIDE(i, n) p[i] = z[i];
+    } else {
+        const T s = T(st->rho / st->rho_1);
+        GRID_STRIDE(i, n) p[i] = z[i] + s * p[i];
+    }
+}
+
+// (lazy(q) = A*p) || (lazy(alpha_1) = lazy_dot(p, q)); alpha = rho / alpha_1 (cg.hpp:104-105)
+template <typename T>
+__global__ __launch_bounds__(kSpmvNT) void k_cg_spmv_dot(int64_t n, int64_t nslices, int64_t chunk,
+                                                          const int64_t* __restrict__ slice_ptr,
+                                                          const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                          const T* __restrict__ p, T* __restrict__ q, KState* st,
+                                                          double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    const int64_t g = xcd_logical_block(chunk);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t s = g * kSliceWaves + wave;
+    Sums<1> acc{{0}};
+    if (s < nslices) {
+        const int64_t base = slice_ptr[s];
+        const int width = int((slice_ptr[s + 1] - base) / kWave);
+        const T qv = sell_row_sum(cols, vals, p, base, width, lane);
+        const int64_t row = s * kWave + lane;
+        if (row < n) {
+            q[row] = qv;
+            acc.v[0] = double(p[row] * qv);
+        }
+    }
+    Sums<1> t;
+    if (reduce_last<kSpmvNT, 1>(acc, partials, counter, t)) {
+        st->alpha_1 = t.v[0];
+        st->alpha = st->rho / st->alpha_1;
+    }
+}
+
+// x += alpha*p; rho_1 = rho; (lazy(r) -= alpha*q) || (lazy(rr) = lazy_unary_dot(r)); then the
+// loop test finished(sqrt|rr|) (cg.hpp:107-109, 95-96)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_update(int64_t n, const T* __restrict__ p, const T* __restrict__ q,
+                                                    T* __restrict__ x, T* __restrict__ r, KState* st, int identity_pc,
+                                                    double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    const T alpha = T(st->alpha);
+    T acc = 0;
+    GRID_STRIDE(i, n)
+    {
+        x[i] = x[i] + alpha * p[i];
+        T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    Sums<1> s{{double(acc)}}, t;
+    if (reduce_last<kNT, 1>(s, partials, counter, t)) {
+        st->rho_1 = st->rho;
+        st->rr = t.v[0];
+        if (identity_pc) st->rho = t.v[0];
+        top_check(st, sqrt(fabs(t.v[0])));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab kernels (bicgstab.hpp:40-76)
+// ---------------------------------------------------------------------------
+// Loop-top: finished(two_norm(r)); ++iter; rho_1 = dot(rtilde, r) with its
+// zero test; the omega test and beta for the p update (:43-53).
+__device__ inline void bicg_top(KState* st, double rr, double rtr)
+{
+    if (top_check(st, sqrt(rr))) return;
+    st->rho_1 = rtr;
+    if (st->rho_1 == 0.0) {
+        st->error = M4G_ERR_ORTHOGONALITY;
+        st->done = 1;
+        return;
+    }
+    if (st->i > 1) {
+        if (st->omega == 0.0) {
+            st->error = M4G_ERR_ORTHOGONALITY;
+            st->done = 1;
+            return;
+        }
+        st->beta = (st->rho_1 / st->rho_2) * (st->alpha / st->omega);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_bicg_start(int64_t n, const T* __restrict__ r, T* __restrict__ rtilde,
+                                                     KState* st, double* partials, unsigned* counter)
+{
+    Sums<2> s{{0, 0}};
+    T a0 = 0, a1 = 0;
+    GRID_STRIDE(i, n)
+    {
+        T ri = r[i];
+        rtilde[i] = ri;                                   // rtilde = r (:41)
+        a0 += ri * ri;
+        a1 += ri * ri;
+    }
+    s.v[0] = double(a0);
+    s.v[1] = double(a1);
+    Sums<2> t;
+    if (reduce_last<kNT, 2>(s, partials, counter, t)) bicg_top(st, t.v[0], t.v[1]);
+}
+
+// p = r (first) or p = r + beta*(p - omega*v) (:48-54)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_bicg_p(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
+                                                 T* __restrict__ p, const KState* st)
+{
+    if (st->done) return;
+    if (st->i <= 1) {
+        GRID_STRIDE(i, n) p[i] = r[i];
+    } else {
+        const T beta = T(st->beta), omega = T(st->omega);
+        GRID_STRIDE(i, n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    }
+}
+
+// v = A*phat; gamma = dot(rtilde, v) with its zero test; alpha = rho_1/gamma (:56-61)
+template <typename T>
+__global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_gamma(int64_t n, int64_t nslices, int64_t chunk,
+                                                              const int64_t* __restrict__ slice_ptr,
+                                                              const int32_t* __restrict__ cols,
+                                                              const T* __restrict__ vals, const T* __restrict__ phat,
+                                                              const T* __restrict__ rtilde, T* __restrict__ v,
+                                                              KState* st, double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    const int64_t g = xcd_logical_block(chunk);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t s = g * kSliceWaves + wave;
+    Sums<1> acc{{0}};
+    if (s < nslices) {
+        const int64_t base = slice_ptr[s];
+        const int width = int((slice_ptr[s + 1] - base) / kWave);
+        const T vv = sell_row_sum(cols, vals, phat, base, width, lane);
+        const int64_t row = s * kWave + lane;
+        if (row < n) {
+            v[row] = vv;
+            acc.v[0] = double(rtilde[row] * vv);
+        }
+    }
+    Sums<1> t;
+    if (reduce_last<kSpmvNT, 1>(acc, partials, counter, t)) {
+        st->gamma = t.v[0];
+        if (st->gamma == 0.0) {
+            st->error = M4G_ERR_ORTHOGONALITY;
+            st->done = 1;
+            return;
+        }
+        st->alpha = st->rho_1 / st->gamma;
+    }
+}
+
+// s = r - alpha*v; finished(two_norm(s)) -> early exit flag (:62-67)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_bicg_s(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
+                                                 T* __restrict__ s_, KState* st, double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    const T alpha = T(st->alpha);
+    T acc = 0;
+    GRID_STRIDE(i, n)
+    {
+        T si = r[i] - alpha * v[i];
+        s_[i] = si;
+        acc += si * si;
+    }
+    Sums<1> s{{double(acc)}}, t;
+    if (reduce_last<kNT, 1>(s, partials, counter, t)) {
+        if (finished(st, sqrt(t.v[0]))) st->mid_exit = 1;
+    }
+}
+
+// early exit: x += alpha*phat, once (:65)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_bicg_exit(int64_t n, const T* __restrict__ phat, T* __restrict__ x, KState* st,
+                                                    unsigned* counter)
+{
+    if (st->mid_exit != 1) return;
+    const T alpha = T(st->alpha);
+    GRID_STRIDE(i, n) x[i] = x[i] + alpha * phat[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(counter, 1u) == gridDim.x - 1) {
+            st->mid_exit = 2;
+            *counter = 0;
+        }
+    }
+}
+
+// t = A*shat; omega = dot(t, s) / dot(t, t) (:69-70)
+template <typename T>
+__global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_omega(int64_t n, int64_t nslices, int64_t chunk,
+                                                              const int64_t* __restrict__ slice_ptr,
+                                                              const int32_t* __restrict__ cols,
+                                                              const T* __restrict__ vals, const T* __restrict__ shat,
+                                                              const T* __restrict__ s_, T* __restrict__ t_, KState* st,
+                                                              double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    const int64_t g = xcd_logical_block(chunk);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t s = g * kSliceWaves + wave;
+    Sums<2> acc{{0, 0}};
+    if (s < nslices) {
+        const int64_t base = slice_ptr[s];
+        const int width = int((slice_ptr[s + 1] - base) / kWave);
+        const T tv = sell_row_sum(cols, vals, shat, base, width, lane);
+        const int64_t row = s * kWave + lane;
+        if (row < n) {
+            t_[row] = tv;
+            acc.v[0] = double(tv * s_[row]);
+            acc.v[1] = double(tv * tv);
+        }
+    }
+    Sums<2> t;
+    if (reduce_last<kSpmvNT, 2>(acc, partials, counter, t)) st->omega = t.v[0] / t.v[1];
+}
+
+// x += omega*shat + alpha*phat; r = s - omega*t; rho_2 = rho_1; loop top (:72-75, 43-53)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_bicg_update(int64_t n, const T* __restrict__ shat, const T* __restrict__ phat,
+                                                      const T* __restrict__ s_, const T* __restrict__ t_,
+                                                      const T* __restrict__ rtilde, T* __restrict__ x,
+                                                      T* __restrict__ r, KState* st, double* partials, unsigned* counter)
+{
+    if (st->done) return;
+    const T alpha = T(st->alpha), omega = T(st->omega);
+    T a0 = 0, a1 = 0;
+    GRID_STRIDE(i, n)
+    {
+        x[i] = x[i] + (omega * shat[i] + alpha * phat[i]);
+        T ri = s_[i] - omega * t_[i];
+        r[i] = ri;
+        a0 += ri * ri;
+        a1 += rtilde[i] * ri;
+    }
+    Sums<2> s{{double(a0), double(a1)}}, t;
+    if (reduce_last<kNT, 2>(s, partials, counter, t)) {
+        st->rho_2 = st->rho_1;
+        bicg_top(st, t.v[0], t.v[1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static int red_grid(int64_t n)
+{
+    return int(std::max<int64_t>(1, std::min<int64_t>((n + kNT * 4 - 1) / (kNT * 4), kRedBlocks)));
+}
+
+void KrylovWs::prepare(int64_t spmv_grid, int history_cap)
+{
+    const int64_t need = 2 * std::max<int64_t>(spmv_grid, kRedBlocks);
+    if (int64_t(partials.count) < need) partials.reset(need);
+    if (!counter.ptr) {
+        counter.reset(4);
+        M4G_HIP(hipMemsetAsync(counter.ptr, 0, 4 * sizeof(unsigned), current_stream()));
+    }
+    if (!state.ptr) state.reset(1);
+    if (history_cap > 0 && int64_t(history.count) < history_cap) history.reset(history_cap);
+    if (!host_state) M4G_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_state), 2 * sizeof(KState), hipHostMallocDefault));
+}
+
+KrylovWs::~KrylovWs()
+{
+    if (host_state) (void)hipHostFree(host_state);
+}
+
+static void spmv_geom(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
+{
+    int64_t groups = (A.sell.nslices + kSliceWaves - 1) / kSliceWaves;
+    chunk = (groups + kXcds - 1) / kXcds;
+    grid = std::max<int64_t>(1, chunk * kXcds);
+}
+
+static int batch_size()
+{
+    static int b = [] {
+        const char* e = std::getenv("M4G_ITER_BATCH");
+        return e ? std::max(1, std::atoi(e)) : 8;
+    }();
+    return b;
+}
+
+static void init_state(KState& h, const m4g_iteration* it, double* hist_dev)
+{
+    std::memset(&h, 0, sizeof(h));
+    h.norm_r0 = it->norm_r0;
+    h.rtol = it->rtol;
+    h.atol = it->atol;
+    h.i = it->iterations;
+    h.error = it->error;
+    h.max_iter = it->max_iter;
+    h.history = it->history ? hist_dev : nullptr;
+    h.history_cap = it->history ? it->history_capacity : 0;
+}
+
+// Runs `body(k)` for batches of iterations until the device state reports
+// finished; polls the flag of batch b while batch b+1 is queued.
+template <typename Body>
+static void run_batches(KrylovWs& ws, const m4g_iteration* it, Body&& body)
+{
+    hipStream_t s = current_stream();
+    hipEvent_t ev[2];
+    M4G_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    M4G_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const int B = batch_size();
+    int slot = 0, pending = -1;
+    int64_t launched = 0;
+    bool finished = false;
+    // guard: never queue more batches than max_iter allows (+1 for the check)
+    const int64_t cap = int64_t(std::max(it->max_iter - it->iterations, 0)) + B + 1;
+    try {
+        while (!finished) {
+            for (int k = 0; k < B; k++) body();
+            launched += B;
+            M4G_HIP(hipMemcpyAsync(ws.host_state + slot, ws.state.ptr, sizeof(KState), hipMemcpyDeviceToHost, s));
+            M4G_HIP(hipEventRecord(ev[slot], s));
+            if (pending >= 0) {
+                M4G_HIP(hipEventSynchronize(ev[pending]));
+                if (ws.host_state[pending].done) finished = true;
+                if (it->interrupt && *it->interrupt) finished = true;
+            }
+            pending = slot;
+            slot ^= 1;
+            if (launched > cap) {
+                M4G_HIP(hipEventSynchronize(ev[pending]));
+                finished = true;
+            }
+        }
+        M4G_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+}
+
+static int finish_state(KrylovWs& ws, m4g_iteration* it, int extra_fault)
+{
+    KState h;
+    M4G_HIP(hipMemcpy(&h, ws.state.ptr, sizeof(KState), hipMemcpyDeviceToHost));
+    const int i0 = it->iterations;
+    if (it->history && h.history) {
+        int hi = std::min(h.i, it->history_capacity - 1);
+        if (hi >= i0)
+            M4G_HIP(hipMemcpy(it->history + i0, ws.history.ptr + i0, size_t(hi - i0 + 1) * sizeof(double),
+                              hipMemcpyDeviceToHost));
+    }
+    it->iterations = h.i;
+    it->resid = h.resid;
+    bool interrupted = !h.done && it->interrupt && *it->interrupt;
+    M4G_REQUIRE(extra_fault == 0, M4G_ERR_TIMEOUT, "sweep wait timed out");
+    if (interrupted) {
+        it->error = M4G_ERR_INTERRUPTED;
+        return M4G_ERR_INTERRUPTED;
+    }
+    M4G_REQUIRE(h.done, M4G_ERR_INVALID, "solver stopped before the iteration finished");
+    it->error = h.error;
+    return h.error;
+}
+
+template <typename T>
+static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iteration* it, KrylovWs& ws)
+{
+    const int64_t n = A.nrows;
+    hipStream_t st = current_stream();
+    const int vt = sizeof(T) == 8 ? M4G_F64 : M4G_F32;
+    const T* vals = sizeof(T) == 8 ? (const T*)A.sell.vals.ptr : (const T*)A.sell.vals32.ptr;
+    M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
+    int64_t chunk, sgrid;
+    spmv_geom(A, chunk, sgrid);
+    ws.prepare(sgrid, it->history ? it->history_capacity : 0);
+    KState h;
+    init_state(h, it, ws.history.ptr);
+    M4G_HIP(hipMemcpyAsync(ws.state.ptr, &h, sizeof(KState), hipMemcpyHostToDevice, st));
+    DevBuf<T> p(n), q(n), r(n), z(n);
+    T* x = (T*)xp;
+    const T* b = (const T*)bp;
+    KState* S = ws.state.ptr;
+    double* part = ws.partials.ptr;
+    unsigned* cnt = ws.counter.ptr;
+    const int rg = red_grid(n);
+    // r = b - A*x (cg.hpp:93)
+    launch_residual_t<T>(A, x, b, r.ptr, st);
+    const bool identity = !P || P->kind == M4G_PC_IDENTITY;
+    hipLaunchKernelGGL(k_cg_start<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, S, part, cnt);
+    M4G_HIP(hipGetLastError());
+    auto body = [&] {
+        const T* zz = r.ptr;
+        if (!identity) {
+            if (P->kind == M4G_PC_DIAGONAL) {
+                hipLaunchKernelGGL(k_cg_diag_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const double*)P->inv_diag.ptr,
+                                   (const T*)r.ptr, z.ptr, S, part, cnt);
+            } else {
+                pc_apply_skip(*P, r.ptr, z.ptr, vt, &S->done, st);
+                hipLaunchKernelGGL(k_cg_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)z.ptr, S,
+                                   part, cnt);
+            }
+            zz = z.ptr;
+        }
+        hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
+        hipLaunchKernelGGL(k_cg_spmv_dot<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
+                           A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)p.ptr, q.ptr, S, part, cnt);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr, x, r.ptr,
+                           S, int(identity), part, cnt);
+        M4G_HIP(hipGetLastError());
+    };
+    run_batches(ws, it, body);
+    int fault = 0;
+    if (P && P->kind >= M4G_PC_IC0) fault = check_sweep_fault(P->lower) | check_sweep_fault(P->upper);
+    return finish_state(ws, it, fault);
+}
+
+template <typename T>
+static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iteration* it, KrylovWs& ws)
+{
+    const int64_t n = A.nrows;
+    hipStream_t st = current_stream();
+    const int vt = sizeof(T) == 8 ? M4G_F64 : M4G_F32;
+    const T* vals = sizeof(T) == 8 ? (const T*)A.sell.vals.ptr : (const T*)A.sell.vals32.ptr;
+    M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
+    int64_t chunk, sgrid;
+    spmv_geom(A, chunk, sgrid);
+    ws.prepare(sgrid, it->history ? it->history_capacity : 0);
+    KState h;
+    init_state(h, it, ws.history.ptr);
+    M4G_HIP(hipMemcpyAsync(ws.state.ptr, &h, sizeof(KState), hipMemcpyHostToDevice, st));
+    DevBuf<T> p(n), phat(n), s_(n), shat(n), t_(n), v(n), r(n), rtilde(n);
+    T* x = (T*)xp;
+    const T* b = (const T*)bp;
+    KState* S = ws.state.ptr;
+    double* part = ws.partials.ptr;
+    unsigned* cnt = ws.counter.ptr;
+    const int rg = red_grid(n);
+    launch_residual_t<T>(A, x, b, r.ptr, st);                       // r = b - A*x (:40)
+    hipLaunchKernelGGL(k_bicg_start<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, rtilde.ptr, S, part, cnt);
+    M4G_HIP(hipGetLastError());
+    const bool identity = !P || P->kind == M4G_PC_IDENTITY;
+    auto apply = [&](const T* in, T* out) {
+        if (identity) M4G_HIP(hipMemcpyAsync(out, in, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+        else pc_apply_skip(*P, in, out, vt, &S->done, st);
+    };
+    auto body = [&] {
+        hipLaunchKernelGGL(k_bicg_p<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)v.ptr, p.ptr,
+                           (const KState*)S);
+        apply(p.ptr, phat.ptr);                                      // phat = solve(M, p) (:55)
+        hipLaunchKernelGGL(k_bicg_spmv_gamma<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
+                           A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)phat.ptr, (const T*)rtilde.ptr,
+                           v.ptr, S, part, cnt);
+        hipLaunchKernelGGL(k_bicg_s<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)v.ptr, s_.ptr, S,
+                           part, cnt);
+        hipLaunchKernelGGL(k_bicg_exit<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)phat.ptr, x, S, cnt + 1);
+        apply(s_.ptr, shat.ptr);                                     // shat = solve(M, s) (:68)
+        hipLaunchKernelGGL(k_bicg_spmv_omega<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
+                           A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)shat.ptr, (const T*)s_.ptr, t_.ptr,
+                           S, part, cnt);
+        hipLaunchKernelGGL(k_bicg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)shat.ptr, (const T*)phat.ptr,
+                           (const T*)s_.ptr, (const T*)t_.ptr, (const T*)rtilde.ptr, x, r.ptr, S, part, cnt);
+        M4G_HIP(hipGetLastError());
+    };
+    run_batches(ws, it, body);
+    int fault = 0;
+    if (P && P->kind >= M4G_PC_IC0) fault = check_sweep_fault(P->lower) | check_sweep_fault(P->upper);
+    return finish_state(ws, it, fault);
+}
+
+int krylov_solve(int method, m4g_matrix_s& A, void* x, const void* b, int vtype, m4g_pc_s* P, m4g_iteration* it,
+                 KrylovWs& ws)
+{
+    if (vtype == M4G_F32 && !A.sell.vals32.ptr) sell_build_f32(A);
+    if (method == M4G_METHOD_CG)
+        return vtype == M4G_F64 ? cg_t<double>(A, x, b, P, it, ws) : cg_t<float>(A, x, b, P, it, ws);
+    return vtype == M4G_F64 ? bicgstab_t<double>(A, x, b, P, it, ws) : bicgstab_t<float>(A, x, b, P, it, ws);
+}
+
+KrylovWs& default_ws()
+{
+    static std::map<int, std::unique_ptr<KrylovWs>> m;
+    static std::mutex mu;
+    DeviceState& d = device_state();
+    std::lock_guard<std::mutex> lock(mu);
+    auto& p = m[d.device];
+    if (!p) p = std::make_unique<KrylovWs>();
+    return *p;
+}
+
+}  // namespace m4g
+
+using namespace m4g;
+
+extern "C" {
+
+static int solve_common(int method, m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* it)
+{
+    return guard([&] {
+        M4G_REQUIRE(A && x && b && it, M4G_ERR_INVALID, "solver: null argument");
+        M4G_REQUIRE(A->nrows == A->ncols, M4G_ERR_NOT_SQUARE, "Matrix is not square");
+        M4G_REQUIRE(x->n == A->ncols && b->n == A->nrows, M4G_ERR_SIZE, "solver: incompatible size");
+        M4G_REQUIRE(x->type == b->type, M4G_ERR_INVALID, "solver: x and b types differ");
+        M4G_REQUIRE(!P || P->n == A->nrows, M4G_ERR_SIZE, "solver: preconditioner size differs");
+        M4G_REQUIRE(it->max_iter >= 0, M4G_ERR_INVALID, "solver: negative max_iter");
+        if (it->history) M4G_REQUIRE(it->history_capacity > 0, M4G_ERR_INVALID, "solver: history capacity");
+        return krylov_solve(method, *A, x->ptr, b->ptr, x->type, P, it, default_ws());
+    });
+}
+
+int m4g_cg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* it)
+{
+    return solve_common(M4G_METHOD_CG, A, x, b, P, it);
+}
+
+int m4g_bicgstab(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* it)
+{
+    return solve_common(M4G_METHOD_BICGSTAB, A, x, b, P, it);
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Device-side helpers shared by the kernels (wave64 reductions, XCD mapping).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace m4g {
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;    // MI355X: workgroup i runs on XCD i % 8
+
+// Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.
+// The combination order is fixed, so results are deterministic.
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over a workgroup of NT threads; the result is valid in thread 0.
+// scratch must hold NT/64 values.
+template <int NT, typename T>
+__device__ inline T block_sum(T v, T* scratch)
+{
+    v = wave_sum(v);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (NT == kWave) return v;
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    T t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < NT / kWave; w++) t += scratch[w];
+    __syncthreads();
+    return t;
+}
+
+// Two-stage deterministic reduction epilogue: each block stores its partial;
+// the last block to arrive sums all partials in block order and returns true
+// in thread 0 with *total set.  counter is left at zero for the next launch.
+template <int NT, typename T>
+__device__ inline bool last_block_total(T partial, T* partials, unsigned* counter, T* total, T* scratch)
+{
+    __shared__ bool am_last;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = partial;
+        __threadfence();
+        unsigned ticket = atomicAdd(counter, 1u);
+        am_last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    T acc = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += NT) acc += __hip_atomic_load(partials + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = block_sum<NT>(acc, scratch);
+    if (threadIdx.x == 0) {
+        *total = acc;
+        *counter = 0;
+    }
+    return threadIdx.x == 0;
+}
+
+// Logical workgroup index with XCD-contiguous ranges: the hardware places
+// workgroup b on XCD b % 8; this mapping gives each XCD one contiguous chunk
+// of `chunk` logical groups, so neighbouring rows share an L2.
+__device__ inline int64_t xcd_logical_block(int64_t chunk)
+{
+    const int64_t b = blockIdx.x;
+    return (b % kXcds) * chunk + b / kXcds;
+}
+
+}  // namespace m4g

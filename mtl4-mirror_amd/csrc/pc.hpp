@@ -1,0 +1,58 @@
+// Preconditioner handle and compiled sweeps (device side).
+#pragma once
+
+#include "m4g_internal.hpp"
+#include "schedule.hpp"
+
+namespace m4g {
+
+// A compiled sweep on the device.
+struct SweepDev {
+    SweepPlan plan_meta;                  // scalars and tile/group offsets (large arrays dropped)
+    DevBuf<int32_t> tile_row_begin, rows, tile_group_begin, gr_slot, gr_len, gr_export, ent_src;
+    DevBuf<int64_t> group_row_begin, group_ent_begin, ent_val_pos, gr_diag_pos;
+    DevBuf<double> ent_val64, diag64;
+    DevBuf<float> ent_val32, diag32;
+    DevBuf<unsigned char> xbuf;           // exchange buffer (double or float entries)
+    int xbuf_type = -1;
+    DevBuf<unsigned> tickets;
+    DevBuf<int> fault;
+    int factor_type = M4G_F64;
+
+    void upload(const SweepPlan& p, const double* fstore_dev, int ftype);
+    void refresh_values(const double* fstore_dev, int ftype);
+    void ensure_exchange(int vtype);
+};
+
+void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp, void* y, int vtype,
+                    const int* skip, hipStream_t s);
+int check_sweep_fault(SweepDev& d);
+int64_t sweep_tile_cap(int vtype);
+SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap);
+
+}  // namespace m4g
+
+struct m4g_pc_s {
+    int kind = M4G_PC_IDENTITY;
+    int factor_type = M4G_F64;
+    int64_t n = 0;
+    m4g::GridInfo grid;
+    // factor storage in double: IC(0) -> upper(A) CSR; ILU(0) -> A's CSR
+    std::vector<int64_t> fS, fI;          // pattern of the storage
+    std::vector<int64_t> diag_pos;        // ILU(0): storage index of each diagonal
+    m4g::DevBuf<double> fstore;
+    m4g::DevBuf<double> inv_diag;         // diagonal PC
+    m4g::SweepDev lower, upper;           // solve: lower then upper
+    m4g::SweepDev adj_lower, adj_upper;   // ILU(0) adjoint_solve
+    bool adj_ready = false;
+    m4g::DevBuf<unsigned char> tmp;       // intermediate vector of a PC application
+    size_t tmp_bytes = 0;
+    int64_t factor_levels = 0;
+    ~m4g_pc_s();
+};
+
+namespace m4g {
+// y = P^{-1} x (or the adjoint) on stream s; vtype is the vectors' value type.
+void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipStream_t s,
+              const int* skip = nullptr);
+}  // namespace m4g

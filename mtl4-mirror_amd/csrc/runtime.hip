@@ -1,0 +1,493 @@
+// Library state, buffer pool, vectors and matrices of the C ABI.
+#include "m4g_internal.hpp"
+
+#include <algorithm>
+
+namespace m4g {
+
+static thread_local std::string tl_last_error;
+static thread_local int tl_device = 0;
+static thread_local hipStream_t tl_stream = nullptr;   // user stream (nullptr = library stream)
+
+void set_last_error(const std::string& msg) { tl_last_error = msg; }
+
+static std::mutex g_states_mutex;
+static std::map<int, std::unique_ptr<DeviceState>> g_states;
+
+DeviceState& device_state()
+{
+    std::lock_guard<std::mutex> lock(g_states_mutex);
+    auto it = g_states.find(tl_device);
+    if (it != g_states.end()) {
+        M4G_HIP(hipSetDevice(tl_device));
+        return *it->second;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw Error(M4G_ERR_NO_DEVICE, "no HIP device visible");
+    M4G_REQUIRE(tl_device >= 0 && tl_device < count, M4G_ERR_INVALID, "device index out of range");
+    M4G_HIP(hipSetDevice(tl_device));
+    auto st = std::make_unique<DeviceState>();
+    st->device = tl_device;
+    M4G_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    M4G_HIP(hipGetDeviceProperties(&prop, tl_device));
+    st->num_cus = prop.multiProcessorCount;
+    M4G_HIP(hipHostMalloc(reinterpret_cast<void**>(&st->pinned), 64 * sizeof(double), hipHostMallocDefault));
+    DeviceState& ref = *st;
+    g_states[tl_device] = std::move(st);
+    return ref;
+}
+
+hipStream_t current_stream()
+{
+    DeviceState& st = device_state();
+    return tl_stream ? tl_stream : st.stream;
+}
+
+void* pool_alloc(size_t bytes)
+{
+    DeviceState& st = device_state();
+    {
+        std::lock_guard<std::mutex> lock(st.pool_mutex);
+        auto it = st.pool.find(bytes);
+        if (it != st.pool.end()) {
+            void* p = it->second;
+            st.pool.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        // release cached buffers and retry once
+        {
+            std::lock_guard<std::mutex> lock(st.pool_mutex);
+            M4G_HIP(hipDeviceSynchronize());
+            for (auto& kv : st.pool) (void)hipFree(kv.second);
+            st.pool.clear();
+        }
+        M4G_HIP(hipMalloc(&p, bytes));
+    }
+    return p;
+}
+
+void pool_free(void* p, size_t bytes)
+{
+    if (!p) return;
+    DeviceState& st = device_state();
+    std::lock_guard<std::mutex> lock(st.pool_mutex);
+    st.pool.emplace(bytes, p);
+}
+
+double read_scalar(const double* d)
+{
+    DeviceState& st = device_state();
+    hipStream_t s = current_stream();
+    M4G_HIP(hipMemcpyAsync(st.pinned, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    M4G_HIP(hipStreamSynchronize(s));
+    return st.pinned[0];
+}
+
+ReduceWs& reduce_ws()
+{
+    static std::map<int, ReduceWs> ws;
+    static std::mutex mu;
+    DeviceState& st = device_state();
+    std::lock_guard<std::mutex> lock(mu);
+    ReduceWs& w = ws[st.device];
+    if (!w.partials) {
+        M4G_HIP(hipMalloc(&w.partials, sizeof(double) * (kReduceBlocks * 16 + 64)));
+        M4G_HIP(hipMalloc(&w.counters, sizeof(unsigned) * kCounters));
+        M4G_HIP(hipMemsetAsync(w.counters, 0, sizeof(unsigned) * kCounters, current_stream()));
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+    }
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// host-side CSR ingest
+// ---------------------------------------------------------------------------
+template <typename T>
+static void widen(const void* src, int64_t count, std::vector<int64_t>& dst)
+{
+    const T* s = static_cast<const T*>(src);
+    dst.resize(count);
+    for (int64_t i = 0; i < count; i++) dst[i] = static_cast<int64_t>(s[i]);
+}
+
+static void widen_index(const void* src, int64_t count, int type, std::vector<int64_t>& dst)
+{
+    switch (type) {
+    case M4G_IDX_I32: widen<int32_t>(src, count, dst); break;
+    case M4G_IDX_I64: widen<int64_t>(src, count, dst); break;
+    case M4G_IDX_U32: widen<uint32_t>(src, count, dst); break;
+    default: throw Error(M4G_ERR_INVALID, "unknown index type");
+    }
+}
+
+}  // namespace m4g
+
+using namespace m4g;
+
+extern "C" {
+
+const char* m4g_last_error(void) { return tl_last_error.c_str(); }
+
+const char* m4g_version(void)
+{
+    return "mtl4gpu 0.1 gfx950 (fp-contract=off, packed-fp32-ops disabled)";
+}
+
+int m4g_set_device(int device)
+{
+    return guard([&] {
+        tl_device = device;
+        device_state();
+        return M4G_OK;
+    });
+}
+
+int m4g_set_stream(void* s)
+{
+    return guard([&] {
+        tl_stream = static_cast<hipStream_t>(s);
+        return M4G_OK;
+    });
+}
+
+void* m4g_get_stream(void)
+{
+    try {
+        return current_stream();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+int m4g_synchronize(void)
+{
+    return guard([&] {
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+        return M4G_OK;
+    });
+}
+
+// ------------------------------- vectors -----------------------------------
+int m4g_vector_create(m4g_vector* v, int64_t n, int type)
+{
+    return guard([&] {
+        M4G_REQUIRE(v && n >= 0 && (type == M4G_F64 || type == M4G_F32), M4G_ERR_INVALID,
+                    "m4g_vector_create: bad argument");
+        auto h = std::make_unique<m4g_vector_s>();
+        h->n = n;
+        h->type = type;
+        h->owned = true;
+        h->ptr = n ? pool_alloc(h->bytes()) : nullptr;
+        *v = h.release();
+        return M4G_OK;
+    });
+}
+
+int m4g_vector_wrap(m4g_vector* v, void* dptr, int64_t n, int type)
+{
+    return guard([&] {
+        M4G_REQUIRE(v && n >= 0 && (dptr || n == 0) && (type == M4G_F64 || type == M4G_F32),
+                    M4G_ERR_INVALID, "m4g_vector_wrap: bad argument");
+        auto h = std::make_unique<m4g_vector_s>();
+        h->n = n;
+        h->type = type;
+        h->owned = false;
+        h->ptr = dptr;
+        *v = h.release();
+        return M4G_OK;
+    });
+}
+
+int m4g_vector_destroy(m4g_vector v)
+{
+    return guard([&] {
+        if (!v) return M4G_OK;
+        if (v->owned && v->ptr) {
+            // queued work may still use the buffer: pool reuse is stream ordered
+            pool_free(v->ptr, v->bytes());
+        }
+        delete v;
+        return M4G_OK;
+    });
+}
+
+int m4g_vector_size(m4g_vector v, int64_t* n)
+{
+    return guard([&] {
+        M4G_REQUIRE(v && n, M4G_ERR_INVALID, "m4g_vector_size: null argument");
+        *n = v->n;
+        return M4G_OK;
+    });
+}
+
+void* m4g_vector_device_ptr(m4g_vector v) { return v ? v->ptr : nullptr; }
+
+int m4g_vector_upload(m4g_vector v, const void* host)
+{
+    return guard([&] {
+        M4G_REQUIRE(v && (host || v->n == 0), M4G_ERR_INVALID, "m4g_vector_upload: null argument");
+        if (v->n)
+            M4G_HIP(hipMemcpyAsync(v->ptr, host, v->bytes(), hipMemcpyHostToDevice, current_stream()));
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+        return M4G_OK;
+    });
+}
+
+int m4g_vector_download(m4g_vector v, void* host)
+{
+    return guard([&] {
+        M4G_REQUIRE(v && (host || v->n == 0), M4G_ERR_INVALID, "m4g_vector_download: null argument");
+        if (v->n)
+            M4G_HIP(hipMemcpyAsync(host, v->ptr, v->bytes(), hipMemcpyDeviceToHost, current_stream()));
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+        return M4G_OK;
+    });
+}
+
+int m4g_vector_fill(m4g_vector v, double value)
+{
+    return guard([&] {
+        M4G_REQUIRE(v, M4G_ERR_INVALID, "m4g_vector_fill: null vector");
+        launch_fill(v->n, value, v->ptr, v->type, current_stream());
+        return M4G_OK;
+    });
+}
+
+int m4g_vector_copy(m4g_vector dst, m4g_vector src)
+{
+    return guard([&] {
+        M4G_REQUIRE(dst && src, M4G_ERR_INVALID, "m4g_vector_copy: null vector");
+        M4G_REQUIRE(dst->n == src->n, M4G_ERR_SIZE, "m4g_vector_copy: incompatible size");
+        if (dst->type == src->type) {
+            if (dst->n)
+                M4G_HIP(hipMemcpyAsync(dst->ptr, src->ptr, src->bytes(), hipMemcpyDeviceToDevice,
+                                       current_stream()));
+        } else {
+            launch_convert(src->n, src->ptr, src->type, dst->ptr, dst->type, current_stream());
+        }
+        return M4G_OK;
+    });
+}
+
+// ------------------------------- matrices ----------------------------------
+int m4g_matrix_create(m4g_matrix* out, int64_t nrows, int64_t ncols, int64_t nnz, const void* starts,
+                      const void* indices, const void* values, int index_type, int value_type)
+{
+    return guard([&] {
+        M4G_REQUIRE(out && nrows >= 0 && ncols >= 0 && nnz >= 0 && starts && (nnz == 0 || (indices && values)),
+                    M4G_ERR_INVALID, "m4g_matrix_create: bad argument");
+        M4G_REQUIRE(value_type == M4G_F64 || value_type == M4G_F32, M4G_ERR_INVALID,
+                    "m4g_matrix_create: unknown value type");
+        M4G_REQUIRE(nnz < (int64_t(1) << 31) && nrows < (int64_t(1) << 31) && ncols < (int64_t(1) << 31),
+                    M4G_ERR_INVALID, "m4g_matrix_create: dimensions exceed int32 device indices");
+        auto A = std::make_unique<m4g_matrix_s>();
+        A->nrows = nrows;
+        A->ncols = ncols;
+        A->nnz = nnz;
+        A->input_value_type = value_type;
+        widen_index(starts, nrows + 1, index_type, A->h_starts);
+        widen_index(indices, nnz, index_type, A->h_indices);
+        M4G_REQUIRE(A->h_starts[0] == 0 && A->h_starts[nrows] == nnz, M4G_ERR_INVALID,
+                    "m4g_matrix_create: starts[0] must be 0 and starts[nrows] == nnz");
+        for (int64_t r = 0; r < nrows; r++) {
+            int64_t b = A->h_starts[r], e = A->h_starts[r + 1];
+            M4G_REQUIRE(b <= e, M4G_ERR_INVALID, "m4g_matrix_create: starts not monotone");
+            for (int64_t j = b; j < e; j++) {
+                int64_t c = A->h_indices[j];
+                M4G_REQUIRE(c >= 0 && c < ncols, M4G_ERR_INVALID, "m4g_matrix_create: column out of range");
+                M4G_REQUIRE(j == b || A->h_indices[j - 1] < c, M4G_ERR_LOGIC,
+                            "Matrix entries must be sorted for this.");
+            }
+        }
+        std::vector<double> vals(nnz);
+        if (value_type == M4G_F64) std::memcpy(vals.data(), values, nnz * sizeof(double));
+        else
+            for (int64_t j = 0; j < nnz; j++) vals[j] = static_cast<const float*>(values)[j];
+        std::vector<int32_t> s32(A->h_starts.begin(), A->h_starts.end());
+        std::vector<int32_t> i32(A->h_indices.begin(), A->h_indices.end());
+        A->starts.reset(nrows + 1);
+        A->indices.reset(nnz);
+        A->values.reset(nnz);
+        A->starts.upload(s32.data(), nrows + 1);
+        A->indices.upload(i32.data(), nnz);
+        A->values.upload(vals.data(), nnz);
+        sell_build(*A);
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+        *out = A.release();
+        return M4G_OK;
+    });
+}
+
+int m4g_matrix_destroy(m4g_matrix A)
+{
+    return guard([&] {
+        delete A;
+        return M4G_OK;
+    });
+}
+
+int m4g_matrix_dims(m4g_matrix A, int64_t* nrows, int64_t* ncols, int64_t* nnz)
+{
+    return guard([&] {
+        M4G_REQUIRE(A, M4G_ERR_INVALID, "m4g_matrix_dims: null matrix");
+        if (nrows) *nrows = A->nrows;
+        if (ncols) *ncols = A->ncols;
+        if (nnz) *nnz = A->nnz;
+        return M4G_OK;
+    });
+}
+
+int m4g_matrix_download(m4g_matrix A, int64_t* starts, int64_t* indices, double* values)
+{
+    return guard([&] {
+        M4G_REQUIRE(A, M4G_ERR_INVALID, "m4g_matrix_download: null matrix");
+        if (starts) std::copy(A->h_starts.begin(), A->h_starts.end(), starts);
+        if (indices) std::copy(A->h_indices.begin(), A->h_indices.end(), indices);
+        if (values) A->values.download(values, A->nnz);
+        return M4G_OK;
+    });
+}
+
+int m4g_matrix_set_values(m4g_matrix A, const double* values, int values_device)
+{
+    return guard([&] {
+        M4G_REQUIRE(A && (values || A->nnz == 0), M4G_ERR_INVALID, "m4g_matrix_set_values: null argument");
+        if (A->nnz)
+            M4G_HIP(hipMemcpyAsync(A->values.ptr, values, A->nnz * sizeof(double),
+                                   values_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                   current_stream()));
+        sell_refresh_values(*A);
+        if (!values_device) M4G_HIP(hipStreamSynchronize(current_stream()));
+        return M4G_OK;
+    });
+}
+
+// ------------------------------- level 1 -----------------------------------
+static void check_f64(m4g_vector v, const char* who)
+{
+    M4G_REQUIRE(v, M4G_ERR_INVALID, std::string(who) + ": null vector");
+    M4G_REQUIRE(v->type == M4G_F64, M4G_ERR_INVALID, std::string(who) + ": vector must be double");
+}
+
+int m4g_dot(m4g_vector x, m4g_vector y, double* result)
+{
+    return guard([&] {
+        check_f64(x, "m4g_dot");
+        check_f64(y, "m4g_dot");
+        M4G_REQUIRE(x->n == y->n, M4G_ERR_SIZE, "m4g_dot: incompatible size");
+        ReduceWs& ws = reduce_ws();
+        double* out = ws.partials + kReduceBlocks * 16;
+        launch_dot(x->n, (const double*)x->ptr, (const double*)y->ptr, out, current_stream());
+        *result = read_scalar(out);
+        return M4G_OK;
+    });
+}
+
+int m4g_unary_dot(m4g_vector x, double* result)
+{
+    return guard([&] {
+        check_f64(x, "m4g_unary_dot");
+        ReduceWs& ws = reduce_ws();
+        double* out = ws.partials + kReduceBlocks * 16;
+        launch_sumsq(x->n, (const double*)x->ptr, out, current_stream());
+        *result = read_scalar(out);
+        return M4G_OK;
+    });
+}
+
+int m4g_two_norm(m4g_vector x, double* result)
+{
+    int st = m4g_unary_dot(x, result);
+    if (st == M4G_OK) *result = std::sqrt(*result);
+    return st;
+}
+
+int m4g_axpy(double alpha, m4g_vector x, m4g_vector y)
+{
+    return guard([&] {
+        check_f64(x, "m4g_axpy");
+        check_f64(y, "m4g_axpy");
+        M4G_REQUIRE(x->n == y->n, M4G_ERR_SIZE, "m4g_axpy: incompatible size");
+        launch_axpy(x->n, alpha, (const double*)x->ptr, (double*)y->ptr, current_stream());
+        return M4G_OK;
+    });
+}
+
+int m4g_xpay(m4g_vector x, double beta, m4g_vector y)
+{
+    return guard([&] {
+        check_f64(x, "m4g_xpay");
+        check_f64(y, "m4g_xpay");
+        M4G_REQUIRE(x->n == y->n, M4G_ERR_SIZE, "m4g_xpay: incompatible size");
+        launch_xpay(x->n, (const double*)x->ptr, beta, (double*)y->ptr, current_stream());
+        return M4G_OK;
+    });
+}
+
+int m4g_axpy_unary_dot(double alpha, m4g_vector q, m4g_vector r, double* rr)
+{
+    return guard([&] {
+        check_f64(q, "m4g_axpy_unary_dot");
+        check_f64(r, "m4g_axpy_unary_dot");
+        M4G_REQUIRE(q->n == r->n, M4G_ERR_SIZE, "m4g_axpy_unary_dot: incompatible size");
+        ReduceWs& ws = reduce_ws();
+        double* out = ws.partials + kReduceBlocks * 16;
+        launch_axpy_sumsq(q->n, alpha, (const double*)q->ptr, (double*)r->ptr, out, current_stream());
+        *rr = read_scalar(out);
+        return M4G_OK;
+    });
+}
+
+// ------------------------------- SpMV ---------------------------------------
+static void check_spmv(m4g_matrix A, m4g_vector v, m4g_vector w, const char* who)
+{
+    M4G_REQUIRE(A, M4G_ERR_INVALID, std::string(who) + ": null matrix");
+    check_f64(v, who);
+    check_f64(w, who);
+    M4G_REQUIRE(v->n == A->ncols && w->n == A->nrows, M4G_ERR_SIZE, std::string(who) + ": incompatible size");
+    M4G_REQUIRE(v->ptr != w->ptr || A->nrows == 0, M4G_ERR_INVALID, std::string(who) + ": v and w alias");
+}
+
+int m4g_spmv(m4g_matrix A, m4g_vector v, m4g_vector w, int mode)
+{
+    return guard([&] {
+        check_spmv(A, v, w, "m4g_spmv");
+        M4G_REQUIRE(mode == M4G_ASSIGN || mode == M4G_PLUS || mode == M4G_MINUS, M4G_ERR_INVALID,
+                    "m4g_spmv: unknown mode");
+        launch_spmv(*A, (const double*)v->ptr, (double*)w->ptr, mode, nullptr, current_stream());
+        return M4G_OK;
+    });
+}
+
+int m4g_residual(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_vector r)
+{
+    return guard([&] {
+        check_spmv(A, x, r, "m4g_residual");
+        check_f64(b, "m4g_residual");
+        M4G_REQUIRE(b->n == A->nrows, M4G_ERR_SIZE, "m4g_residual: incompatible size");
+        launch_spmv(*A, (const double*)x->ptr, (double*)r->ptr, M4G_ASSIGN, (const double*)b->ptr,
+                    current_stream());
+        return M4G_OK;
+    });
+}
+
+int m4g_spmv_dot(m4g_matrix A, m4g_vector p, m4g_vector q, double* pq)
+{
+    return guard([&] {
+        check_spmv(A, p, q, "m4g_spmv_dot");
+        M4G_REQUIRE(A->nrows == A->ncols, M4G_ERR_NOT_SQUARE, "m4g_spmv_dot: matrix not square");
+        ReduceWs& ws = reduce_ws();
+        double* out = ws.partials + kReduceBlocks * 16;
+        launch_spmv_dot(*A, (const double*)p->ptr, (double*)q->ptr, out, current_stream());
+        *pq = read_scalar(out);
+        return M4G_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,648 @@
+// Host-side compilation of triangular sweeps into tile-pipelined schedules.
+//
+// A sweep (forward on L or backward on U) is split into tiles of rows.  Each
+// tile is executed by one workgroup that keeps the tile's vector entries in
+// LDS and walks the tile's local dependency levels, one LDS barrier per level.
+// Values needed by later tiles are exported through a device exchange buffer.
+// Tiles are numbered in a topological order of the tile graph and handed to
+// workgroups by an atomic ticket, so every wait of a tile is on a tile with a
+// smaller ticket, which already runs or has finished: the pipeline cannot
+// deadlock whatever the residency.
+//
+// For matrices on a natural-order grid (inferred from the pattern) the tiles
+// are boxes in skewed coordinates t1 = k, t2 = j + a*k, t3 = i + b*j + c*k in
+// which every dependency is monotone; otherwise tiles are blocks of
+// consecutive rows.  Neither choice changes any arithmetic: the operation
+// order per row is the reference's (lower_trisolve.hpp:191-209 / 172-189,
+// upper_trisolve.hpp:159-176).
+#include "schedule.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <limits>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+
+namespace m4g {
+
+// ---------------------------------------------------------------------------
+// grid inference
+// ---------------------------------------------------------------------------
+// Number of distinct neighbour offsets (di, dj, dk) of the pattern on the
+// grid, or -1 when some entry couples cells further apart than one step.
+static int64_t grid_score(int64_t n, const int64_t* S, const int64_t* I, int64_t nx, int64_t ny, int64_t stride)
+{
+    const int64_t nxy = nx * ny;
+    std::set<int64_t> seen;
+    for (int64_t r = 0; r < n; r += stride) {
+        int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
+        for (int64_t p = S[r]; p < S[r + 1]; p++) {
+            int64_t c = I[p];
+            int64_t di = c % nx - ri, dj = (c / nx) % ny - rj, dk = c / nxy - rk;
+            if (std::llabs(di) > 1 || std::llabs(dj) > 1 || std::llabs(dk) > 1) return -1;
+            seen.insert((di + 1) + 3 * (dj + 1) + 9 * (dk + 1));
+        }
+    }
+    return int64_t(seen.size());
+}
+
+GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
+{
+    GridInfo best;
+    if (n < 4) return best;
+    std::set<int64_t> offs;
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t p = S[r]; p < S[r + 1]; p++)
+            if (I[p] < r) {
+                offs.insert(r - I[p]);
+                if (offs.size() > 64) return best;    // not a stencil pattern
+            }
+    if (offs.empty()) return best;
+    std::set<int64_t> nxs;
+    for (int64_t d : offs)
+        for (int64_t e = -1; e <= 1; e++) {
+            int64_t nx = d + e;
+            if (nx >= 2 && nx <= n && n % nx == 0) nxs.insert(nx);
+        }
+    const int64_t sample = std::max<int64_t>(1, n / 4096);
+    int64_t best_score = -1;
+    for (int64_t nx : nxs) {
+        std::set<int64_t> nxys;
+        for (int64_t d : offs) {
+            if (d < nx) continue;
+            for (int64_t e = -nx - 1; e <= nx + 1; e++) {
+                int64_t v = d + e;
+                if (v > nx && v < n && v % nx == 0 && n % v == 0) nxys.insert(v);
+            }
+        }
+        nxys.insert(n);   // 2D (or 1D when nx == n)
+        for (int64_t nxy : nxys) {
+            int64_t ny = nxy / nx, nz = n / nxy;
+            int64_t sc = grid_score(n, S, I, nx, ny, sample);
+            if (sc < 0) continue;
+            // fewest distinct offsets wins (the tightest stencil); ties keep the larger nx
+            if (best_score < 0 || sc < best_score || (sc == best_score && nx >= best.nx)) {
+                if (grid_score(n, S, I, nx, ny, 1) < 0) continue;
+                best_score = sc;
+                best.nx = nx;
+                best.ny = ny;
+                best.nz = nz;
+            }
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// levels
+// ---------------------------------------------------------------------------
+std::vector<int32_t> natural_levels(const SweepRows& s, int64_t* nlevels)
+{
+    std::vector<int32_t> lvl(s.n, 0);
+    int32_t maxl = -1;
+    for (int64_t q = 0; q < s.n; q++) {
+        int64_t r = s.lower ? q : s.n - 1 - q;
+        int32_t l = 0;
+        for (int64_t p = s.ptr[r]; p < s.ptr[r + 1]; p++) l = std::max(l, lvl[s.col[p]] + 1);
+        lvl[r] = l;
+        maxl = std::max(maxl, l);
+    }
+    if (nlevels) *nlevels = maxl + 1;
+    return lvl;
+}
+
+// ---------------------------------------------------------------------------
+// tile coordinates
+// ---------------------------------------------------------------------------
+struct Coef {
+    bool ok = false;
+    int64_t a = 0, b = 0, c = 0;
+};
+
+static Coef skew_coefficients(const SweepRows& s, const GridInfo& g)
+{
+    Coef k;
+    if (!g.nx) return k;
+    const int64_t nx = g.nx, ny = g.ny, nxy = g.nx * g.ny;
+    int64_t a = 0, b = 0;
+    std::vector<std::array<int, 3>> deltas;
+    std::set<std::array<int, 3>> seen;
+    for (int64_t r = 0; r < s.n; r++) {
+        int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
+        for (int64_t p = s.ptr[r]; p < s.ptr[r + 1]; p++) {
+            int64_t c = s.col[p];
+            int di = int(c % nx - ri), dj = int((c / nx) % ny - rj), dk = int(c / nxy - rk);
+            if (!s.lower) { di = -di; dj = -dj; dk = -dk; }
+            std::array<int, 3> d = {di, dj, dk};
+            if (seen.insert(d).second) deltas.push_back(d);
+        }
+    }
+    for (auto& d : deltas) {
+        if (d[2] > 0) return k;
+        if (d[2] == 0 && d[1] > 0) return k;
+        if (d[2] == 0 && d[1] == 0 && d[0] >= 0) return k;
+        if (d[2] < 0) a = std::max<int64_t>(a, d[1]);
+        if (d[2] == 0 && d[1] < 0) b = std::max<int64_t>(b, d[0]);
+    }
+    int64_t c = 0;
+    for (auto& d : deltas)
+        if (d[2] < 0) c = std::max<int64_t>(c, d[0] + b * d[1]);
+    k.ok = true;
+    k.a = a;
+    k.b = b;
+    k.c = c;
+    return k;
+}
+
+struct TileCoords {
+    int64_t n1 = 1, n2 = 1, n3 = 1;   // tile counts per coordinate
+};
+
+static inline void grid_coords(const GridInfo& g, int64_t r, int64_t& i, int64_t& j, int64_t& k)
+{
+    i = r % g.nx;
+    j = (r / g.nx) % g.ny;
+    k = r / (g.nx * g.ny);
+}
+
+TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, double hop_us, double step_us)
+{
+    TileShape best;
+    Coef cf = skew_coefficients(s, g);
+    if (!cf.ok) {
+        best.grid = false;
+        best.rows_per_block = std::min<int64_t>(cap, std::max<int64_t>(s.n, 1));
+        return best;
+    }
+    // level function of the natural DAG is linear on a grid: measure it
+    int64_t nlev = 0;
+    std::vector<int32_t> lvl = natural_levels(s, &nlev);
+    auto at = [&](int64_t i, int64_t j, int64_t k) -> double {
+        int64_t r = i + g.nx * (j + g.ny * k);
+        return r < s.n ? double(lvl[r]) : 0.0;
+    };
+    double base = at(0, 0, 0);
+    double al = g.nx > 1 ? std::fabs(at(1, 0, 0) - base) : 0;
+    double be = g.ny > 1 ? std::fabs(at(0, 1, 0) - base) : 0;
+    double ga = g.nz > 1 ? std::fabs(at(0, 0, 1) - base) : 0;
+    if (!s.lower) {   // levels count from the last row
+        al = g.nx > 1 ? std::fabs(at(g.nx - 2, g.ny - 1, g.nz - 1) - at(g.nx - 1, g.ny - 1, g.nz - 1)) : 0;
+        be = g.ny > 1 ? std::fabs(at(g.nx - 1, g.ny - 2, g.nz - 1) - at(g.nx - 1, g.ny - 1, g.nz - 1)) : 0;
+        ga = g.nz > 1 ? std::fabs(at(g.nx - 1, g.ny - 1, g.nz - 2) - at(g.nx - 1, g.ny - 1, g.nz - 1)) : 0;
+    }
+    // level = al*t3 + (be - al*b)*t2 + (ga - al*c - a*(be - al*b))*t1
+    double w3 = al, w2 = std::fabs(be - al * cf.b), w1 = std::fabs(ga - al * cf.c - cf.a * (be - al * cf.b));
+    int64_t ext1 = g.nz, ext2 = (g.ny - 1) + cf.a * (g.nz - 1) + 1,
+            ext3 = (g.nx - 1) + cf.b * (g.ny - 1) + cf.c * (g.nz - 1) + 1;
+    std::vector<int64_t> c1, c2, c3;
+    for (int64_t e = 1; e < ext1; e *= 2) c1.push_back(e);
+    c1.push_back(ext1);
+    for (int64_t e = 1; e < ext2; e *= 2) c2.push_back(e);
+    c2.push_back(ext2);
+    for (int64_t e = 16; e < ext3; e *= 2) c3.push_back(e);
+    c3.push_back(ext3);
+    double best_cost = std::numeric_limits<double>::infinity();
+    for (int64_t e1 : c1)
+        for (int64_t e2 : c2)
+            for (int64_t e3 : c3) {
+                // rows in a tile: at most e1 planes x e2 lines x min(e3, nx) cells
+                int64_t rows = e1 * std::min<int64_t>(e2, g.ny) * std::min<int64_t>(e3, g.nx);
+                if (rows > cap) continue;
+                int64_t n1 = (ext1 + e1 - 1) / e1, n2 = (ext2 + e2 - 1) / e2, n3 = (ext3 + e3 - 1) / e3;
+                double hops = double(n1 - 1 + n2 - 1 + n3 - 1);
+                double span = w3 * double(std::min(e3, ext3) - 1) + w2 * double(std::min(e2, ext2) - 1) +
+                              w1 * double(std::min(e1, ext1) - 1) + 1.0;
+                span = std::min(span, double(nlev));
+                double per_level = double(rows) / std::max(span, 1.0);
+                double passes = std::max(1.0, std::ceil(per_level / 256.0));
+                double cost = hops * hop_us + span * passes * step_us;
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best.grid = true;
+                    best.a = cf.a;
+                    best.b = cf.b;
+                    best.c = cf.c;
+                    best.e1 = e1;
+                    best.e2 = e2;
+                    best.e3 = e3;
+                }
+            }
+    if (!best.grid) {
+        best.rows_per_block = std::min<int64_t>(cap, std::max<int64_t>(s.n, 1));
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// plan construction
+// ---------------------------------------------------------------------------
+SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t cap)
+{
+    SweepPlan p;
+    const int64_t n = s.n;
+    p.n = n;
+    p.lower = s.lower;
+    p.unit = true;
+    for (int64_t r = 0; r < n; r++)
+        if (s.diag_pos[r] >= 0) { p.unit = false; break; }
+    p.shape = shape;
+    natural_levels(s, &p.levels);
+
+    // 1. tile key of every row, in topological order
+    std::vector<int64_t> key(n);
+    if (shape.grid) {
+        for (int64_t r = 0; r < n; r++) {
+            int64_t i, j, k;
+            grid_coords(g, r, i, j, k);
+            int64_t t1 = k, t2 = j + shape.a * k, t3 = i + shape.b * j + shape.c * k;
+            int64_t T1 = t1 / shape.e1, T2 = t2 / shape.e2, T3 = t3 / shape.e3;
+            int64_t N2 = ((g.ny - 1) + shape.a * (g.nz - 1)) / shape.e2 + 1;
+            int64_t N3 = ((g.nx - 1) + shape.b * (g.ny - 1) + shape.c * (g.nz - 1)) / shape.e3 + 1;
+            int64_t lin = (T1 * N2 + T2) * N3 + T3;
+            key[r] = s.lower ? lin : -lin;
+        }
+    } else {
+        int64_t rb = std::max<int64_t>(1, shape.rows_per_block);
+        for (int64_t r = 0; r < n; r++) key[r] = s.lower ? r / rb : -(r / rb);
+    }
+    // compact keys to ticket order
+    std::vector<int64_t> keys(key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    p.ntiles = int64_t(keys.size());
+    std::vector<int32_t> tile(n);
+    for (int64_t r = 0; r < n; r++)
+        tile[r] = int32_t(std::lower_bound(keys.begin(), keys.end(), key[r]) - keys.begin());
+
+    // 2. rows of each tile in natural order -> LDS slots
+    p.tile_row_begin.assign(p.ntiles + 1, 0);
+    for (int64_t r = 0; r < n; r++) p.tile_row_begin[tile[r] + 1]++;
+    for (int64_t t = 0; t < p.ntiles; t++) p.tile_row_begin[t + 1] += p.tile_row_begin[t];
+    p.rows.assign(n, 0);
+    std::vector<int32_t> slot(n);
+    {
+        std::vector<int32_t> fill(p.tile_row_begin.begin(), p.tile_row_begin.end() - 1);
+        for (int64_t r = 0; r < n; r++) {
+            int32_t t = tile[r];
+            slot[r] = fill[t] - p.tile_row_begin[t];
+            p.rows[fill[t]++] = int32_t(r);
+        }
+    }
+    for (int64_t t = 0; t < p.ntiles; t++)
+        p.max_tile_rows = std::max<int64_t>(p.max_tile_rows, p.tile_row_begin[t + 1] - p.tile_row_begin[t]);
+    if (p.max_tile_rows > cap) throw std::runtime_error("sweep tile exceeds LDS capacity");
+
+    // 3. local levels and exports
+    std::vector<int32_t> llev(n, 0);
+    std::vector<char> exported(n, 0);
+    for (int64_t q = 0; q < n; q++) {
+        int64_t r = s.lower ? q : n - 1 - q;
+        int32_t l = 0;
+        for (int64_t e = s.ptr[r]; e < s.ptr[r + 1]; e++) {
+            int64_t c = s.col[e];
+            if (tile[c] == tile[r]) l = std::max(l, llev[c] + 1);
+            else exported[c] = 1;
+        }
+        llev[r] = l;
+    }
+    // export slots: contiguous per tile, in slot order
+    std::vector<int32_t> xslot(n, -1);
+    p.tile_export_begin.assign(p.ntiles + 1, 0);
+    {
+        int32_t next = 0;
+        for (int64_t t = 0; t < p.ntiles; t++) {
+            p.tile_export_begin[t] = next;
+            for (int32_t q = p.tile_row_begin[t]; q < p.tile_row_begin[t + 1]; q++) {
+                int32_t r = p.rows[q];
+                if (exported[r]) xslot[r] = next++;
+            }
+        }
+        p.tile_export_begin[p.ntiles] = next;
+        p.nexport = next;
+    }
+
+    // 4. level groups per tile: rows ordered by (tile, local level, processing order)
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        if (tile[x] != tile[y]) return tile[x] < tile[y];
+        if (llev[x] != llev[y]) return llev[x] < llev[y];
+        return s.lower ? x < y : x > y;
+    });
+    p.tile_group_begin.assign(p.ntiles + 1, 0);
+    p.group_row_begin.clear();
+    p.group_width.clear();
+    p.gr_slot.resize(n);
+    p.gr_len.resize(n);
+    p.gr_export.resize(n);
+    p.gr_diag_pos.resize(n);
+    int64_t maxgroup = 1;
+    for (int64_t q = 0; q < n;) {
+        int32_t r0 = order[q];
+        int64_t q1 = q;
+        int32_t width = 0;
+        while (q1 < n && tile[order[q1]] == tile[r0] && llev[order[q1]] == llev[r0]) {
+            int32_t r = order[q1];
+            width = std::max<int32_t>(width, int32_t(s.ptr[r + 1] - s.ptr[r]));
+            q1++;
+        }
+        p.group_row_begin.push_back(q);
+        p.group_width.push_back(width);
+        maxgroup = std::max<int64_t>(maxgroup, q1 - q);
+        q = q1;
+    }
+    p.ngroups = int64_t(p.group_width.size());
+    p.group_row_begin.push_back(n);
+    // tiles are contiguous in order[]; fix group ranges per tile
+    {
+        std::vector<int32_t> tg(p.ntiles + 1, int32_t(p.ngroups));
+        for (int64_t gi = p.ngroups - 1; gi >= 0; gi--) tg[tile[order[p.group_row_begin[gi]]]] = int32_t(gi);
+        for (int64_t t = p.ntiles - 1; t >= 0; t--)
+            if (p.tile_row_begin[t + 1] == p.tile_row_begin[t]) tg[t] = tg[t + 1];
+        p.tile_group_begin.assign(tg.begin(), tg.end());
+        p.tile_group_begin[p.ntiles] = int32_t(p.ngroups);
+    }
+    for (int64_t t = 0; t < p.ntiles; t++)
+        p.max_steps = std::max<int64_t>(p.max_steps, p.tile_group_begin[t + 1] - p.tile_group_begin[t]);
+
+    // 5. entries (ELL inside each group)
+    p.group_ent_begin.assign(p.ngroups + 1, 0);
+    for (int64_t gi = 0; gi < p.ngroups; gi++) {
+        int64_t m = p.group_row_begin[gi + 1] - p.group_row_begin[gi];
+        p.group_ent_begin[gi + 1] = p.group_ent_begin[gi] + m * p.group_width[gi];
+    }
+    p.nent = p.group_ent_begin[p.ngroups];
+    p.ent_src.assign(p.nent, 0);
+    p.ent_val_pos.assign(p.nent, -1);
+    for (int64_t gi = 0; gi < p.ngroups; gi++) {
+        int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
+        for (int64_t q = 0; q < m; q++) {
+            int32_t r = order[b + q];
+            p.gr_slot[b + q] = slot[r];
+            p.gr_len[b + q] = int32_t(s.ptr[r + 1] - s.ptr[r]);
+            p.gr_export[b + q] = xslot[r];
+            p.gr_diag_pos[b + q] = s.diag_pos[r];
+            for (int64_t k = 0; k < s.ptr[r + 1] - s.ptr[r]; k++) {
+                int64_t e = s.ptr[r] + k, c = s.col[e];
+                int64_t d = p.group_ent_begin[gi] + k * m + q;
+                p.ent_src[d] = tile[c] == tile[r] ? slot[c] : -(xslot[c] + 1);
+                p.ent_val_pos[d] = s.val_pos[e];
+            }
+        }
+    }
+    int64_t blk = ((std::min<int64_t>(maxgroup, 1024) + 63) / 64) * 64;
+    p.block = int(std::max<int64_t>(64, std::min<int64_t>(blk, 256)));
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// validation and CPU emulation
+// ---------------------------------------------------------------------------
+std::string validate_sweep_plan(const SweepPlan& p, const SweepRows& s)
+{
+    const int64_t n = p.n;
+    if (int64_t(p.rows.size()) != n) return "rows size";
+    std::vector<int32_t> tile_of(n, -1), slot_of(n, -1), grp_of(n, -1), x_of(n, -1);
+    for (int64_t t = 0; t < p.ntiles; t++) {
+        if (p.tile_row_begin[t + 1] - p.tile_row_begin[t] > p.max_tile_rows) return "tile rows";
+        for (int32_t q = p.tile_row_begin[t]; q < p.tile_row_begin[t + 1]; q++) {
+            int32_t r = p.rows[q];
+            if (r < 0 || r >= n || tile_of[r] != -1) return "row listed twice or out of range";
+            tile_of[r] = int32_t(t);
+            slot_of[r] = q - p.tile_row_begin[t];
+        }
+    }
+    for (int64_t r = 0; r < n; r++)
+        if (tile_of[r] < 0) return "row missing";
+    std::vector<int64_t> group_rank(n, -1);
+    int64_t seen_rows = 0;
+    for (int64_t t = 0; t < p.ntiles; t++) {
+        for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
+            for (int64_t q = p.group_row_begin[gi]; q < p.group_row_begin[gi + 1]; q++) {
+                int32_t sl = p.gr_slot[q];
+                if (sl < 0 || sl >= p.tile_row_begin[t + 1] - p.tile_row_begin[t]) return "group slot range";
+                int32_t r = p.rows[p.tile_row_begin[t] + sl];
+                if (grp_of[r] != -1) return "row in two groups";
+                grp_of[r] = gi;
+                x_of[r] = p.gr_export[q];
+                if (p.gr_len[q] != s.ptr[r + 1] - s.ptr[r]) return "row length";
+                if (p.gr_len[q] > p.group_width[gi]) return "group width";
+                if (p.gr_diag_pos[q] != s.diag_pos[r]) return "diag pos";
+                seen_rows++;
+            }
+        }
+    }
+    if (seen_rows != n) return "groups do not cover all rows";
+    for (int64_t t = 0; t < p.ntiles; t++)
+        for (int32_t q = p.tile_row_begin[t]; q < p.tile_row_begin[t + 1]; q++) {
+            int32_t r = p.rows[q];
+            int32_t xs = x_of[r];
+            if (xs >= 0 && (xs < p.tile_export_begin[t] || xs >= p.tile_export_begin[t + 1])) return "export slot owner";
+        }
+    // dependency checks: same tile -> earlier group; other tile -> earlier ticket and exported
+    for (int64_t gi = 0; gi < p.ngroups; gi++) {
+        int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
+        for (int64_t q = 0; q < m; q++) {
+            int32_t t = -1;
+            // find the row: slot -> row via its tile (the tile of group gi)
+            // (groups belong to exactly one tile)
+            int64_t lo = 0, hi = p.ntiles;
+            while (hi - lo > 1) {
+                int64_t mid = (lo + hi) / 2;
+                if (p.tile_group_begin[mid] <= gi) lo = mid; else hi = mid;
+            }
+            t = int32_t(lo);
+            int32_t r = p.rows[p.tile_row_begin[t] + p.gr_slot[b + q]];
+            for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
+                int64_t e = s.ptr[r] + k, c = s.col[e];
+                int64_t d = p.group_ent_begin[gi] + k * m + q;
+                if (p.ent_val_pos[d] != s.val_pos[e]) return "entry value position";
+                int32_t src = p.ent_src[d];
+                if (tile_of[c] == t) {
+                    if (src != slot_of[c]) return "intra-tile source";
+                    if (grp_of[c] >= gi) return "intra-tile dependency not in an earlier group";
+                } else {
+                    if (tile_of[c] >= t) return "dependency on a later tile";
+                    if (x_of[c] < 0 || src != -(x_of[c] + 1)) return "missing export";
+                }
+            }
+        }
+    }
+    return "";
+}
+
+template <typename FV, typename VT>
+void emulate_sweep(const SweepPlan& p, const FV* f, const VT* v, VT* y)
+{
+    std::vector<VT> lds(p.max_tile_rows), xbuf(p.nexport);
+    for (int64_t t = 0; t < p.ntiles; t++) {
+        int32_t rb = p.tile_row_begin[t], re = p.tile_row_begin[t + 1];
+        for (int32_t q = rb; q < re; q++) lds[q - rb] = v[p.rows[q]];
+        for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
+            int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
+            for (int64_t q = 0; q < m; q++) {
+                VT rr = lds[p.gr_slot[b + q]];
+                for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
+                    int64_t d = p.group_ent_begin[gi] + k * m + q;
+                    int32_t src = p.ent_src[d];
+                    VT yv = src >= 0 ? lds[src] : xbuf[-(src + 1)];
+                    rr -= VT(f[p.ent_val_pos[d]]) * yv;
+                }
+                VT out = p.gr_diag_pos[b + q] >= 0 ? VT(rr * VT(f[p.gr_diag_pos[b + q]])) : rr;
+                lds[p.gr_slot[b + q]] = out;
+                if (p.gr_export[b + q] >= 0) xbuf[p.gr_export[b + q]] = out;
+            }
+        }
+        for (int32_t q = rb; q < re; q++) y[p.rows[q]] = lds[q - rb];
+    }
+}
+
+template void emulate_sweep<double, double>(const SweepPlan&, const double*, const double*, double*);
+template void emulate_sweep<float, double>(const SweepPlan&, const float*, const double*, double*);
+template void emulate_sweep<float, float>(const SweepPlan&, const float*, const float*, float*);
+
+}  // namespace m4g
+
+namespace m4g {
+
+bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<int64_t>& US, std::vector<int64_t>& UI,
+                   std::vector<int64_t>& a_pos)
+{
+    US.assign(n + 1, 0);
+    UI.clear();
+    a_pos.clear();
+    for (int64_t r = 0; r < n; r++) {
+        bool first = true;
+        for (int64_t p = S[r]; p < S[r + 1]; p++) {
+            if (I[p] < r) continue;
+            if (first && I[p] != r) return false;   // U row must start with its diagonal
+            first = false;
+            UI.push_back(I[p]);
+            a_pos.push_back(p);
+        }
+        if (first) return false;
+        US[r + 1] = int64_t(UI.size());
+    }
+    return true;
+}
+
+bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64_t>& UI, SweepRows& lower,
+              SweepRows& upper)
+{
+    // upper sweep on U: diagonal first, then entries in column order
+    upper.n = n;
+    upper.lower = false;
+    upper.ptr.assign(n + 1, 0);
+    upper.col.clear();
+    upper.val_pos.clear();
+    upper.diag_pos.assign(n, -1);
+    for (int64_t r = 0; r < n; r++) {
+        if (US[r] == US[r + 1] || UI[US[r]] != r) return false;
+        upper.diag_pos[r] = US[r];
+        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
+            upper.col.push_back(UI[p]);
+            upper.val_pos.push_back(p);
+        }
+        upper.ptr[r + 1] = int64_t(upper.col.size());
+    }
+    // lower sweep on L = trans(U): row i holds U's column i (rows k < i
+    // ascending), inverted diagonal last
+    lower.n = n;
+    lower.lower = true;
+    lower.ptr.assign(n + 1, 0);
+    lower.diag_pos.assign(n, -1);
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) lower.ptr[UI[p] + 1]++;
+    for (int64_t r = 0; r < n; r++) lower.ptr[r + 1] += lower.ptr[r];
+    lower.col.assign(lower.ptr[n], 0);
+    lower.val_pos.assign(lower.ptr[n], 0);
+    std::vector<int64_t> fill(lower.ptr.begin(), lower.ptr.end() - 1);
+    for (int64_t r = 0; r < n; r++) {
+        lower.diag_pos[r] = US[r];
+        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
+            int64_t c = UI[p], d = fill[c]++;
+            lower.col[d] = r;
+            lower.val_pos[d] = p;
+        }
+    }
+    return true;
+}
+
+bool ilu0_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper,
+               std::vector<int64_t>& diag_pos)
+{
+    lower.n = upper.n = n;
+    lower.lower = true;
+    upper.lower = false;
+    lower.ptr.assign(n + 1, 0);
+    upper.ptr.assign(n + 1, 0);
+    lower.col.clear(); lower.val_pos.clear();
+    upper.col.clear(); upper.val_pos.clear();
+    lower.diag_pos.assign(n, -1);
+    upper.diag_pos.assign(n, -1);
+    diag_pos.assign(n, -1);
+    for (int64_t r = 0; r < n; r++) {
+        for (int64_t p = S[r]; p < S[r + 1]; p++) {
+            if (I[p] < r) { lower.col.push_back(I[p]); lower.val_pos.push_back(p); }
+            else if (I[p] == r) diag_pos[r] = p;
+            else { upper.col.push_back(I[p]); upper.val_pos.push_back(p); }
+        }
+        if (diag_pos[r] < 0) return false;
+        upper.diag_pos[r] = diag_pos[r];
+        lower.ptr[r + 1] = int64_t(lower.col.size());
+        upper.ptr[r + 1] = int64_t(upper.col.size());
+    }
+    return true;
+}
+
+bool ilu0_adjoint_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper)
+{
+    std::vector<int64_t> dpos(n, -1);
+    lower.n = upper.n = n;
+    lower.lower = true;
+    upper.lower = false;
+    lower.ptr.assign(n + 1, 0);
+    upper.ptr.assign(n + 1, 0);
+    lower.diag_pos.assign(n, -1);
+    upper.diag_pos.assign(n, -1);
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t p = S[r]; p < S[r + 1]; p++) {
+            if (I[p] == r) dpos[r] = p;
+            else if (I[p] > r) lower.ptr[I[p] + 1]++;   // U^T row I[p] gets (I[p], r)
+            else upper.ptr[I[p] + 1]++;                 // L^T row I[p] gets (I[p], r)
+        }
+    for (int64_t r = 0; r < n; r++) {
+        if (dpos[r] < 0) return false;
+        lower.ptr[r + 1] += lower.ptr[r];
+        upper.ptr[r + 1] += upper.ptr[r];
+        lower.diag_pos[r] = dpos[r];
+    }
+    lower.col.assign(lower.ptr[n], 0);
+    lower.val_pos.assign(lower.ptr[n], 0);
+    upper.col.assign(upper.ptr[n], 0);
+    upper.val_pos.assign(upper.ptr[n], 0);
+    std::vector<int64_t> lf(lower.ptr.begin(), lower.ptr.end() - 1);
+    // ascending r for U^T rows (scatter order of lower_trisolve.hpp:155-170)
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t p = S[r]; p < S[r + 1]; p++)
+            if (I[p] > r) {
+                int64_t d = lf[I[p]]++;
+                lower.col[d] = r;
+                lower.val_pos[d] = p;
+            }
+    // descending r for L^T rows (scatter order of upper_trisolve.hpp:131-149)
+    std::vector<int64_t> uf(upper.ptr.begin(), upper.ptr.end() - 1);
+    for (int64_t r = n - 1; r >= 0; r--)
+        for (int64_t p = S[r]; p < S[r + 1]; p++)
+            if (I[p] < r) {
+                int64_t d = uf[I[p]]++;
+                upper.col[d] = r;
+                upper.val_pos[d] = p;
+            }
+    return true;
+}
+
+}  // namespace m4g

@@ -1,0 +1,237 @@
+// CSR SpMV on MI355X through a sliced-ELL copy of the matrix.
+//
+// Layout: rows are grouped in slices of 64 (one wavefront, one row per lane).
+// Inside a slice the k-th stored entry of the 64 rows is contiguous, so every
+// value/column load of a wavefront is one 512 B / 256 B coalesced access, and
+// for stencil matrices the x gathers of a wavefront are contiguous as well.
+// Each row keeps the reference's stored (column-sorted) order and is summed
+// sequentially from zero, so results equal mat_vec_mult.hpp:662-687 bit for
+// bit (the library is built with -ffp-contract=off).  Workgroups are mapped so
+// that every XCD streams one contiguous range of slices, keeping the x window
+// of a range inside that XCD's L2.
+#include "m4g_internal.hpp"
+#include "m4g_device.hpp"
+#include "spmv_dev.hpp"
+#include "krylov.hpp"
+
+#include <algorithm>
+
+namespace m4g {
+
+__global__ __launch_bounds__(kSpmvNT) void k_sell_fill(int64_t n, const int64_t* __restrict__ slice_ptr,
+                                                       const int32_t* __restrict__ starts,
+                                                       const int32_t* __restrict__ indices,
+                                                       const double* __restrict__ values, int32_t* __restrict__ cols,
+                                                       double* __restrict__ vals, int64_t* __restrict__ pos)
+{
+    int64_t r = blockIdx.x * int64_t(kSpmvNT) + threadIdx.x;
+    int64_t s = r / kWave;
+    int lane = int(r % kWave);
+    int64_t nslices = (n + kWave - 1) / kWave;
+    if (s >= nslices) return;
+    int64_t base = slice_ptr[s];
+    int width = int((slice_ptr[s + 1] - base) / kWave);
+    int64_t b = 0, len = 0;
+    if (r < n) {
+        b = starts[r];
+        len = starts[r + 1] - b;
+    }
+    for (int k = 0; k < width; k++) {
+        int64_t d = base + int64_t(k) * kWave + lane;
+        if (k < len) {
+            cols[d] = indices[b + k];
+            vals[d] = values[b + k];
+            pos[d] = b + k;
+        } else {
+            cols[d] = -1;
+            vals[d] = 0.0;
+            pos[d] = -1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSpmvNT) void k_sell_refresh(int64_t padded, const int64_t* __restrict__ pos,
+                                                          const double* __restrict__ values, double* __restrict__ vals)
+{
+    for (int64_t d = blockIdx.x * int64_t(kSpmvNT) + threadIdx.x; d < padded; d += int64_t(gridDim.x) * kSpmvNT) {
+        int64_t p = pos[d];
+        vals[d] = p >= 0 ? values[p] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kSpmvNT) void k_sell_to_f32(int64_t padded, const double* __restrict__ v,
+                                                         float* __restrict__ f)
+{
+    for (int64_t d = blockIdx.x * int64_t(kSpmvNT) + threadIdx.x; d < padded; d += int64_t(gridDim.x) * kSpmvNT)
+        f[d] = float(v[d]);
+}
+
+void sell_build(m4g_matrix_s& A)
+{
+    Sell& S = A.sell;
+    S.n = A.nrows;
+    S.nslices = (A.nrows + kWave - 1) / kWave;
+    std::vector<int64_t> sp(S.nslices + 1, 0);
+    int maxw = 0;
+    for (int64_t s = 0; s < S.nslices; s++) {
+        int64_t r0 = s * kWave, r1 = std::min<int64_t>(A.nrows, r0 + kWave);
+        int64_t w = 0;
+        for (int64_t r = r0; r < r1; r++) w = std::max<int64_t>(w, A.h_starts[r + 1] - A.h_starts[r]);
+        maxw = std::max<int>(maxw, int(w));
+        sp[s + 1] = sp[s] + w * kWave;
+    }
+    S.padded = sp[S.nslices];
+    S.max_width = maxw;
+    M4G_REQUIRE(S.padded < (int64_t(1) << 40), M4G_ERR_INVALID, "matrix too large");
+    S.slice_ptr.reset(S.nslices + 1);
+    S.slice_ptr.upload(sp.data(), S.nslices + 1);
+    S.cols.reset(S.padded);
+    S.vals.reset(S.padded);
+    S.csr_pos.reset(S.padded);
+    S.vals32.release();
+    if (S.nslices) {
+        int64_t threads = S.nslices * kWave;
+        hipLaunchKernelGGL(k_sell_fill, dim3((threads + kSpmvNT - 1) / kSpmvNT), dim3(kSpmvNT), 0, current_stream(),
+                           A.nrows, S.slice_ptr.ptr, A.starts.ptr, A.indices.ptr, A.values.ptr, S.cols.ptr,
+                           S.vals.ptr, S.csr_pos.ptr);
+        M4G_HIP(hipGetLastError());
+    }
+    // reduction workspace for the fused SpMV + dot
+    int64_t groups = (S.nslices + kSliceWaves - 1) / kSliceWaves;
+    int64_t chunk = (groups + kXcds - 1) / kXcds;
+    A.red_partials.reset(std::max<int64_t>(1, chunk * kXcds));
+    A.red_counter.reset(1);
+    M4G_HIP(hipMemsetAsync(A.red_counter.ptr, 0, sizeof(unsigned), current_stream()));
+}
+
+void sell_refresh_values(m4g_matrix_s& A)
+{
+    Sell& S = A.sell;
+    if (!S.padded) return;
+    int64_t g = std::min<int64_t>((S.padded + kSpmvNT - 1) / kSpmvNT, 16384);
+    hipLaunchKernelGGL(k_sell_refresh, dim3(g), dim3(kSpmvNT), 0, current_stream(), S.padded, S.csr_pos.ptr,
+                       A.values.ptr, S.vals.ptr);
+    M4G_HIP(hipGetLastError());
+    if (S.vals32.ptr) sell_build_f32(A);
+}
+
+void sell_build_f32(m4g_matrix_s& A)
+{
+    Sell& S = A.sell;
+    if (!S.padded) return;
+    if (!S.vals32.ptr) S.vals32.reset(S.padded);
+    int64_t g = std::min<int64_t>((S.padded + kSpmvNT - 1) / kSpmvNT, 16384);
+    hipLaunchKernelGGL(k_sell_to_f32, dim3(g), dim3(kSpmvNT), 0, current_stream(), S.padded, S.vals.ptr,
+                       S.vals32.ptr);
+    M4G_HIP(hipGetLastError());
+}
+
+// MODE 0: w = A v; 1: w += A v; 2: w -= A v; 3: w = b - A v.  DOT adds p.q.
+template <int MODE, bool DOT>
+__global__ __launch_bounds__(kSpmvNT) void k_sell_spmv(int64_t n, int64_t nslices, int64_t chunk,
+                                                       const int64_t* __restrict__ slice_ptr,
+                                                       const int32_t* __restrict__ cols,
+                                                       const double* __restrict__ vals, const double* __restrict__ x,
+                                                       double* __restrict__ w, const double* __restrict__ b,
+                                                       double* partials, unsigned* counter, double* out)
+{
+    __shared__ double scratch[kSliceWaves];
+    const int64_t g = xcd_logical_block(chunk);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t s = g * kSliceWaves + wave;
+    double dotp = 0;
+    if (s < nslices) {
+        const int64_t base = slice_ptr[s];
+        const int width = int((slice_ptr[s + 1] - base) / kWave);
+        const double acc = sell_row_sum(cols, vals, x, base, width, lane);
+        const int64_t row = s * kWave + lane;
+        if (row < n) {
+            if (MODE == 0) w[row] = acc;
+            else if (MODE == 1) w[row] = w[row] + acc;
+            else if (MODE == 2) w[row] = w[row] - acc;
+            else w[row] = b[row] - acc;
+            if (DOT) dotp = x[row] * acc;
+        }
+    }
+    if (DOT) {
+        double blk = block_sum<kSpmvNT>(dotp, scratch);
+        double total;
+        if (last_block_total<kSpmvNT>(blk, partials, counter, &total, scratch)) *out = total;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSpmvNT) void k_sell_residual(int64_t n, int64_t nslices, int64_t chunk,
+                                                           const int64_t* __restrict__ slice_ptr,
+                                                           const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                           const T* __restrict__ x, const T* __restrict__ b,
+                                                           T* __restrict__ r)
+{
+    const int64_t g = xcd_logical_block(chunk);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t s = g * kSliceWaves + wave;
+    if (s >= nslices) return;
+    const int64_t base = slice_ptr[s];
+    const int width = int((slice_ptr[s + 1] - base) / kWave);
+    const T acc = sell_row_sum(cols, vals, x, base, width, lane);
+    const int64_t row = s * kWave + lane;
+    if (row < n) r[row] = b[row] - acc;
+}
+
+static void spmv_geometry(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
+{
+    int64_t groups = (A.sell.nslices + kSliceWaves - 1) / kSliceWaves;
+    chunk = (groups + kXcds - 1) / kXcds;
+    grid = std::max<int64_t>(1, chunk * kXcds);
+}
+
+void launch_spmv(const m4g_matrix_s& A, const double* x, double* w, int mode, const double* b, hipStream_t st)
+{
+    if (A.nrows == 0) return;
+    int64_t chunk, grid;
+    spmv_geometry(A, chunk, grid);
+    const Sell& S = A.sell;
+#define M4G_SPMV_ARGS                                                                                   \
+    dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk, S.slice_ptr.ptr, S.cols.ptr, S.vals.ptr, x, w, b, \
+        (double*)nullptr, (unsigned*)nullptr, (double*)nullptr
+    if (b) hipLaunchKernelGGL((k_sell_spmv<3, false>), M4G_SPMV_ARGS);
+    else if (mode == M4G_ASSIGN) hipLaunchKernelGGL((k_sell_spmv<0, false>), M4G_SPMV_ARGS);
+    else if (mode == M4G_PLUS) hipLaunchKernelGGL((k_sell_spmv<1, false>), M4G_SPMV_ARGS);
+    else hipLaunchKernelGGL((k_sell_spmv<2, false>), M4G_SPMV_ARGS);
+#undef M4G_SPMV_ARGS
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* out, hipStream_t st)
+{
+    if (A.nrows == 0) {
+        M4G_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
+        return;
+    }
+    int64_t chunk, grid;
+    spmv_geometry(A, chunk, grid);
+    const Sell& S = A.sell;
+    hipLaunchKernelGGL((k_sell_spmv<0, true>), dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
+                       S.slice_ptr.ptr, S.cols.ptr, S.vals.ptr, p, q, (const double*)nullptr,
+                       A.red_partials.ptr, A.red_counter.ptr, out);
+    M4G_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_residual_t(const m4g_matrix_s& A, const T* x, const T* b, T* r, hipStream_t st)
+{
+    if (A.nrows == 0) return;
+    int64_t chunk, grid;
+    spmv_geometry(A, chunk, grid);
+    const Sell& S = A.sell;
+    const T* vals = sizeof(T) == 8 ? (const T*)S.vals.ptr : (const T*)S.vals32.ptr;
+    M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
+    hipLaunchKernelGGL(k_sell_residual<T>, dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
+                       S.slice_ptr.ptr, S.cols.ptr, vals, x, b, r);
+    M4G_HIP(hipGetLastError());
+}
+
+template void launch_residual_t<double>(const m4g_matrix_s&, const double*, const double*, double*, hipStream_t);
+template void launch_residual_t<float>(const m4g_matrix_s&, const float*, const float*, float*, hipStream_t);
+
+}  // namespace m4g

@@ -1,0 +1,495 @@
+"""Python binding of libmtl4gpu (the MI355X sparse iterative-solve library).
+
+A thin ctypes layer over the C ABI in include/mtl4gpu.h, used by the tests,
+the benchmark and the torch.autograd test binding.  It mirrors the
+reference's objects: ``Matrix`` is a device-resident ``mtl::compressed2D``,
+``Vector`` a device-resident ``mtl::dense_vector``, ``PC`` one of
+``itl::pc::{identity, diagonal, ic_0, ilu_0}``, and ``cg`` / ``bicgstab`` are
+``itl::cg`` / ``itl::bicgstab`` driven by an ``Iteration`` (``basic_iteration``).
+
+Importing this module fails loudly when the HIP library has not been built;
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmtl4gpu.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libmtl4gpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+# ---- constants (include/mtl4gpu.h) ------------------------------------------
+OK, MAX_ITER = 0, 1
+ERR_NOT_SQUARE, ERR_MISSING_DIAGONAL, ERR_ORTHOGONALITY, ERR_LOGIC, ERR_SIZE, ERR_INVALID, ERR_INTERRUPTED = range(2, 9)
+ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3
+F64, F32 = 0, 1
+IDX_I32, IDX_I64, IDX_U32 = 0, 1, 2
+ASSIGN, PLUS, MINUS = 0, 1, 2
+PC_IDENTITY, PC_DIAGONAL, PC_IC0, PC_ILU0 = 0, 1, 2, 3
+METHOD_CG, METHOD_BICGSTAB = 0, 1
+
+_c_i64 = ctypes.c_int64
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class PcStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "levels_lower", "levels_upper", "tiles_lower", "tiles_upper", "steps_lower", "steps_upper",
+        "grid_nx", "grid_ny", "grid_nz", "tile_nx", "tile_ny", "tile_nz")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Iteration(ctypes.Structure):
+    """m4g_iteration: the state of itl::basic_iteration (basic_iteration.hpp:24-165)."""
+    _fields_ = [("max_iter", ctypes.c_int), ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+                ("norm_r0", ctypes.c_double), ("iterations", ctypes.c_int), ("error", ctypes.c_int),
+                ("resid", ctypes.c_double), ("history", _c_dp), ("history_capacity", ctypes.c_int),
+                ("interrupt", ctypes.POINTER(ctypes.c_int))]
+
+
+_SIGS = {
+    "m4g_last_error": (ctypes.c_char_p, []),
+    "m4g_version": (ctypes.c_char_p, []),
+    "m4g_set_device": (ctypes.c_int, [ctypes.c_int]),
+    "m4g_set_stream": (ctypes.c_int, [_vp]),
+    "m4g_synchronize": (ctypes.c_int, []),
+    "m4g_get_stream": (_vp, []),
+    "m4g_pc_apply_async": (ctypes.c_int, [_vp, _vp, _vp]),
+    "m4g_matrix_create": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, ctypes.c_int,
+                                         ctypes.c_int]),
+    "m4g_matrix_destroy": (ctypes.c_int, [_vp]),
+    "m4g_matrix_dims": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i64p]),
+    "m4g_matrix_download": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "m4g_matrix_set_values": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "m4g_vector_create": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, ctypes.c_int]),
+    "m4g_vector_wrap": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _c_i64, ctypes.c_int]),
+    "m4g_vector_destroy": (ctypes.c_int, [_vp]),
+    "m4g_vector_size": (ctypes.c_int, [_vp, _c_i64p]),
+    "m4g_vector_device_ptr": (_vp, [_vp]),
+    "m4g_vector_upload": (ctypes.c_int, [_vp, _vp]),
+    "m4g_vector_download": (ctypes.c_int, [_vp, _vp]),
+    "m4g_vector_fill": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m4g_vector_copy": (ctypes.c_int, [_vp, _vp]),
+    "m4g_dot": (ctypes.c_int, [_vp, _vp, _c_dp]),
+    "m4g_two_norm": (ctypes.c_int, [_vp, _c_dp]),
+    "m4g_unary_dot": (ctypes.c_int, [_vp, _c_dp]),
+    "m4g_axpy": (ctypes.c_int, [ctypes.c_double, _vp, _vp]),
+    "m4g_xpay": (ctypes.c_int, [_vp, ctypes.c_double, _vp]),
+    "m4g_axpy_unary_dot": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _c_dp]),
+    "m4g_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "m4g_residual": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "m4g_spmv_dot": (ctypes.c_int, [_vp, _vp, _vp, _c_dp]),
+    "m4g_pc_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int]),
+    "m4g_pc_destroy": (ctypes.c_int, [_vp]),
+    "m4g_pc_solve": (ctypes.c_int, [_vp, _vp, _vp]),
+    "m4g_pc_adjoint_solve": (ctypes.c_int, [_vp, _vp, _vp]),
+    "m4g_pc_factor_nnz": (ctypes.c_int, [_vp, ctypes.c_int, _c_i64p]),
+    "m4g_pc_download_factor": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    "m4g_pc_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(PcStats)]),
+    "m4g_cg": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_bicgstab": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    # host-only analysis (mtl4gpu_analysis.h)
+    "m4g_analysis_grid": (ctypes.c_int, [_c_i64, _vp, _vp, _c_i64p, _c_i64p, _c_i64p]),
+    "m4g_analysis_pc_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_i64, _vp, _vp, _vp, ctypes.c_int,
+                                             ctypes.c_int, _vp, _vp, _c_i64, ctypes.POINTER(PcStats)]),
+    # generators (mtl4gpu_gen.h)
+    "m4g_gen_laplacian2d": (_c_i64, [_c_i64, _c_i64, _vp, _vp, _vp]),
+    "m4g_gen_poisson3d": (_c_i64, [_c_i64, _c_i64, _c_i64, _vp, _vp, _vp]),
+    "m4g_gen_convdiff27": (_c_i64, [_c_i64, _c_i64, _c_i64, ctypes.c_uint64, ctypes.c_double, _vp, _vp, _vp]),
+    "m4g_gen_varcoef5": (_c_i64, [_c_i64, _c_i64, ctypes.c_uint64, _vp, _vp, _vp]),
+    "m4g_gen_edge_laplacian": (_c_i64, [_c_i64, _c_i64, _vp, _c_i64p, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "m4g_rng_uniform": (None, [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double,
+                               _vp]),
+    "m4g_rng_normal": (None, [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i64, _vp]),
+    "m4g_rng_lognormal": (None, [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i64, _vp]),
+    "m4g_rng_int": (None, [ctypes.c_uint64, ctypes.c_uint64, _c_i64, _c_i64, _c_i64, _c_i64, _vp]),
+}
+
+
+def _bind(name, restype, argtypes):
+    f = getattr(_lib, name, None)
+    if f is None:
+        return None
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+_f = {name: _bind(name, *sig) for name, sig in _SIGS.items()}
+globals().update({"_" + k: v for k, v in _f.items()})
+
+
+class M4GError(RuntimeError):
+    def __init__(self, status, where):
+        msg = _f["m4g_last_error"]().decode(errors="replace")
+        super().__init__(f"{where} failed with status {status}: {msg}")
+        self.status = status
+
+
+def _check(status, where, allow=(OK,)):
+    if status not in allow:
+        raise M4GError(status, where)
+    return status
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _dtype_code(dt):
+    dt = np.dtype(dt)
+    if dt == np.float64:
+        return F64
+    if dt == np.float32:
+        return F32
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+def _np_dtype(code):
+    return np.float64 if code == F64 else np.float32
+
+
+def version():
+    return _f["m4g_version"]().decode()
+
+
+def synchronize():
+    _check(_f["m4g_synchronize"](), "m4g_synchronize")
+
+
+def get_stream():
+    return _f["m4g_get_stream"]()
+
+
+def set_stream(stream_ptr):
+    _check(_f["m4g_set_stream"](ctypes.c_void_p(stream_ptr) if stream_ptr else None), "m4g_set_stream")
+
+
+# ---------------------------------------------------------------------------
+# handles
+# ---------------------------------------------------------------------------
+class Matrix:
+    """Device-resident compressed2D built from host CSR arrays (copied)."""
+
+    def __init__(self, starts, indices, values, ncols=None):
+        starts = np.ascontiguousarray(starts, dtype=np.int64)
+        indices = np.ascontiguousarray(indices, dtype=np.int64)
+        values = np.ascontiguousarray(values)
+        if values.dtype not in (np.float64, np.float32):
+            values = values.astype(np.float64)
+        self.nrows = len(starts) - 1
+        self.ncols = self.nrows if ncols is None else ncols
+        self.nnz = int(starts[-1])
+        h = ctypes.c_void_p()
+        _check(_f["m4g_matrix_create"](ctypes.byref(h), self.nrows, self.ncols, self.nnz, _ptr(starts),
+                                       _ptr(indices), _ptr(values), IDX_I64, _dtype_code(values.dtype)),
+               "m4g_matrix_create")
+        self.h = h
+
+    def download(self):
+        S = np.zeros(self.nrows + 1, np.int64)
+        I = np.zeros(self.nnz, np.int64)
+        V = np.zeros(self.nnz, np.float64)
+        _check(_f["m4g_matrix_download"](self.h, _ptr(S), _ptr(I), _ptr(V)), "m4g_matrix_download")
+        return S, I, V
+
+    def set_values(self, values):
+        """values: numpy float64 array (host) or a contiguous float64 torch tensor on the GPU."""
+        if hasattr(values, "data_ptr"):
+            _check(_f["m4g_matrix_set_values"](self.h, ctypes.c_void_p(values.data_ptr()), 1), "m4g_matrix_set_values")
+        else:
+            v = np.ascontiguousarray(values, np.float64)
+            _check(_f["m4g_matrix_set_values"](self.h, _ptr(v), 0), "m4g_matrix_set_values")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _f["m4g_matrix_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Vector:
+    """Device-resident dense_vector; owns its memory, or borrows a torch tensor's."""
+
+    def __init__(self, n=None, dtype=np.float64, data=None, tensor=None):
+        h = ctypes.c_void_p()
+        self._tensor = tensor
+        if tensor is not None:
+            import torch
+            assert tensor.is_cuda and tensor.is_contiguous(), "tensor must be contiguous on the GPU"
+            code = F64 if tensor.dtype == torch.float64 else F32
+            self.n, self.dtype = tensor.numel(), _np_dtype(code)
+            _check(_f["m4g_vector_wrap"](ctypes.byref(h), ctypes.c_void_p(tensor.data_ptr()), self.n, code),
+                   "m4g_vector_wrap")
+        else:
+            if data is not None:
+                data = np.ascontiguousarray(data, dtype=dtype)
+                n = len(data)
+            self.n, self.dtype = int(n), np.dtype(dtype).type
+            _check(_f["m4g_vector_create"](ctypes.byref(h), self.n, _dtype_code(dtype)), "m4g_vector_create")
+        self.h = h
+        if data is not None:
+            self.upload(data)
+
+    def upload(self, data):
+        data = np.ascontiguousarray(data, dtype=self.dtype)
+        assert len(data) == self.n
+        _check(_f["m4g_vector_upload"](self.h, _ptr(data)), "m4g_vector_upload")
+
+    def numpy(self):
+        out = np.zeros(self.n, dtype=self.dtype)
+        _check(_f["m4g_vector_download"](self.h, _ptr(out)), "m4g_vector_download")
+        return out
+
+    def fill(self, value):
+        _check(_f["m4g_vector_fill"](self.h, float(value)), "m4g_vector_fill")
+
+    def copy_from(self, other):
+        _check(_f["m4g_vector_copy"](self.h, other.h), "m4g_vector_copy")
+
+    def device_ptr(self):
+        return _f["m4g_vector_device_ptr"](self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _f["m4g_vector_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PC:
+    """itl::pc::identity / diagonal / ic_0 / ilu_0 built from a Matrix."""
+
+    def __init__(self, A, kind, factor_dtype=np.float64):
+        h = ctypes.c_void_p()
+        self.kind, self.n = kind, A.nrows
+        _check(_f["m4g_pc_create"](ctypes.byref(h), A.h, kind, _dtype_code(factor_dtype)), "m4g_pc_create")
+        self.h = h
+
+    def solve(self, x, y):
+        return _check(_f["m4g_pc_solve"](self.h, x.h, y.h), "m4g_pc_solve")
+
+    def apply_async(self, x, y):
+        return _check(_f["m4g_pc_apply_async"](self.h, x.h, y.h), "m4g_pc_apply_async")
+
+    def adjoint_solve(self, x, y):
+        return _check(_f["m4g_pc_adjoint_solve"](self.h, x.h, y.h), "m4g_pc_adjoint_solve")
+
+    def factor(self, which):
+        """Returns (starts, indices, values) of L (which=0) or U (which=1)."""
+        nnz = ctypes.c_int64()
+        _check(_f["m4g_pc_factor_nnz"](self.h, which, ctypes.byref(nnz)), "m4g_pc_factor_nnz")
+        S = np.zeros(self.n + 1, np.int64)
+        I = np.zeros(nnz.value, np.int64)
+        V = np.zeros(nnz.value, np.float64)
+        _check(_f["m4g_pc_download_factor"](self.h, which, _ptr(S), _ptr(I), _ptr(V)), "m4g_pc_download_factor")
+        return S, I, V
+
+    def stats(self):
+        st = PcStats()
+        _check(_f["m4g_pc_stats_get"](self.h, ctypes.byref(st)), "m4g_pc_stats_get")
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            _f["m4g_pc_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def _scalar(fn, where, *args):
+    out = ctypes.c_double()
+    _check(fn(*args, ctypes.byref(out)), where)
+    return out.value
+
+
+def dot(x, y):
+    return _scalar(_f["m4g_dot"], "m4g_dot", x.h, y.h)
+
+
+def two_norm(x):
+    return _scalar(_f["m4g_two_norm"], "m4g_two_norm", x.h)
+
+
+def unary_dot(x):
+    return _scalar(_f["m4g_unary_dot"], "m4g_unary_dot", x.h)
+
+
+def axpy(alpha, x, y):
+    _check(_f["m4g_axpy"](float(alpha), x.h, y.h), "m4g_axpy")
+
+
+def xpay(x, beta, y):
+    _check(_f["m4g_xpay"](x.h, float(beta), y.h), "m4g_xpay")
+
+
+def axpy_unary_dot(alpha, q, r):
+    return _scalar(_f["m4g_axpy_unary_dot"], "m4g_axpy_unary_dot", float(alpha), q.h, r.h)
+
+
+def spmv(A, v, w, mode=ASSIGN):
+    _check(_f["m4g_spmv"](A.h, v.h, w.h, mode), "m4g_spmv")
+
+
+def residual(A, x, b, r):
+    _check(_f["m4g_residual"](A.h, x.h, b.h, r.h), "m4g_residual")
+
+
+def spmv_dot(A, p, q):
+    return _scalar(_f["m4g_spmv_dot"], "m4g_spmv_dot", A.h, p.h, q.h)
+
+
+def make_iteration(max_iter, rtol, atol=0.0, norm_r0=0.0, history=True):
+    it = Iteration()
+    it.max_iter, it.rtol, it.atol, it.norm_r0 = int(max_iter), float(rtol), float(atol), float(norm_r0)
+    it.iterations, it.error, it.resid = 0, 0, 0.0
+    hist = None
+    if history:
+        hist = np.full(int(max_iter) + 2, np.nan)
+        it.history = hist.ctypes.data_as(_c_dp)
+        it.history_capacity = len(hist)
+    return it, hist
+
+
+def _solve(fn, where, A, x, b, P, it):
+    st = fn(A.h, x.h, b.h, P.h, ctypes.byref(it))
+    return _check(st, where, allow=(OK, MAX_ITER))
+
+
+def cg(A, x, b, P, it):
+    """itl::cg(A, x, b, P, iter); returns the iteration's error code."""
+    return _solve(_f["m4g_cg"], "m4g_cg", A, x, b, P, it)
+
+
+def bicgstab(A, x, b, P, it):
+    """itl::bicgstab(A, x, b, P, iter); returns the iteration's error code."""
+    return _solve(_f["m4g_bicgstab"], "m4g_bicgstab", A, x, b, P, it)
+
+
+# ---------------------------------------------------------------------------
+# host-only helpers: analysis and generators
+# ---------------------------------------------------------------------------
+def analysis_grid(starts, indices):
+    S = np.ascontiguousarray(starts, np.int64)
+    I = np.ascontiguousarray(indices, np.int64)
+    nx, ny, nz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _f["m4g_analysis_grid"](len(S) - 1, _ptr(S), _ptr(I), ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz))
+    return nx.value, ny.value, nz.value
+
+
+def analysis_pc_solve(kind, starts, indices, factor, rhs, adjoint=False, factor_dtype=np.float64,
+                      vector_dtype=np.float64, max_tile_rows=0):
+    S = np.ascontiguousarray(starts, np.int64)
+    I = np.ascontiguousarray(indices, np.int64)
+    f = np.ascontiguousarray(factor, np.float64)
+    rhs = np.ascontiguousarray(rhs, vector_dtype)
+    out = np.zeros_like(rhs)
+    st = PcStats()
+    _check(_f["m4g_analysis_pc_solve"](kind, int(adjoint), len(S) - 1, _ptr(S), _ptr(I), _ptr(f),
+                                       _dtype_code(factor_dtype), _dtype_code(vector_dtype), _ptr(rhs), _ptr(out),
+                                       int(max_tile_rows), ctypes.byref(st)), "m4g_analysis_pc_solve")
+    return out, st.as_dict()
+
+
+def _gen(fn, *args):
+    nnz = fn(*args, None, None, None)
+    n_rows = None
+    return nnz
+
+
+def _csr_from(fn, n, *args):
+    nnz = fn(*args, None, None, None)
+    S = np.zeros(n + 1, np.int64)
+    I = np.zeros(nnz, np.int64)
+    V = np.zeros(nnz, np.float64)
+    fn(*args, _ptr(S), _ptr(I), _ptr(V))
+    return S, I, V
+
+
+def laplacian2d(m, n):
+    """mtl::mat::laplacian_setup(A, m, n) (laplacian_setup.hpp:25-44)."""
+    return _csr_from(_f["m4g_gen_laplacian2d"], m * n, m, n)
+
+
+def poisson3d(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _csr_from(_f["m4g_gen_poisson3d"], nx * ny * nz, nx, ny, nz)
+
+
+def convdiff27(nx, ny=None, nz=None, seed=3, pe=0.5):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _csr_from(_f["m4g_gen_convdiff27"], nx * ny * nz, nx, ny, nz, ctypes.c_uint64(seed), float(pe))
+
+
+def varcoef5(nx, ny=None, seed=4):
+    ny = nx if ny is None else ny
+    return _csr_from(_f["m4g_gen_varcoef5"], nx * ny, nx, ny, ctypes.c_uint64(seed))
+
+
+def edge_laplacian(nx, ny, theta=None):
+    """Returns dict(S, I, V, ei, ej, aptr, aidx, acoef, n_edges) for A(theta)."""
+    ne = ctypes.c_int64()
+    fn = _f["m4g_gen_edge_laplacian"]
+    nnz = fn(nx, ny, None, ctypes.byref(ne), None, None, None, None, None, None, None, None)
+    n = nx * ny
+    S = np.zeros(n + 1, np.int64); I = np.zeros(nnz, np.int64); V = np.zeros(nnz, np.float64)
+    ei = np.zeros(ne.value, np.int64); ej = np.zeros(ne.value, np.int64)
+    aptr = np.zeros(nnz + 1, np.int64); aidx = np.zeros(4 * ne.value, np.int64); acoef = np.zeros(4 * ne.value)
+    th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+    fn(nx, ny, _ptr(th), ctypes.byref(ne), _ptr(S), _ptr(I), _ptr(V), _ptr(ei), _ptr(ej), _ptr(aptr), _ptr(aidx),
+       _ptr(acoef))
+    na = int(aptr[-1])
+    return dict(S=S, I=I, V=V, ei=ei, ej=ej, aptr=aptr, aidx=aidx[:na], acoef=acoef[:na], n_edges=ne.value)
+
+
+def rng_uniform(seed, stream, count, lo=-1.0, hi=1.0, offset=0):
+    out = np.zeros(count)
+    _f["m4g_rng_uniform"](seed, stream, offset, count, lo, hi, _ptr(out))
+    return out
+
+
+def rng_normal(seed, stream, count, offset=0):
+    out = np.zeros(count)
+    _f["m4g_rng_normal"](seed, stream, offset, count, _ptr(out))
+    return out
+
+
+def rng_lognormal(seed, stream, count, offset=0):
+    out = np.zeros(count)
+    _f["m4g_rng_lognormal"](seed, stream, offset, count, _ptr(out))
+    return out
+
+
+def rng_int(seed, stream, count, lo, hi, offset=0):
+    out = np.zeros(count, np.int64)
+    _f["m4g_rng_int"](seed, stream, offset, count, lo, hi, _ptr(out))
+    return out

@@ -1,0 +1,595 @@
+// CPU oracle for the sparse iterative-solve path of MTL4/ITL.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, loop by loop, the reference
+// algorithms on the path (CSR SpMV, dot/two_norm/unary_dot, the fused CG
+// loops, IC(0) and ILU(0) factorisation, the compact-CSR triangular sweeps,
+// CG, BiCGStab and the iteration control).  It is linked only by tests/,
+// by __graft_entry__.smoke() as a checker and by bench.py's cpu_baseline
+// leg.  The product (the HIP library) never calls it.
+//
+// Every function cites the reference file:line it follows (paths relative to
+// the reference checkout; mtl/ = boost/numeric/mtl, itl/ = boost/numeric/itl).
+// Summation orders, operand orders and reciprocal-then-multiply sequences are
+// kept exactly, so results are bit-identical to an MTL4 build without FMA
+// contraction (build this file with -ffp-contract=off).
+//
+// Index type: int64_t, standing in for MTL4's default std::size_t
+// (mtl/matrix/parameter.hpp:33).
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+typedef int64_t idx_t;
+
+enum {
+    ORC_OK = 0,
+    ORC_MAX_ITER = 1,                  // basic_iteration.hpp:43-48
+    ORC_ERR_NOT_SQUARE = 2,            // mtl::matrix_not_square
+    ORC_ERR_MISSING_DIAGONAL = 3,      // mtl::missing_diagonal
+    ORC_ERR_ORTHOGONALITY = 4,         // itl::unexpected_orthogonality
+};
+
+// ---------------------------------------------------------------------------
+// Iteration control: itl/iteration/basic_iteration.hpp:24-165
+// ---------------------------------------------------------------------------
+struct orc_iteration {
+    int max_iter;
+    double rtol, atol, norm_r0;
+    int i, error, is_finished;
+    double resid;
+    double* history;      // optional: resid at every finished() call, indexed by i
+    int history_cap;
+};
+
+static bool orc_converged(orc_iteration* it, double r)
+{
+    it->resid = r;                                   // basic_iteration.hpp:83
+    if (it->history && it->i < it->history_cap) it->history[it->i] = r;
+    if (it->norm_r0 == 0)                            // :90-91
+        return r <= it->atol;
+    return r <= it->rtol * it->norm_r0 || r <= it->atol;   // :92
+}
+
+static bool orc_finished(orc_iteration* it, double r)
+{
+    if (orc_converged(it, r)) return (it->is_finished = 1);  // :60-64
+    if (it->i >= it->max_iter) {                           // check_max :43-48
+        it->error = ORC_MAX_ITER;
+        it->is_finished = 1;
+    }
+    return it->is_finished;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+
+// dot(v1, v2) -> sfunctor::dot<4>: mtl/operation/dot.hpp:136-147 with the
+// index distribution of dot_aux (mm/loop1.hpp:21-31): element i+k goes to
+// partial k; the tail goes to partial 0; combine ((t0+t1)+(t2+t3)) + 0.
+static double dot4(idx_t n, const double* a, const double* b)
+{
+    double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0, t7 = 0;
+    idx_t nb = 4 * (n / 4);
+    for (idx_t i = 0; i < nb; i += 4) {
+        t0 += a[i] * b[i];
+        t1 += a[i + 1] * b[i + 1];
+        t2 += a[i + 2] * b[i + 2];
+        t3 += a[i + 3] * b[i + 3];
+    }
+    for (idx_t i = nb; i < n; i++) t0 += a[i] * b[i];
+    return ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7));
+}
+
+// reduction<U, two_norm_functor>: mtl/vector/reduction.hpp:154-173; the
+// finish tree of impl::reduction (:51-57) gives t0 + (t1 + (t2 + ...)).
+template <int U>
+static double sq_reduction(idx_t n, const double* v)
+{
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    idx_t nb = U * (n / U);
+    for (idx_t i = 0; i < nb; i += U)
+        for (int k = 0; k < U; k++) t[k] += v[i + k] * v[i + k];   // squared_abs
+    for (idx_t i = nb; i < n; i++) t[0] += v[i] * v[i];
+    double acc = t[U - 1];
+    for (int k = U - 2; k >= 0; k--) acc = t[k] + acc;   // t[k] += t[k+1] chain
+    return acc;
+}
+
+// two_norm: mtl/operation/two_norm.hpp:52-57 (reduction<4>, then sqrt).
+static double two_norm(idx_t n, const double* v) { return std::sqrt(sq_reduction<4>(n, v)); }
+// unary_dot: mtl/operation/unary_dot.hpp:56-59 (reduction<8>).
+static double unary_dot(idx_t n, const double* v) { return sq_reduction<8>(n, v); }
+
+// ---------------------------------------------------------------------------
+// SpMV: mtl/operation/mat_vec_mult.hpp:619-689 (row-major compressed2D).
+// Each row is summed sequentially in stored order starting from zero; the
+// 4-row blocking only interleaves independent rows, so it does not change any
+// result.  Assign::first_update: '=' stores, '+=' adds, '-=' subtracts
+// (mtl/operation/assign_mode.hpp).  The very-sparse / adaptive variants
+// (:421-449, :551-617) compute the same per-row sums.
+// ---------------------------------------------------------------------------
+template <typename MV>
+static inline double row_sum(const idx_t* S, const idx_t* I, const MV* D, const double* v, idx_t r)
+{
+    double tmp = 0;
+    for (idx_t j = S[r]; j != S[r + 1]; ++j) tmp += D[j] * v[I[j]];
+    return tmp;
+}
+
+// ---------------------------------------------------------------------------
+// IC(0): itl/pc/ic_0.hpp:135-186.  U = upper(A); for each row k:
+// U_kk <- 1/sqrt(U_kk); for each i > k in row k: U_ki <- U_ki * U_kk and
+// U_ii <- U_ii - U_ki * U[k][i] (the only j in (k, i] of row i is j = i).
+// ---------------------------------------------------------------------------
+static idx_t lookup(const idx_t* S, const idx_t* I, idx_t r, idx_t c)
+{
+    // compressed2D::operator() finds (r, c) by search; rows are sorted.
+    // Returns the storage index or -1 (value 0), compressed2D.hpp:181-204, 536-547.
+    const idx_t* b = I + S[r];
+    const idx_t* e = I + S[r + 1];
+    const idx_t* p = std::lower_bound(b, e, c);
+    if (p != e && *p == c) return p - I;
+    return -1;
+}
+
+extern "C" {
+
+// ------------------------------- generators --------------------------------
+
+// mtl/matrix/laplacian_setup.hpp:25-44 on an m x n grid, row = i*n + j.
+// The inserter keeps each row sorted by column (compressed2D.hpp:945-986).
+idx_t orc_laplacian_nnz(int m, int n)
+{
+    idx_t nnz = 0;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++)
+            nnz += 1 + (j < n - 1) + (i < m - 1) + (j > 0) + (i > 0);
+    return nnz;
+}
+
+void orc_laplacian_setup(int m, int n, idx_t* S, idx_t* I, double* D)
+{
+    idx_t pos = 0;
+    S[0] = 0;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) {
+            idx_t row = (idx_t)i * n + j;
+            // sorted column order of the five inserted entries
+            if (i > 0)     { I[pos] = row - n; D[pos++] = -1.0; }
+            if (j > 0)     { I[pos] = row - 1; D[pos++] = -1.0; }
+            I[pos] = row; D[pos++] = 4.0;
+            if (j < n - 1) { I[pos] = row + 1; D[pos++] = -1.0; }
+            if (i < m - 1) { I[pos] = row + n; D[pos++] = -1.0; }
+            S[row + 1] = pos;
+        }
+}
+
+// ------------------------------- kernels -----------------------------------
+
+// w = A*v (mode 0), w += A*v (mode 1), w -= A*v (mode 2)
+void orc_spmv(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+              const double* v, double* w, int mode)
+{
+    for (idx_t r = 0; r < n; r++) {
+        double tmp = row_sum(S, I, D, v, r);
+        if (mode == 0) w[r] = tmp;
+        else if (mode == 1) w[r] += tmp;
+        else w[r] -= tmp;
+    }
+}
+
+// r = b - A*x: compute_summand materialises A*x into a temporary
+// (mtl/operation/compute_summand.hpp:42-60), then element-wise minus.
+void orc_residual(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+                  const double* x, const double* b, double* r)
+{
+    for (idx_t i = 0; i < n; i++) r[i] = b[i] - row_sum(S, I, D, x, i);
+}
+
+double orc_dot(idx_t n, const double* a, const double* b) { return dot4(n, a, b); }
+double orc_two_norm(idx_t n, const double* v) { return two_norm(n, v); }
+double orc_unary_dot(idx_t n, const double* v) { return unary_dot(n, v); }
+
+// (lazy(q) = A*p) || (lazy(a) = lazy_dot(p, q)): fused_expr.hpp:56-77 unrolled
+// (row_mat_cvec_index_evaluator :35-44; dot_index_evaluator :28-39): partial
+// i%4, tail into partial 0, result tmp[0]+tmp[1]+tmp[2]+tmp[3] left to right.
+double orc_fused_spmv_dot(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+                          const double* p, double* q)
+{
+    double t[4] = {0, 0, 0, 0};
+    idx_t sb = n >> 2 << 2;
+    for (idx_t i = 0; i < sb; i += 4)
+        for (int k = 0; k < 4; k++) {
+            q[i + k] = row_sum(S, I, D, p, i + k);
+            t[k] += p[i + k] * q[i + k];
+        }
+    for (idx_t i = sb; i < n; i++) {
+        q[i] = row_sum(S, I, D, p, i);
+        t[0] += p[i] * q[i];
+    }
+    return t[0] + t[1] + t[2] + t[3];
+}
+
+// (lazy(r) -= alpha*q) || (lazy(rr) = lazy_unary_dot(r)): vec_vec_aop_expr
+// :167-171 and reduction_index_evaluator :29-41: partials i%4, tail into
+// partial 0, result (t0+t1)+(t2+t3).
+double orc_fused_axpy_unary_dot(idx_t n, double alpha, const double* q, double* r)
+{
+    double t[4] = {0, 0, 0, 0};
+    idx_t sb = n >> 2 << 2;
+    for (idx_t i = 0; i < sb; i += 4)
+        for (int k = 0; k < 4; k++) {
+            r[i + k] -= alpha * q[i + k];
+            t[k] += r[i + k] * r[i + k];
+        }
+    for (idx_t i = sb; i < n; i++) {
+        r[i] -= alpha * q[i];
+        t[0] += r[i] * r[i];
+    }
+    return (t[0] + t[1]) + (t[2] + t[3]);
+}
+
+// ------------------------------- IC(0) -------------------------------------
+
+// Number of entries of upper(A) (U = upper(A), ic_0.hpp:148).
+idx_t orc_upper_nnz(idx_t n, const idx_t* S, const idx_t* I)
+{
+    idx_t c = 0;
+    for (idx_t r = 0; r < n; r++)
+        for (idx_t j = S[r]; j < S[r + 1]; j++) c += (I[j] >= r);
+    return c;
+}
+
+// ic_0::factorizer::factorize (ic_0.hpp:135-186) in double; U gets the
+// inverted diagonal first in each row.  Returns ORC_ERR_MISSING_DIAGONAL when
+// a row of upper(A) does not start with its diagonal (:157).
+int orc_ic0_factor(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+                   idx_t* US, idx_t* UI, double* UD)
+{
+    // U = upper(A)
+    idx_t pos = 0;
+    US[0] = 0;
+    for (idx_t r = 0; r < n; r++) {
+        for (idx_t j = S[r]; j < S[r + 1]; j++)
+            if (I[j] >= r) { UI[pos] = I[j]; UD[pos] = D[j]; pos++; }
+        US[r + 1] = pos;
+    }
+    for (idx_t k = 0; k < n; k++) {
+        idx_t ic = US[k], iend = US[k + 1];
+        if (ic == iend || UI[ic] != k) return ORC_ERR_MISSING_DIAGONAL;
+        double inv_dia = 1.0 / std::sqrt(UD[ic]);      // reciprocal(sqrt()) :160
+        UD[ic] = inv_dia;
+        for (++ic; ic != iend; ++ic) {
+            double d = UD[ic] * inv_dia;                // :166
+            UD[ic] = d;
+            idx_t i = UI[ic];
+            // row i of U: columns in (k, i] -> only the diagonal (i, i)
+            for (idx_t jc = US[i]; jc < US[i + 1]; jc++) {
+                idx_t j = UI[jc];
+                if (j <= k) continue;
+                if (j > i) break;
+                idx_t kj = lookup(US, UI, k, j);
+                double ukj = kj >= 0 ? UD[kj] : 0.0;    // U[k][j]
+                UD[jc] = UD[jc] - d * ukj;              // :181
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+// CSR transpose as done by the inserter copy of trans(U) (ic_0.hpp:57):
+// rows of the result are sorted by column.
+void orc_csr_transpose(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+                       idx_t* TS, idx_t* TI, double* TD)
+{
+    std::vector<idx_t> cnt(n + 1, 0);
+    for (idx_t j = 0; j < S[n]; j++) cnt[I[j] + 1]++;
+    for (idx_t r = 0; r < n; r++) cnt[r + 1] += cnt[r];
+    for (idx_t r = 0; r <= n; r++) TS[r] = cnt[r];
+    for (idx_t r = 0; r < n; r++)
+        for (idx_t j = S[r]; j < S[r + 1]; j++) {
+            idx_t c = I[j], p = cnt[c]++;
+            TI[p] = r;
+            TD[p] = D[j];
+        }
+}
+
+}  // extern "C"
+
+// Lower sweep, compact CSR, inverted diagonal stored last
+// (lower_trisolve.hpp:191-209, version 6).  FV = factor value type.
+template <typename FV>
+static int lower_v6(idx_t n, const idx_t* S, const idx_t* I, const FV* D, const double* v, double* w)
+{
+    for (idx_t r = 0; r < n; r++) {
+        idx_t j0 = S[r], j1 = S[r + 1];
+        if (j0 == j1) return ORC_ERR_MISSING_DIAGONAL;
+        --j1;
+        if (I[j1] != r) return ORC_ERR_MISSING_DIAGONAL;
+        FV dia = D[j1];
+        double rr = v[r];
+        for (; j0 != j1; ++j0) rr -= D[j0] * w[I[j0]];
+        w[r] = rr * dia;
+    }
+    return ORC_OK;
+}
+
+// Lower sweep, compact CSR, unit diagonal (lower_trisolve.hpp:172-189,
+// version 5): starts at ref_major[1], so row 0 is taken to be empty.
+template <typename FV>
+static void lower_v5(idx_t n, const idx_t* S, const idx_t* I, const FV* D, const double* v, double* w)
+{
+    if (n == 0) return;
+    idx_t j1 = S[1];
+    for (idx_t r = 0; r != n; ++r) {
+        idx_t j0 = j1;
+        j1 = S[r + 1];
+        double rr = v[r];
+        for (; j0 != j1; ++j0) rr -= D[j0] * w[I[j0]];
+        w[r] = rr;
+    }
+}
+
+// Upper sweep, compact CSR, inverted diagonal first
+// (upper_trisolve.hpp:159-176, version 3, with crs_row_init :151-156).
+template <typename FV>
+static void upper_v3(idx_t n, const idx_t* S, const idx_t* I, const FV* D, const double* v, double* w)
+{
+    for (idx_t r = n; r-- > 0;) {
+        idx_t j0 = S[r];
+        const idx_t cj1 = S[r + 1];
+        double rr = v[r], dia;
+        dia = D[j0++];
+        for (; j0 != cj1; ++j0) rr -= D[j0] * w[I[j0]];
+        w[r] = rr * dia;                      // row_update, inverse_diagonal :102
+    }
+}
+
+extern "C" {
+
+// ic_0::solve (ic_0.hpp:78-87): lower sweep on L = trans(U), then upper on U.
+int orc_ic0_solve(idx_t n, const idx_t* LS, const idx_t* LI, const double* LD,
+                  const idx_t* US, const idx_t* UI, const double* UD,
+                  const double* x, double* y)
+{
+    std::vector<double> y0(n);
+    int st = lower_v6(n, LS, LI, LD, x, y0.data());
+    if (st) return st;
+    upper_v3(n, US, UI, UD, y0.data(), y);
+    return ORC_OK;
+}
+
+// Same with factors rounded to float (ic_0<Matrix, float>: factorise in A's
+// type, then U = U_tmp converts, ic_0.hpp:122-130); sweeps accumulate in
+// the double output vector (lower_trisolve.hpp:202-207).
+int orc_ic0_solve_f32(idx_t n, const idx_t* LS, const idx_t* LI, const float* LD,
+                      const idx_t* US, const idx_t* UI, const float* UD,
+                      const double* x, double* y)
+{
+    std::vector<double> y0(n);
+    int st = lower_v6(n, LS, LI, LD, x, y0.data());
+    if (st) return st;
+    upper_v3(n, US, UI, UD, y0.data(), y);
+    return ORC_OK;
+}
+
+// ------------------------------- ILU(0) ------------------------------------
+
+idx_t orc_strict_lower_nnz(idx_t n, const idx_t* S, const idx_t* I)
+{
+    idx_t c = 0;
+    for (idx_t r = 0; r < n; r++)
+        for (idx_t j = S[r]; j < S[r + 1]; j++) c += (I[j] < r);
+    return c;
+}
+
+// ilu_0_factorizer::factorize (ilu_0.hpp:45-82) + invert_diagonal
+// (mtl/operation/invert_diagonal.hpp:47-54); L = strict_lower(LU) and
+// U = upper(LU).  LU[k][j] of an absent entry reads 0 (compressed2D.hpp:536-547).
+int orc_ilu0_factor(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+                    idx_t* LS, idx_t* LI, double* LD, idx_t* US, idx_t* UI, double* UD)
+{
+    idx_t nnz = S[n];
+    std::vector<double> LU(D, D + nnz);
+    std::vector<double> inv_dia(n, 0.0);
+    for (idx_t i = 0; i < n; i++) {
+        // a row without its diagonal would run the k-loop past k == i
+        if (lookup(S, I, i, i) < 0) return ORC_ERR_MISSING_DIAGONAL;
+        for (idx_t kc = S[i], kend = S[i + 1]; kc != kend; ++kc) {
+            idx_t k = I[kc];
+            if (k == i) break;
+            double aik = LU[kc] * inv_dia[k];                       // :70
+            LU[kc] = aik;
+            for (idx_t jc = kc + 1; jc != kend; ++jc) {
+                idx_t kj = lookup(S, I, k, I[jc]);
+                double lukj = kj >= 0 ? LU[kj] : 0.0;
+                LU[jc] = LU[jc] - aik * lukj;                        // :74
+            }
+        }
+        idx_t ii = lookup(S, I, i, i);
+        if (ii < 0) return ORC_ERR_MISSING_DIAGONAL;
+        inv_dia[i] = 1.0 / LU[ii];                                   // :77
+    }
+    for (idx_t i = 0; i < n; i++) {                                  // invert_diagonal
+        idx_t ii = lookup(S, I, i, i);
+        LU[ii] = 1.0 / LU[ii];
+    }
+    idx_t lp = 0, up = 0;
+    LS[0] = US[0] = 0;
+    for (idx_t r = 0; r < n; r++) {
+        for (idx_t j = S[r]; j < S[r + 1]; j++) {
+            if (I[j] < r) { LI[lp] = I[j]; LD[lp++] = LU[j]; }
+            else          { UI[up] = I[j]; UD[up++] = LU[j]; }
+        }
+        LS[r + 1] = lp;
+        US[r + 1] = up;
+    }
+    return ORC_OK;
+}
+
+// ilu::solve (ilu.hpp:83-91): unit lower sweep (v5) then upper (v3).
+void orc_ilu0_solve(idx_t n, const idx_t* LS, const idx_t* LI, const double* LD,
+                    const idx_t* US, const idx_t* UI, const double* UD,
+                    const double* x, double* y)
+{
+    std::vector<double> y0(n);
+    lower_v5(n, LS, LI, LD, x, y0.data());
+    upper_v3(n, US, UI, UD, y0.data(), y);
+}
+
+void orc_ilu0_solve_f32(idx_t n, const idx_t* LS, const idx_t* LI, const float* LD,
+                        const idx_t* US, const idx_t* UI, const float* UD,
+                        const double* x, double* y)
+{
+    std::vector<double> y0(n);
+    lower_v5(n, LS, LI, LD, x, y0.data());
+    upper_v3(n, US, UI, UD, y0.data(), y);
+}
+
+// ilu::adjoint_solve (ilu.hpp:105-115): lower sweep on adjoint(U) with the
+// inverted diagonal (column-major generic version 4, lower_trisolve.hpp:155-170)
+// followed by the unit upper sweep on adjoint(L) (column-major version 2,
+// upper_trisolve.hpp:131-149).  Both scatter along rows of U and L.
+void orc_ilu0_adjoint_solve(idx_t n, const idx_t* LS, const idx_t* LI, const double* LD,
+                            const idx_t* US, const idx_t* UI, const double* UD,
+                            const double* x, double* y)
+{
+    std::vector<double> w(x, x + n);
+    // adjoint(U) column r == row r of U: diagonal first, then entries c > r
+    for (idx_t r = 0; r < n; r++) {
+        idx_t j = US[r];
+        double rr = w[r] *= UD[j];
+        for (++j; j < US[r + 1]; ++j) w[UI[j]] -= UD[j] * rr;
+    }
+    // adjoint(L) column r == row r of L (strict lower, unit diagonal), r descending
+    for (idx_t i = 0; i < n; i++) y[i] = w[i];
+    for (idx_t r = n; r-- > 0;) {
+        double rr = y[r];
+        for (idx_t j = LS[r]; j < LS[r + 1]; ++j) y[LI[j]] -= LD[j] * rr;
+    }
+}
+
+// ------------------------------- solvers -----------------------------------
+
+// Preconditioner kinds for the drivers.
+enum { ORC_PC_IDENTITY = 0, ORC_PC_DIAGONAL = 1, ORC_PC_IC0 = 2, ORC_PC_ILU0 = 3 };
+
+struct orc_pc {
+    int kind;
+    idx_t n;
+    const double* inv_diag;                               // diagonal
+    const idx_t *LS, *LI; const double* LD;               // ic0: L = trans(U); ilu0: strict lower
+    const idx_t *US, *UI; const double* UD;
+};
+
+static void pc_solve(const orc_pc* P, const double* x, double* y)
+{
+    idx_t n = P->n;
+    switch (P->kind) {
+    case ORC_PC_IDENTITY: for (idx_t i = 0; i < n; i++) y[i] = x[i]; break;
+    case ORC_PC_DIAGONAL:                                // itl/pc/diagonal.hpp:60-67
+        for (idx_t i = 0; i < n; i++) y[i] = P->inv_diag[i] * x[i];
+        break;
+    case ORC_PC_IC0: orc_ic0_solve(n, P->LS, P->LI, P->LD, P->US, P->UI, P->UD, x, y); break;
+    case ORC_PC_ILU0: orc_ilu0_solve(n, P->LS, P->LI, P->LD, P->US, P->UI, P->UD, x, y); break;
+    }
+}
+
+// diagonal PC setup: inv_diag[i] = reciprocal(A[i][i]) (itl/pc/diagonal.hpp:36-45).
+void orc_diagonal_setup(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* inv_diag)
+{
+    for (idx_t i = 0; i < n; i++) {
+        idx_t p = lookup(S, I, i, i);
+        inv_diag[i] = 1.0 / (p >= 0 ? D[p] : 0.0);
+    }
+}
+
+// itl::cg: without PC (cg.hpp:39-72) when the PC is the identity (:80-82),
+// otherwise the left-preconditioned variant (:75-112).
+int orc_cg(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+           double* x, const double* b, const orc_pc* P, orc_iteration* it)
+{
+    std::vector<double> p(n), q(n), r(n), z(n);
+    orc_residual(n, S, I, D, x, b, r.data());              // r = b - A*x
+    if (P->kind == ORC_PC_IDENTITY) {
+        double rho = 0, rho_1 = 0, alpha = 0, alpha_1 = 0;
+        rho = dot4(n, r.data(), r.data());
+        while (!orc_finished(it, std::sqrt(std::fabs(rho)))) {
+            ++it->i;
+            if (it->i <= 1) for (idx_t k = 0; k < n; k++) p[k] = r[k];
+            else {
+                double s = rho / rho_1;
+                for (idx_t k = 0; k < n; k++) p[k] = r[k] + s * p[k];
+            }
+            alpha_1 = orc_fused_spmv_dot(n, S, I, D, p.data(), q.data());
+            alpha = rho / alpha_1;
+            for (idx_t k = 0; k < n; k++) x[k] += alpha * p[k];
+            rho_1 = rho;
+            rho = orc_fused_axpy_unary_dot(n, alpha, q.data(), r.data());
+        }
+        return it->error;
+    }
+    double rho = 0, rho_1 = 0, rr, alpha = 0, alpha_1;
+    rr = dot4(n, r.data(), r.data());                        // :94
+    while (!orc_finished(it, std::sqrt(std::fabs(rr)))) {    // :95
+        ++it->i;
+        pc_solve(P, r.data(), z.data());                     // :97 (not fused, Q8)
+        rho = dot4(n, r.data(), z.data());
+        if (it->i <= 1) for (idx_t k = 0; k < n; k++) p[k] = z[k];
+        else {
+            double s = rho / rho_1;                          // :102
+            for (idx_t k = 0; k < n; k++) p[k] = z[k] + s * p[k];
+        }
+        alpha_1 = orc_fused_spmv_dot(n, S, I, D, p.data(), q.data());   // :104
+        alpha = rho / alpha_1;
+        for (idx_t k = 0; k < n; k++) x[k] += alpha * p[k];  // :107
+        rho_1 = rho;
+        rr = orc_fused_axpy_unary_dot(n, alpha, q.data(), r.data());   // :109
+    }
+    return it->error;
+}
+
+// itl::bicgstab (bicgstab.hpp:27-78).
+int orc_bicgstab(idx_t n, const idx_t* S, const idx_t* I, const double* D,
+                 double* x, const double* b, const orc_pc* P, orc_iteration* it)
+{
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, gamma, omega = 0;
+    std::vector<double> p(n), phat(n), s(n), shat(n), t(n), v(n), r(n), rtilde(n);
+    orc_residual(n, S, I, D, x, b, r.data());
+    rtilde = r;
+    while (!orc_finished(it, two_norm(n, r.data()))) {
+        ++it->i;
+        rho_1 = dot4(n, rtilde.data(), r.data());
+        if (rho_1 == 0.0) return ORC_ERR_ORTHOGONALITY;
+        if (it->i <= 1) p = r;
+        else {
+            if (omega == 0.0) return ORC_ERR_ORTHOGONALITY;
+            beta = (rho_1 / rho_2) * (alpha / omega);
+            for (idx_t k = 0; k < n; k++) p[k] = r[k] + beta * (p[k] - omega * v[k]);
+        }
+        pc_solve(P, p.data(), phat.data());
+        orc_spmv(n, S, I, D, phat.data(), v.data(), 0);
+        gamma = dot4(n, rtilde.data(), v.data());
+        if (gamma == 0.0) return ORC_ERR_ORTHOGONALITY;
+        alpha = rho_1 / gamma;
+        for (idx_t k = 0; k < n; k++) s[k] = r[k] - alpha * v[k];
+        if (orc_finished(it, two_norm(n, s.data()))) {
+            for (idx_t k = 0; k < n; k++) x[k] += alpha * phat[k];
+            break;
+        }
+        pc_solve(P, s.data(), shat.data());
+        orc_spmv(n, S, I, D, shat.data(), t.data(), 0);
+        omega = dot4(n, t.data(), s.data()) / dot4(n, t.data(), t.data());
+        for (idx_t k = 0; k < n; k++) x[k] += omega * shat[k] + alpha * phat[k];
+        for (idx_t k = 0; k < n; k++) r[k] = s[k] - omega * t[k];
+        rho_2 = rho_1;
+    }
+    return it->error;
+}
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+"""ctypes view of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: used by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg only.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+
+PC_IDENTITY, PC_DIAGONAL, PC_IC0, PC_ILU0 = 0, 1, 2, 3
+OK, MAX_ITER, ERR_NOT_SQUARE, ERR_MISSING_DIAGONAL, ERR_ORTHOGONALITY = 0, 1, 2, 3, 4
+
+
+class Iteration(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+                ("norm_r0", ctypes.c_double), ("i", ctypes.c_int), ("error", ctypes.c_int),
+                ("is_finished", ctypes.c_int), ("resid", ctypes.c_double),
+                ("history", ctypes.POINTER(ctypes.c_double)), ("history_cap", ctypes.c_int)]
+
+
+class PC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int64), ("inv_diag", ctypes.c_void_p),
+                ("LS", ctypes.c_void_p), ("LI", ctypes.c_void_p), ("LD", ctypes.c_void_p),
+                ("US", ctypes.c_void_p), ("UI", ctypes.c_void_p), ("UD", ctypes.c_void_p)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "orc_laplacian_nnz": (_i64, [ctypes.c_int, ctypes.c_int]),
+        "orc_laplacian_setup": (None, [ctypes.c_int, ctypes.c_int, _i64p, _i64p, _f64p]),
+        "orc_spmv": (None, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.c_int]),
+        "orc_residual": (None, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, _f64p]),
+        "orc_dot": (ctypes.c_double, [_i64, _f64p, _f64p]),
+        "orc_two_norm": (ctypes.c_double, [_i64, _f64p]),
+        "orc_unary_dot": (ctypes.c_double, [_i64, _f64p]),
+        "orc_fused_spmv_dot": (ctypes.c_double, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p]),
+        "orc_fused_axpy_unary_dot": (ctypes.c_double, [_i64, ctypes.c_double, _f64p, _f64p]),
+        "orc_upper_nnz": (_i64, [_i64, _i64p, _i64p]),
+        "orc_strict_lower_nnz": (_i64, [_i64, _i64p, _i64p]),
+        "orc_ic0_factor": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p]),
+        "orc_csr_transpose": (None, [_i64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p]),
+        "orc_ic0_solve": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _f64p, _f64p]),
+        "orc_ic0_solve_f32": (ctypes.c_int, [_i64, _i64p, _i64p, _f32p, _i64p, _i64p, _f32p, _f64p, _f64p]),
+        "orc_ilu0_factor": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p]),
+        "orc_ilu0_solve": (None, [_i64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _f64p, _f64p]),
+        "orc_ilu0_solve_f32": (None, [_i64, _i64p, _i64p, _f32p, _i64p, _i64p, _f32p, _f64p, _f64p]),
+        "orc_ilu0_adjoint_solve": (None, [_i64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _f64p, _f64p]),
+        "orc_diagonal_setup": (None, [_i64, _i64p, _i64p, _f64p, _f64p]),
+        "orc_cg": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
+        "orc_bicgstab": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class CSR:
+    """Host CSR triple with int64 indices (MTL4's default size_type)."""
+
+    def __init__(self, starts, indices, data):
+        self.starts = np.ascontiguousarray(starts, dtype=np.int64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int64)
+        self.data = np.ascontiguousarray(data)
+        self.n = len(self.starts) - 1
+
+    @property
+    def nnz(self):
+        return int(self.starts[-1])
+
+    def dense(self):
+        A = np.zeros((self.n, self.n), dtype=np.float64)
+        for r in range(self.n):
+            for j in range(self.starts[r], self.starts[r + 1]):
+                A[r, self.indices[j]] = self.data[j]
+        return A
+
+
+def laplacian_setup(m, n):
+    nnz = lib.orc_laplacian_nnz(m, n)
+    S = np.zeros(m * n + 1, np.int64)
+    I = np.zeros(nnz, np.int64)
+    D = np.zeros(nnz, np.float64)
+    lib.orc_laplacian_setup(m, n, S, I, D)
+    return CSR(S, I, D)
+
+
+def spmv(A, v, w=None, mode=0):
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.zeros(A.n) if w is None else np.array(w, np.float64)
+    lib.orc_spmv(A.n, A.starts, A.indices, A.data, v, w, mode)
+    return w
+
+
+def residual(A, x, b):
+    r = np.zeros(A.n)
+    lib.orc_residual(A.n, A.starts, A.indices, A.data, np.ascontiguousarray(x, np.float64),
+                     np.ascontiguousarray(b, np.float64), r)
+    return r
+
+
+def dot(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    return lib.orc_dot(len(a), a, np.ascontiguousarray(b, np.float64))
+
+
+def two_norm(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return lib.orc_two_norm(len(a), a)
+
+
+def unary_dot(a):
+    a = np.ascontiguousarray(a, np.float64)
+    return lib.orc_unary_dot(len(a), a)
+
+
+def fused_spmv_dot(A, p):
+    q = np.zeros(A.n)
+    d = lib.orc_fused_spmv_dot(A.n, A.starts, A.indices, A.data, np.ascontiguousarray(p, np.float64), q)
+    return q, d
+
+
+def fused_axpy_unary_dot(alpha, q, r):
+    r = np.array(r, np.float64)
+    d = lib.orc_fused_axpy_unary_dot(len(r), alpha, np.ascontiguousarray(q, np.float64), r)
+    return r, d
+
+
+def csr_transpose(A):
+    TS = np.zeros(A.n + 1, np.int64)
+    TI = np.zeros(A.nnz, np.int64)
+    TD = np.zeros(A.nnz, np.float64)
+    lib.orc_csr_transpose(A.n, A.starts, A.indices, np.ascontiguousarray(A.data, np.float64), TS, TI, TD)
+    return CSR(TS, TI, TD)
+
+
+def ic0_factor(A):
+    """Returns (status, U, L) with U = IC(0) upper factor (inverted diagonal first), L = trans(U)."""
+    nu = lib.orc_upper_nnz(A.n, A.starts, A.indices)
+    US = np.zeros(A.n + 1, np.int64)
+    UI = np.zeros(nu, np.int64)
+    UD = np.zeros(nu, np.float64)
+    st = lib.orc_ic0_factor(A.n, A.starts, A.indices, A.data, US, UI, UD)
+    U = CSR(US, UI, UD)
+    return st, U, (csr_transpose(U) if st == OK else None)
+
+
+def ic0_solve(L, U, x, factor_dtype=np.float64):
+    y = np.zeros(L.n)
+    x = np.ascontiguousarray(x, np.float64)
+    if factor_dtype == np.float32:
+        st = lib.orc_ic0_solve_f32(L.n, L.starts, L.indices, L.data.astype(np.float32), U.starts, U.indices,
+                                   U.data.astype(np.float32), x, y)
+    else:
+        st = lib.orc_ic0_solve(L.n, L.starts, L.indices, L.data, U.starts, U.indices, U.data, x, y)
+    assert st == OK
+    return y
+
+
+def ilu0_factor(A):
+    nl = lib.orc_strict_lower_nnz(A.n, A.starts, A.indices)
+    nu = A.nnz - nl
+    LS = np.zeros(A.n + 1, np.int64); LI = np.zeros(nl, np.int64); LD = np.zeros(nl)
+    US = np.zeros(A.n + 1, np.int64); UI = np.zeros(nu, np.int64); UD = np.zeros(nu)
+    st = lib.orc_ilu0_factor(A.n, A.starts, A.indices, A.data, LS, LI, LD, US, UI, UD)
+    return st, CSR(LS, LI, LD), CSR(US, UI, UD)
+
+
+def ilu0_solve(L, U, x, factor_dtype=np.float64):
+    y = np.zeros(L.n)
+    x = np.ascontiguousarray(x, np.float64)
+    if factor_dtype == np.float32:
+        lib.orc_ilu0_solve_f32(L.n, L.starts, L.indices, L.data.astype(np.float32), U.starts, U.indices,
+                               U.data.astype(np.float32), x, y)
+    else:
+        lib.orc_ilu0_solve(L.n, L.starts, L.indices, L.data, U.starts, U.indices, U.data, x, y)
+    return y
+
+
+def ilu0_adjoint_solve(L, U, x):
+    y = np.zeros(L.n)
+    lib.orc_ilu0_adjoint_solve(L.n, L.starts, L.indices, L.data, U.starts, U.indices, U.data,
+                               np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def make_pc(A, kind):
+    """Builds an oracle PC; returns (PC struct, keepalive list)."""
+    P = PC()
+    P.kind = kind
+    P.n = A.n
+    keep = []
+    if kind == PC_DIAGONAL:
+        inv = np.zeros(A.n)
+        lib.orc_diagonal_setup(A.n, A.starts, A.indices, A.data, inv)
+        keep.append(inv)
+        P.inv_diag = inv.ctypes.data
+    elif kind in (PC_IC0, PC_ILU0):
+        if kind == PC_IC0:
+            st, U, L = ic0_factor(A)
+        else:
+            st, L, U = ilu0_factor(A)
+        assert st == OK, st
+        keep += [L, U]
+        P.LS, P.LI, P.LD = L.starts.ctypes.data, L.indices.ctypes.data, L.data.ctypes.data
+        P.US, P.UI, P.UD = U.starts.ctypes.data, U.indices.ctypes.data, U.data.ctypes.data
+    return P, keep
+
+
+def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0, norm_r0=None,
+          history=True):
+    """Runs itl::cg / itl::bicgstab on the oracle; returns dict(x, status, iterations, resid, history)."""
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
+    P, keep = make_pc(A, pc)
+    it = Iteration()
+    it.max_iter = max_iter
+    it.rtol, it.atol = rtol, atol
+    it.norm_r0 = abs(two_norm(b)) if norm_r0 is None else norm_r0
+    hist = np.full(max_iter + 2, np.nan)
+    if history:
+        it.history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        it.history_cap = len(hist)
+    f = lib.orc_cg if method == "cg" else lib.orc_bicgstab
+    st = f(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), ctypes.byref(it))
+    del keep
+    return dict(x=x, status=st, iterations=it.i, resid=it.resid, history=hist[: it.i + 1])

@@ -1,0 +1,331 @@
+"""Pins the CPU oracle (oracle/mtl4_oracle.cpp) against the reference's own
+known-answer tests and against what the mathematics fixes.  Each test names the
+reference test (libs/numeric/{itl,mtl}/test/...) or the identity it checks."""
+import numpy as np
+import pytest
+
+import oracle_lib as o
+
+
+def dense_ic0_textbook(A):
+    """dense_ic_0 of T-itl/ic_0_cg_test.cpp:17-40 (full textbook IC(0) on the
+    dense upper triangle, stored with the true diagonal)."""
+    U = np.triu(A).copy()
+    n = len(U)
+    for k in range(n):
+        dia = U[k, k] = np.sqrt(U[k, k])
+        for i in range(k + 1, n):
+            d = U[k, i] = U[k, i] / dia
+            for j in range(k + 1, i + 1):
+                if U[j, i] != 0.0:
+                    U[j, i] -= d * U[k, j]
+    return U
+
+
+def dense_ilu0_textbook(A):
+    """dense_ilu_0 of T-itl/ilu_0_test.cpp:31-50."""
+    LU = A.copy()
+    n = len(LU)
+    for i in range(1, n):
+        for k in range(i):
+            LU[i, k] /= LU[k, k]
+            for j in range(k + 1, n):
+                if LU[i, j] != 0:
+                    LU[i, j] -= LU[i, k] * LU[k, j]
+    return LU
+
+
+# ---------------------------------------------------------------- generators
+def test_laplacian_setup_stencil():
+    # T-mtl/laplacian_setup_test.cpp: 4 on the diagonal, -1 to E/S/W/N
+    A = o.laplacian_setup(3, 4).dense()
+    assert A.shape == (12, 12)
+    for r in range(12):
+        i, j = divmod(r, 4)
+        assert A[r, r] == 4
+        for (di, dj) in ((0, 1), (1, 0), (0, -1), (-1, 0)):
+            ii, jj = i + di, j + dj
+            if 0 <= ii < 3 and 0 <= jj < 4:
+                assert A[r, ii * 4 + jj] == -1
+    assert o.laplacian_setup(64, 64).nnz == 20224   # cfg1
+
+
+# ---------------------------------------------------------------- SpMV
+def test_matrix_vector_product_stencil():
+    # T-mtl/matrix_vector_product_test.cpp:56-89 on the 5x5 grid
+    A = o.laplacian_setup(5, 5)
+    v = A.dense()[12].copy()
+    w = o.spmv(A, v)
+    assert (w[12], w[13], w[14], w[15], w[17], w[18], w[22]) == (20, -8, 1, 0, -8, 2, 1)
+    w = o.spmv(A, v, w, mode=1)
+    assert w[12] == 40 and w[18] == 4
+    w = o.spmv(A, v, w, mode=2)
+    assert w[12] == 20 and w[18] == 2
+
+
+@pytest.mark.parametrize("m,n", [(7, 9), (16, 16)])
+def test_spmv_laplacian_eigenvectors(m, n):
+    # A v = lambda v for v = sin(pi i p/(m+1)) sin(pi j q/(n+1)), lambda = 4 - 2cos - 2cos
+    A = o.laplacian_setup(m, n)
+    i = np.arange(m)[:, None] + 1
+    j = np.arange(n)[None, :] + 1
+    for p, q in ((1, 1), (2, 3), (m, n)):
+        v = (np.sin(np.pi * i * p / (m + 1)) * np.sin(np.pi * j * q / (n + 1))).ravel()
+        lam = 4 - 2 * np.cos(np.pi * p / (m + 1)) - 2 * np.cos(np.pi * q / (n + 1))
+        np.testing.assert_allclose(o.spmv(A, v), lam * v, atol=1e-13)
+
+
+def test_residual_is_b_minus_Ax():
+    A = o.laplacian_setup(6, 5)
+    rng = np.random.default_rng(1)
+    x, b = rng.standard_normal(30), rng.standard_normal(30)
+    np.testing.assert_array_equal(o.residual(A, x, b), b - o.spmv(A, x))
+
+
+# ---------------------------------------------------------------- reductions
+def test_dot_known_answer():
+    # T-mtl/dot_test.cpp: dot(1..9, 1..9) = 285
+    u = np.arange(1, 10, dtype=float)
+    assert o.dot(u, u) == 285.0
+
+
+def test_unary_dot_and_two_norm():
+    # T-mtl/unary_dot_test.cpp: unary_dot(u) == dot(u, u) == two_norm(u)^2
+    u = np.arange(1, 12, dtype=float)
+    assert o.unary_dot(u) == o.dot(u, u) == 506.0
+    assert abs(o.two_norm(u) ** 2 - 506.0) < 1e-12
+
+
+def test_reduction_orders():
+    # dot<4> puts element i into partial i%4 and the tail into partial 0
+    # (dot.hpp:136-147): a cancellation pattern that only that order gets exactly right
+    a = np.array([1e16, 1.0, -1e16, 1.0, 3.0], dtype=float)
+    b = np.ones(5)
+    # ((1e16 + 3) + 1) + ((-1e16) + 1)
+    t0 = 1e16 + 3.0
+    exp = (t0 + 1.0) + (-1e16 + 1.0)
+    assert o.dot(a, b) == exp
+    # fused lazy_dot: ((t0 + t1) + t2) + t3 (dot_index_evaluator.hpp:30)
+    A = o.CSR(np.arange(6), np.arange(5), np.ones(5))
+    _, d = o.fused_spmv_dot(A, a)
+    sq = a * a
+    assert d == ((sq[0] + sq[4] + sq[1]) + sq[2]) + sq[3]
+
+
+def test_fuse_test_values():
+    # T-mtl/fuse_test.cpp:36-49: B = 2 I (6x6), v = 1 -> d = 12; r = 6, q = 2, alpha = 7.8 -> rho = 552.96
+    B = o.CSR(np.arange(7), np.arange(6), np.full(6, 2.0))
+    w, d = o.fused_spmv_dot(B, np.ones(6))
+    assert d == 12.0 and np.all(w == 2.0)
+    r, rho = o.fused_axpy_unary_dot(7.8, np.full(6, 2.0), np.full(6, 6.0))
+    assert abs(rho - 552.96) < 1e-9 and np.allclose(r, -9.6)
+
+
+# ---------------------------------------------------------------- IC(0)
+def test_ic0_golden_solve():
+    # T-itl/ic_0_test.cpp:22-36: ic_0<compressed2D<double>, float> on the 3x3 grid, solve(P, iota)
+    A = o.laplacian_setup(3, 3)
+    st, U, L = o.ic0_factor(A)
+    assert st == o.OK
+    y = o.ic0_solve(L, U, np.arange(9.0), factor_dtype=np.float32)
+    yc = np.array([1.03194, 1.60198, 1.45357, 2.5258, 3.55386, 3.16, 2.91787, 4.09338, 3.81335])
+    assert np.linalg.norm(y - yc) < 1e-4          # reference bound 1e-3; goldens carry 6 digits
+
+
+@pytest.mark.parametrize("m,n", [(3, 3), (4, 5), (8, 8)])
+def test_ic0_equals_dense_textbook_on_5pt(m, n):
+    # T-itl/ic_0_cg_test.cpp:17-40 compares U[2][3] and 1/U[3][3]; on 5-point
+    # grids (no triangles) every entry must agree
+    A = o.laplacian_setup(m, n)
+    st, U, L = o.ic0_factor(A)
+    Ud = U.dense()
+    Ut = dense_ic0_textbook(A.dense())
+    off = ~np.eye(A.n, dtype=bool) & (Ud != 0)
+    np.testing.assert_allclose(Ud[off], Ut[off], rtol=1e-14)
+    np.testing.assert_allclose(np.diag(Ud), 1.0 / np.diag(Ut), rtol=1e-14)
+
+
+def test_ic0_diagonal_recurrence_closed_form():
+    # D_i = a_ii - sum_{k in N-(i)} a_ki^2 / D_k for 5-point stencils
+    m = n = 6
+    A = o.laplacian_setup(m, n)
+    _, U, _ = o.ic0_factor(A)
+    Ad = A.dense()
+    D = np.zeros(A.n)
+    for i in range(A.n):
+        D[i] = Ad[i, i] - sum(Ad[k, i] ** 2 / D[k] for k in range(i) if Ad[k, i] != 0)
+    np.testing.assert_allclose(np.diag(U.dense()), 1 / np.sqrt(D), rtol=1e-14)
+
+
+def test_ic0_pattern_invariant():
+    # U^T U == A on the pattern of A (triangle-free graph); L is exactly trans(U)
+    A = o.laplacian_setup(7, 5)
+    _, U, L = o.ic0_factor(A)
+    Ud = U.dense().copy()
+    np.fill_diagonal(Ud, 1 / np.diag(Ud))          # true diagonal
+    M = Ud.T @ Ud
+    Ad = A.dense()
+    mask = Ad != 0
+    np.testing.assert_allclose(M[mask], Ad[mask], rtol=1e-13, atol=1e-14)
+    np.testing.assert_array_equal(L.dense(), U.dense().T)
+
+
+def test_ic0_trisolve_reference_matrix():
+    # the 5x5 matrix of T-mtl/inverse_trisolve_test.cpp:20-64: upper part
+    # {7, 8, 2, 4, 9} with (1,3)=2, (1,4)=3, diagonal inverted; recover x = 1..5
+    Ufull = np.diag([7.0, 8, 2, 4, 9])
+    Ufull[1, 3], Ufull[1, 4] = 2, 3
+    S, I, D = [0], [], []
+    for r in range(5):
+        for c in range(r, 5):
+            if Ufull[r, c] != 0:
+                I.append(c)
+                D.append(1 / Ufull[r, c] if r == c else Ufull[r, c])
+        S.append(len(I))
+    U = o.CSR(np.array(S), np.array(I), np.array(D))
+    L = o.csr_transpose(U)
+    x = np.arange(1.0, 6.0)
+    b = Ufull.T @ (Ufull @ x)
+    np.testing.assert_allclose(o.ic0_solve(L, U, b), x, rtol=1e-14)
+
+
+def test_ic0_missing_diagonal():
+    A = o.CSR(np.array([0, 1, 2]), np.array([1, 0]), np.array([1.0, 1.0]))
+    st, _, _ = o.ic0_factor(A)
+    assert st == o.ERR_MISSING_DIAGONAL
+
+
+# ---------------------------------------------------------------- ILU(0)
+def _rand_sparse(n, seed, extra=2):
+    rng = np.random.default_rng(seed)
+    A = np.eye(n)
+    for _ in range(extra * n):       # like T-itl/ilu_0_test.cpp:16-28 (A = 1; A[r][c] -= 1; A[r][r] += 1)
+        r, c = rng.integers(n), rng.integers(n)
+        A[r, c] -= 1
+        A[r, r] += 1
+    S, I, V = [0], [], []
+    for r in range(n):
+        for c in range(n):
+            if A[r, c] != 0 or r == c:
+                I.append(c)
+                V.append(A[r, c])
+        S.append(len(I))
+    return o.CSR(np.array(S), np.array(I), np.array(V)), A
+
+
+@pytest.mark.parametrize("case", ["grid3", "rand3", "rand12", "rand30"])
+def test_ilu0_equals_dense_textbook(case):
+    # T-itl/ilu_0_test.cpp:31-50 and ilu_0_bicgstab_test.cpp: sparse ILU(0) == dense textbook ILU(0) on the pattern
+    if case == "grid3":
+        A = o.laplacian_setup(3, 3)
+        Ad = A.dense()
+    else:
+        A, Ad = _rand_sparse(int(case[4:]), seed=len(case) + int(case[4:]))
+    st, L, U = o.ilu0_factor(A)
+    assert st == o.OK
+    LUt = dense_ilu0_textbook(Ad)
+    Ld, Ud = L.dense(), U.dense()
+    mask = (Ad != 0)
+    lo = np.tril(mask, -1)
+    np.testing.assert_allclose(Ld[lo], LUt[lo], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(np.diag(Ud), 1 / np.diag(LUt), rtol=1e-12)
+    up = np.triu(mask, 1)
+    np.testing.assert_allclose(Ud[up], LUt[up], rtol=1e-12, atol=1e-14)
+
+
+def test_ilu0_pattern_invariant_and_solves():
+    # (I + L) U == A on A's pattern; solve and adjoint_solve invert (I+L)U and its transpose
+    A, Ad = _rand_sparse(20, seed=5)
+    _, L, U = o.ilu0_factor(A)
+    Lm = L.dense() + np.eye(A.n)
+    Um = U.dense().copy()
+    np.fill_diagonal(Um, 1 / np.diag(Um))
+    P = Lm @ Um
+    mask = Ad != 0
+    np.testing.assert_allclose(P[mask], Ad[mask], rtol=1e-12, atol=1e-12)
+    x = np.linspace(-1, 2, A.n)
+    np.testing.assert_allclose(o.ilu0_solve(L, U, P @ x), x, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(o.ilu0_adjoint_solve(L, U, P.T @ x), x, rtol=1e-10, atol=1e-10)
+
+
+def test_ilu0_is_exact_lu_without_fill():
+    # tridiagonal: ILU(0) == LU, so the preconditioned CG converges in one iteration
+    n = 12
+    S, I, V = [0], [], []
+    for r in range(n):
+        for c in (r - 1, r, r + 1):
+            if 0 <= c < n:
+                I.append(c)
+                V.append(2.0 if c == r else -1.0)
+        S.append(len(I))
+    A = o.CSR(np.array(S), np.array(I), np.array(V))
+    b = o.spmv(A, np.ones(n))
+    res = o.solve("cg", A, b, pc=o.PC_ILU0, max_iter=n, rtol=1e-12)
+    assert res["status"] == o.OK and res["iterations"] == 1
+
+
+# ---------------------------------------------------------------- solvers
+def test_ilu0_cg_iteration_pin():
+    # T-itl/ilu_0_cg_test.cpp:22-33: CG + ILU(0) on the 3x3 grid reaches 1e-6 in <= 4 iterations
+    A = o.laplacian_setup(3, 3)
+    b = o.spmv(A, np.ones(9))
+    res = o.solve("cg", A, b, pc=o.PC_ILU0, max_iter=9, rtol=1e-6)
+    assert res["status"] == o.OK and res["iterations"] <= 4
+
+
+def test_absolute_termination_criterion():
+    # T-itl/absolut_termination_criterion_test.cpp:21-39: diagonal PC, rtol 0, atol 1e-6
+    A = o.laplacian_setup(4, 4)
+    b = o.spmv(A, np.ones(16))
+    res = o.solve("cg", A, b, pc=o.PC_DIAGONAL, max_iter=500, rtol=0.0, atol=1e-6)
+    assert res["status"] == o.OK
+    assert o.two_norm(o.residual(A, res["x"], b)) <= 1e-6
+
+
+@pytest.mark.parametrize("method,pc", [("cg", o.PC_IC0), ("cg", o.PC_IDENTITY), ("bicgstab", o.PC_ILU0),
+                                       ("bicgstab", o.PC_DIAGONAL)])
+def test_krylov_krylov_test_grid(method, pc):
+    # T-itl/ic_0_krylov_test.cpp:20-66: 4x4 grid, 1e-4 within 150 iterations
+    A = o.laplacian_setup(4, 4)
+    x_true = np.linspace(-1, 1, 16)
+    b = o.spmv(A, x_true)
+    res = o.solve(method, A, b, pc=pc, max_iter=150, rtol=1e-4)
+    assert res["status"] == o.OK
+    assert o.two_norm(o.residual(A, res["x"], b)) <= 1e-4 * o.two_norm(b) * 10
+
+
+def test_cg_exact_arithmetic_bound_and_history():
+    # CG converges in at most n iterations; history[0] = |b|, recurrence residual decreases to tol
+    A = o.laplacian_setup(5, 5)
+    b = o.spmv(A, np.linspace(0, 1, 25))
+    res = o.solve("cg", A, b, pc=o.PC_IDENTITY, max_iter=100, rtol=1e-12)
+    assert res["status"] == o.OK and res["iterations"] <= 25
+    assert res["history"][0] == pytest.approx(o.two_norm(b))
+    assert res["history"][-1] <= 1e-12 * o.two_norm(b)
+
+
+def test_max_iterations_error():
+    A = o.laplacian_setup(8, 8)
+    b = o.spmv(A, np.ones(64))
+    res = o.solve("cg", A, b, pc=o.PC_IDENTITY, max_iter=3, rtol=1e-12)
+    assert res["status"] == o.MAX_ITER and res["iterations"] == 3
+
+
+def test_bicgstab_orthogonality_breakdown():
+    # r = b - A x = 0 with tolerances that no residual meets: the loop is
+    # entered and rho_1 = dot(rtilde, r) = 0 throws (bicgstab.hpp:46)
+    A = o.laplacian_setup(3, 3)
+    res = o.solve("bicgstab", A, np.zeros(9), pc=o.PC_IDENTITY, max_iter=10, rtol=-1.0, norm_r0=1.0, atol=-1.0)
+    assert res["status"] == o.ERR_ORTHOGONALITY
+
+
+def test_oracle_plant_a_mistake_is_detected():
+    # the golden vector must reject a factor with a dropped Schur term
+    A = o.laplacian_setup(3, 3)
+    _, U, L = o.ic0_factor(A)
+    bad = o.CSR(U.starts, U.indices, U.data.copy())
+    bad.data[bad.starts[4]] *= 1.01           # perturb one inverted diagonal
+    Lb = o.csr_transpose(bad)
+    y = o.ic0_solve(Lb, bad, np.arange(9.0), factor_dtype=np.float32)
+    yc = np.array([1.03194, 1.60198, 1.45357, 2.5258, 3.55386, 3.16, 2.91787, 4.09338, 3.81335])
+    assert np.linalg.norm(y - yc) > 1e-3
