@@ -112,19 +112,22 @@ struct SweepArgs {
     int64_t spin_limit;
 };
 
+// Waits for an exported value.  Bounded: after `limit` polls the wait gives up
+// and raises the fault flag, and every later wait of any workgroup returns at
+// once, so a schedule fault ends the kernel instead of hanging it.
 template <typename VT>
 __device__ inline VT exchange_wait(const VT* p, int* fault, int64_t limit)
 {
-    int64_t it = 0;
-    while (true) {
-        VT v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!Sentinel<VT>::is(v)) return v;
-        if (++it > limit) {
-            __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return v;
-        }
+    VT v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!Sentinel<VT>::is(v)) return v;
+    for (int64_t it = 0; it < limit; it++) {
+        if (__hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return v;
         __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!Sentinel<VT>::is(v)) return v;
     }
+    __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
 }
 
 template <typename FV, typename VT, bool UNIT>
@@ -303,7 +306,7 @@ static int64_t spin_limit()
 {
     static int64_t lim = [] {
         const char* e = std::getenv("M4G_SPIN_LIMIT");
-        return e ? std::max<int64_t>(1, std::atoll(e)) : int64_t(1) << 24;
+        return e ? std::max<int64_t>(1, std::atoll(e)) : int64_t(1) << 20;   // about one second
     }();
     return lim;
 }
@@ -331,12 +334,14 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
     const FV* dg = std::is_same<FV, double>::value ? (const FV*)d.diag64.ptr : (const FV*)d.diag32.ptr;
     size_t lds = size_t(p.max_tile_rows) * sizeof(VT);
     if (p.ntiles == 0) return;
-    if (p.unit)
-        hipLaunchKernelGGL((k_sweep<FV, VT, true>), dim3(p.ntiles), dim3(p.block), lds, s, a, ev, dg, v, y,
-                           (VT*)d.xbuf.ptr, (VT*)other.xbuf.ptr, skip);
-    else
-        hipLaunchKernelGGL((k_sweep<FV, VT, false>), dim3(p.ntiles), dim3(p.block), lds, s, a, ev, dg, v, y,
-                           (VT*)d.xbuf.ptr, (VT*)other.xbuf.ptr, skip);
+    auto kern = p.unit ? k_sweep<FV, VT, true> : k_sweep<FV, VT, false>;
+    if (lds > 65536 - 64) {
+        // more than 64 KB of dynamic LDS must be requested explicitly
+        M4G_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    }
+    hipLaunchKernelGGL(kern, dim3(p.ntiles), dim3(p.block), lds, s, a, ev, dg, v, y, (VT*)d.xbuf.ptr,
+                       (VT*)other.xbuf.ptr, skip);
     M4G_HIP(hipGetLastError());
 }
 
@@ -379,9 +384,20 @@ static double env_double(const char* name, double dflt)
 
 int64_t sweep_tile_cap(int vtype)
 {
+    // the tile's vector entries live in LDS: stay within the per-workgroup limit
+    // (minus the kernel's static shared words)
+    int lds = 65536;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            lds = std::min(v, 160 * 1024);
+    }
+    const int64_t elem = vtype == M4G_F64 ? 8 : 4;
+    int64_t cap = std::min<int64_t>(vtype == M4G_F64 ? 8192 : 16384, (lds - 256) / elem);
     const char* e = std::getenv("M4G_TILE_ROWS");
-    if (e) return std::max<int64_t>(64, std::atoll(e));
-    return vtype == M4G_F64 ? 8192 : 16384;   // 64 KB of LDS per workgroup
+    if (e) cap = std::min<int64_t>(cap, std::max<int64_t>(64, std::atoll(e)));
+    return cap;
 }
 
 SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap)
