@@ -175,6 +175,40 @@ int m4g_cg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* it
 /* itl::bicgstab(A, x, b, M, iter) (bicgstab.hpp:27-78). */
 int m4g_bicgstab(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* iter);
 
+/* ---- differentiation of x = A(theta)^{-1} b --------------------------------- */
+/* No reference counterpart.  The adjoint of the solve is A^T lambda = dL/dx,
+ * which runs through the same Krylov kernels on the transposed matrix (or on A
+ * itself when it is symmetric); the gradient on the stored pattern is then
+ * dL/dA_ij = -lambda_i * x_j. */
+/* Re-runs the numeric factorisation of P for A's current values (same pattern
+ * as at creation); the sweep schedules are reused. */
+int m4g_pc_refactor(m4g_pc P, m4g_matrix A);
+/* AT = trans(A) as a new device matrix (rows sorted). */
+int m4g_matrix_transpose(m4g_matrix A, m4g_matrix* AT);
+/* grad[p] = -lambda[row(p)] * x[col(p)] for every stored entry p of A (CSR order);
+ * grad is a vector of length nnz(A). */
+int m4g_pattern_gradient(m4g_matrix A, m4g_vector lambda, m4g_vector x, m4g_vector grad);
+
+/* ---- mixed precision: FP32 inner solves inside FP64 iterative refinement ---- */
+/* No reference counterpart (the reference's "mixed precision" is FP32 factor
+ * storage with FP64 arithmetic, ic_0.hpp:122-130, which m4g_pc_create with
+ * M4G_F32 factors and double vectors reproduces).  Here the matrix values, the
+ * factors of P (created with M4G_F32) and the inner Krylov vectors are FP32
+ * with FP32 accumulation; FP64 is used only for r = b - A*x, the update
+ * x += d and the convergence norms:
+ *     repeat: r = b - A x (FP64); finished(|r|)?; solve A d = r in FP32 with
+ *             `method` + P to inner_rtol relative to |r|; x += d (FP64)
+ * `outer` is the iteration control of the refinement loop (its history holds
+ * |b - A x| per refinement step). */
+typedef struct {
+    int outer_iterations;   /* refinement steps taken */
+    int inner_iterations;   /* FP32 Krylov iterations, summed */
+    double resid;           /* final |b - A x| in FP64 */
+    int inner_error;        /* last inner solve's error code */
+} m4g_refine_info;
+int m4g_refined_solve(int method, m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* outer,
+                      double inner_rtol, int inner_max_iter, m4g_refine_info* info);
+
 #ifdef __cplusplus
 }
 #endif

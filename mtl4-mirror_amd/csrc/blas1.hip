@@ -1,8 +1,9 @@
 // Vector kernels of the CG / BiCGStab loops: dot, squared norm, axpy, xpay,
 // fused axpy + squared norm, fill and precision conversion.  All are
-// HBM-bound streams; reductions are deterministic two-stage sums (a fixed
-// partition into blocks, a fixed butterfly inside each block, and the last
-// block adding the block partials in order).
+// HBM-bound streams; reductions are deterministic two-stage sums: a fixed
+// partition into workgroups with a fixed butterfly inside each, then a
+// one-workgroup kernel adding the partials in workgroup order (kernel
+// ordering makes the partials visible without device-scope fences).
 #include "m4g_internal.hpp"
 #include "m4g_device.hpp"
 
@@ -26,8 +27,7 @@ __global__ __launch_bounds__(kNT) void k_dot(int64_t n, const double* __restrict
     for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
         acc += x[i] * y[i];
     acc = block_sum<kNT>(acc, scratch);
-    double total;
-    if (last_block_total<kNT>(acc, partials, counter, &total, scratch)) *out = total;
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
 __global__ __launch_bounds__(kNT) void k_sumsq(int64_t n, const double* __restrict__ x, double* partials,
@@ -38,8 +38,7 @@ __global__ __launch_bounds__(kNT) void k_sumsq(int64_t n, const double* __restri
     for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kNT)
         acc += x[i] * x[i];
     acc = block_sum<kNT>(acc, scratch);
-    double total;
-    if (last_block_total<kNT>(acc, partials, counter, &total, scratch)) *out = total;
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
 __global__ __launch_bounds__(kNT) void k_axpy_sumsq(int64_t n, double alpha, const double* __restrict__ q,
@@ -54,8 +53,7 @@ __global__ __launch_bounds__(kNT) void k_axpy_sumsq(int64_t n, double alpha, con
         acc += ri * ri;
     }
     acc = block_sum<kNT>(acc, scratch);
-    double total;
-    if (last_block_total<kNT>(acc, partials, counter, &total, scratch)) *out = total;
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
 __global__ __launch_bounds__(kNT) void k_axpy(int64_t n, double alpha, const double* __restrict__ x,
@@ -85,6 +83,20 @@ __global__ __launch_bounds__(kNT) void k_convert(int64_t n, const S* __restrict_
         dst[i] = static_cast<D>(src[i]);
 }
 
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partials, int nblocks, double* out)
+{
+    __shared__ double scratch[256 / kWave];
+    double acc = 0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) acc += partials[b];
+    acc = block_sum<256>(acc, scratch);
+    if (threadIdx.x == 0) *out = acc;
+}
+
+void launch_sum_partials(const double* partials, int nblocks, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, nblocks, out);
+}
+
 static int stream_grid(int64_t n)
 {
     int64_t g = (n + kNT - 1) / kNT;
@@ -97,6 +109,7 @@ void launch_dot(int64_t n, const double* x, const double* y, double* out, hipStr
 {
     ReduceWs& ws = reduce_ws();
     hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, y, ws.partials, ws.counters, out);
+    launch_sum_partials(ws.partials, reduce_grid(n), out, s);
     M4G_HIP(hipGetLastError());
 }
 
@@ -104,6 +117,7 @@ void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s)
 {
     ReduceWs& ws = reduce_ws();
     hipLaunchKernelGGL(k_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, ws.partials, ws.counters, out);
+    launch_sum_partials(ws.partials, reduce_grid(n), out, s);
     M4G_HIP(hipGetLastError());
 }
 
@@ -112,6 +126,7 @@ void launch_axpy_sumsq(int64_t n, double alpha, const double* q, double* r, doub
     ReduceWs& ws = reduce_ws();
     hipLaunchKernelGGL(k_axpy_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, alpha, q, r, ws.partials,
                        ws.counters, out);
+    launch_sum_partials(ws.partials, reduce_grid(n), out, s);
     M4G_HIP(hipGetLastError());
 }
 

@@ -71,31 +71,85 @@ struct Sums {
     double v[K];
 };
 
+// Each reduction kernel stores one partial per workgroup with plain stores;
+// a one-workgroup finalize kernel (ordered after it on the stream, so the
+// partials are visible without device-scope fences) adds them in workgroup
+// order and applies the method's scalar update.
 template <int NT, int K>
-__device__ inline bool reduce_last(Sums<K>& s, double* partials, unsigned* counter, Sums<K>& tot)
+__device__ inline void store_partials(Sums<K>& s, double* partials)
 {
     __shared__ double scratch[NT / kWave];
-    __shared__ bool am_last;
 #pragma unroll
-    for (int k = 0; k < K; k++) s.v[k] = block_sum<NT>(s.v[k], scratch);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) partials[k * gridDim.x + blockIdx.x] = s.v[k];
-        __threadfence();
-        am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    for (int k = 0; k < K; k++) {
+        const double v = block_sum<NT>(s.v[k], scratch);
+        if (threadIdx.x == 0) partials[k * gridDim.x + blockIdx.x] = v;
     }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
+}
+
+enum FinOp {
+    FIN_CG_START,      // rr = r.r; finished(sqrt|rr|)             (cg.hpp:54-55, 94-95)
+    FIN_CG_RHO,        // rho = r.z                                 (cg.hpp:97)
+    FIN_CG_ALPHA,      // alpha_1 = p.q; alpha = rho / alpha_1      (cg.hpp:104-105)
+    FIN_CG_UPDATE,     // rho_1 = rho; rr = r.r; finished(sqrt|rr|) (cg.hpp:108-109, 95)
+    FIN_BICG_START,    // finished(|r|); rho_1 = rtilde.r; tests    (bicgstab.hpp:43-53)
+    FIN_BICG_GAMMA,    // gamma = rtilde.v; alpha = rho_1 / gamma   (bicgstab.hpp:58-61)
+    FIN_BICG_S,        // finished(|s|) -> early exit               (bicgstab.hpp:64)
+    FIN_BICG_OMEGA,    // omega = t.s / t.t                         (bicgstab.hpp:70)
+    FIN_BICG_UPDATE    // rho_2 = rho_1; then the loop top          (bicgstab.hpp:75, 43-53)
+};
+
+__device__ inline void bicg_top(KState* st, double rr, double rtr);
+
+template <int K, int OP>
+__global__ __launch_bounds__(256) void k_fin(const double* __restrict__ partials, int nblocks, KState* st,
+                                             int identity_pc)
+{
+    if (OP != FIN_CG_START && OP != FIN_BICG_START && st->done) return;   // the producer skipped too
+    __shared__ double scratch[256 / kWave];
+    double tot[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double acc = 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += NT)
-            acc += __hip_atomic_load(partials + k * gridDim.x + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tot.v[k] = block_sum<NT>(acc, scratch);
+        for (int b = threadIdx.x; b < nblocks; b += 256) acc += partials[k * nblocks + b];
+        tot[k] = block_sum<256>(acc, scratch);
     }
-    if (threadIdx.x == 0) *counter = 0;
-    return threadIdx.x == 0;
+    if (threadIdx.x != 0) return;
+    switch (OP) {
+    case FIN_CG_START:
+        st->rr = tot[0];
+        st->rho = tot[0];
+        top_check(st, sqrt(fabs(tot[0])));
+        break;
+    case FIN_CG_RHO: st->rho = tot[0]; break;
+    case FIN_CG_ALPHA:
+        st->alpha_1 = tot[0];
+        st->alpha = st->rho / st->alpha_1;
+        break;
+    case FIN_CG_UPDATE:
+        st->rho_1 = st->rho;
+        st->rr = tot[0];
+        if (identity_pc) st->rho = tot[0];
+        top_check(st, sqrt(fabs(tot[0])));
+        break;
+    case FIN_BICG_START: bicg_top(st, tot[0], tot[K - 1]); break;
+    case FIN_BICG_GAMMA:
+        st->gamma = tot[0];
+        if (st->gamma == 0.0) {
+            st->error = M4G_ERR_ORTHOGONALITY;
+            st->done = 1;
+            break;
+        }
+        st->alpha = st->rho_1 / st->gamma;
+        break;
+    case FIN_BICG_S:
+        if (finished(st, sqrt(tot[0]))) st->mid_exit = 1;
+        break;
+    case FIN_BICG_OMEGA: st->omega = tot[0] / tot[K - 1]; break;
+    case FIN_BICG_UPDATE:
+        st->rho_2 = st->rho_1;
+        bicg_top(st, tot[0], tot[K - 1]);
+        break;
+    }
 }
 
 #define GRID_STRIDE(i, n) for (int64_t i = blockIdx.x * int64_t(kNT) + threadIdx.x; i < (n); i += int64_t(gridDim.x) * kNT)
@@ -110,12 +164,7 @@ __global__ __launch_bounds__(kNT) void k_cg_start(int64_t n, const T* __restrict
 {
     Sums<1> s{{0}};
     GRID_STRIDE(i, n) s.v[0] += double(r[i]) * double(r[i]);
-    Sums<1> t;
-    if (reduce_last<kNT, 1>(s, partials, counter, t)) {
-        st->rr = t.v[0];
-        st->rho = t.v[0];
-        top_check(st, sqrt(fabs(t.v[0])));
-    }
+    store_partials<kNT, 1>(s, partials);
 }
 
 template <typename T>
@@ -135,8 +184,7 @@ __global__ __launch_bounds__(kNT) void k_cg_rho(int64_t n, const T* __restrict__
     T acc = 0;
     GRID_STRIDE(i, n) acc += r[i] * z[i];
     s.v[0] = double(acc);
-    Sums<1> t;
-    if (reduce_last<kNT, 1>(s, partials, counter, t)) st->rho = t.v[0];
+    store_partials<kNT, 1>(s, partials);
 }
 
 // diagonal PC fused with the dot: z = inv_diag .* r; rho = r.z
@@ -154,8 +202,7 @@ __global__ __launch_bounds__(kNT) void k_cg_diag_rho(int64_t n, const double* __
         acc += r[i] * zi;
     }
     s.v[0] = double(acc);
-    Sums<1> t;
-    if (reduce_last<kNT, 1>(s, partials, counter, t)) st->rho = t.v[0];
+    store_partials<kNT, 1>(s, partials);
 }
 
 // p = z (first) or p = z + (rho/rho_1) * p (cg.hpp:99-102; no-PC: :57-60)
@@ -194,11 +241,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_cg_spmv_dot(int64_t n, int64_t nsli
             acc.v[0] = double(p[row] * qv);
         }
     }
-    Sums<1> t;
-    if (reduce_last<kSpmvNT, 1>(acc, partials, counter, t)) {
-        st->alpha_1 = t.v[0];
-        st->alpha = st->rho / st->alpha_1;
-    }
+    store_partials<kSpmvNT, 1>(acc, partials);
 }
 
 // x += alpha*p; rho_1 = rho; (lazy(r) -= alpha*q) || (lazy(rr) = lazy_unary_dot(r)); then the
@@ -218,13 +261,8 @@ __global__ __launch_bounds__(kNT) void k_cg_update(int64_t n, const T* __restric
         r[i] = ri;
         acc += ri * ri;
     }
-    Sums<1> s{{double(acc)}}, t;
-    if (reduce_last<kNT, 1>(s, partials, counter, t)) {
-        st->rho_1 = st->rho;
-        st->rr = t.v[0];
-        if (identity_pc) st->rho = t.v[0];
-        top_check(st, sqrt(fabs(t.v[0])));
-    }
+    Sums<1> s{{double(acc)}};
+    store_partials<kNT, 1>(s, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -266,8 +304,7 @@ __global__ __launch_bounds__(kNT) void k_bicg_start(int64_t n, const T* __restri
     }
     s.v[0] = double(a0);
     s.v[1] = double(a1);
-    Sums<2> t;
-    if (reduce_last<kNT, 2>(s, partials, counter, t)) bicg_top(st, t.v[0], t.v[1]);
+    store_partials<kNT, 2>(s, partials);
 }
 
 // p = r (first) or p = r + beta*(p - omega*v) (:48-54)
@@ -308,16 +345,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_gamma(int64_t n, int64_t 
             acc.v[0] = double(rtilde[row] * vv);
         }
     }
-    Sums<1> t;
-    if (reduce_last<kSpmvNT, 1>(acc, partials, counter, t)) {
-        st->gamma = t.v[0];
-        if (st->gamma == 0.0) {
-            st->error = M4G_ERR_ORTHOGONALITY;
-            st->done = 1;
-            return;
-        }
-        st->alpha = st->rho_1 / st->gamma;
-    }
+    store_partials<kSpmvNT, 1>(acc, partials);
 }
 
 // s = r - alpha*v; finished(two_norm(s)) -> early exit flag (:62-67)
@@ -334,10 +362,8 @@ __global__ __launch_bounds__(kNT) void k_bicg_s(int64_t n, const T* __restrict__
         s_[i] = si;
         acc += si * si;
     }
-    Sums<1> s{{double(acc)}}, t;
-    if (reduce_last<kNT, 1>(s, partials, counter, t)) {
-        if (finished(st, sqrt(t.v[0]))) st->mid_exit = 1;
-    }
+    Sums<1> s{{double(acc)}};
+    store_partials<kNT, 1>(s, partials);
 }
 
 // early exit: x += alpha*phat, once (:65)
@@ -383,8 +409,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_omega(int64_t n, int64_t 
             acc.v[1] = double(tv * tv);
         }
     }
-    Sums<2> t;
-    if (reduce_last<kSpmvNT, 2>(acc, partials, counter, t)) st->omega = t.v[0] / t.v[1];
+    store_partials<kSpmvNT, 2>(acc, partials);
 }
 
 // x += omega*shat + alpha*phat; r = s - omega*t; rho_2 = rho_1; loop top (:72-75, 43-53)
@@ -405,11 +430,8 @@ __global__ __launch_bounds__(kNT) void k_bicg_update(int64_t n, const T* __restr
         a0 += ri * ri;
         a1 += rtilde[i] * ri;
     }
-    Sums<2> s{{double(a0), double(a1)}}, t;
-    if (reduce_last<kNT, 2>(s, partials, counter, t)) {
-        st->rho_2 = st->rho_1;
-        bicg_top(st, t.v[0], t.v[1]);
-    }
+    Sums<2> s{{double(a0), double(a1)}};
+    store_partials<kNT, 2>(s, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -558,7 +580,10 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
     // r = b - A*x (cg.hpp:93)
     launch_residual_t<T>(A, x, b, r.ptr, st);
     const bool identity = !P || P->kind == M4G_PC_IDENTITY;
+    const int idpc = int(!P || P->kind == M4G_PC_IDENTITY);
+#define FIN(K, OP, G) hipLaunchKernelGGL((k_fin<K, OP>), dim3(1), dim3(256), 0, st, (const double*)part, int(G), S, idpc)
     hipLaunchKernelGGL(k_cg_start<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, S, part, cnt);
+    FIN(1, FIN_CG_START, rg);
     M4G_HIP(hipGetLastError());
     auto body = [&] {
         const T* zz = r.ptr;
@@ -566,21 +591,26 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
             if (P->kind == M4G_PC_DIAGONAL) {
                 hipLaunchKernelGGL(k_cg_diag_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const double*)P->inv_diag.ptr,
                                    (const T*)r.ptr, z.ptr, S, part, cnt);
+                FIN(1, FIN_CG_RHO, rg);
             } else {
                 pc_apply_skip(*P, r.ptr, z.ptr, vt, &S->done, st);
                 hipLaunchKernelGGL(k_cg_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)z.ptr, S,
                                    part, cnt);
+                FIN(1, FIN_CG_RHO, rg);
             }
             zz = z.ptr;
         }
         hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
         hipLaunchKernelGGL(k_cg_spmv_dot<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
                            A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)p.ptr, q.ptr, S, part, cnt);
+        FIN(1, FIN_CG_ALPHA, sgrid);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr, x, r.ptr,
                            S, int(identity), part, cnt);
+        FIN(1, FIN_CG_UPDATE, rg);
         M4G_HIP(hipGetLastError());
     };
     run_batches(ws, it, body);
+#undef FIN
     int fault = 0;
     if (P && P->kind >= M4G_PC_IC0) fault = check_sweep_fault(P->lower) | check_sweep_fault(P->upper);
     return finish_state(ws, it, fault);
@@ -608,7 +638,10 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
     unsigned* cnt = ws.counter.ptr;
     const int rg = red_grid(n);
     launch_residual_t<T>(A, x, b, r.ptr, st);                       // r = b - A*x (:40)
+    const int idpc = 0;
+#define FIN(K, OP, G) hipLaunchKernelGGL((k_fin<K, OP>), dim3(1), dim3(256), 0, st, (const double*)part, int(G), S, idpc)
     hipLaunchKernelGGL(k_bicg_start<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, rtilde.ptr, S, part, cnt);
+    FIN(2, FIN_BICG_START, rg);
     M4G_HIP(hipGetLastError());
     const bool identity = !P || P->kind == M4G_PC_IDENTITY;
     auto apply = [&](const T* in, T* out) {
@@ -622,18 +655,23 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
         hipLaunchKernelGGL(k_bicg_spmv_gamma<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
                            A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)phat.ptr, (const T*)rtilde.ptr,
                            v.ptr, S, part, cnt);
+        FIN(1, FIN_BICG_GAMMA, sgrid);
         hipLaunchKernelGGL(k_bicg_s<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)v.ptr, s_.ptr, S,
                            part, cnt);
+        FIN(1, FIN_BICG_S, rg);
         hipLaunchKernelGGL(k_bicg_exit<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)phat.ptr, x, S, cnt + 1);
         apply(s_.ptr, shat.ptr);                                     // shat = solve(M, s) (:68)
         hipLaunchKernelGGL(k_bicg_spmv_omega<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
                            A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)shat.ptr, (const T*)s_.ptr, t_.ptr,
                            S, part, cnt);
+        FIN(2, FIN_BICG_OMEGA, sgrid);
         hipLaunchKernelGGL(k_bicg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)shat.ptr, (const T*)phat.ptr,
                            (const T*)s_.ptr, (const T*)t_.ptr, (const T*)rtilde.ptr, x, r.ptr, S, part, cnt);
+        FIN(2, FIN_BICG_UPDATE, rg);
         M4G_HIP(hipGetLastError());
     };
     run_batches(ws, it, body);
+#undef FIN
     int fault = 0;
     if (P && P->kind >= M4G_PC_IC0) fault = check_sweep_fault(P->lower) | check_sweep_fault(P->upper);
     return finish_state(ws, it, fault);

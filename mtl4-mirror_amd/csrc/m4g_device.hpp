@@ -42,32 +42,6 @@ __device__ inline T block_sum(T v, T* scratch)
     return t;
 }
 
-// Two-stage deterministic reduction epilogue: each block stores its partial;
-// the last block to arrive sums all partials in block order and returns true
-// in thread 0 with *total set.  counter is left at zero for the next launch.
-template <int NT, typename T>
-__device__ inline bool last_block_total(T partial, T* partials, unsigned* counter, T* total, T* scratch)
-{
-    __shared__ bool am_last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = partial;
-        __threadfence();
-        unsigned ticket = atomicAdd(counter, 1u);
-        am_last = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    T acc = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += NT) acc += __hip_atomic_load(partials + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    acc = block_sum<NT>(acc, scratch);
-    if (threadIdx.x == 0) {
-        *total = acc;
-        *counter = 0;
-    }
-    return threadIdx.x == 0;
-}
-
 // Logical workgroup index with XCD-contiguous ranges: the hardware places
 // workgroup b on XCD b % 8; this mapping gives each XCD one contiguous chunk
 // of `chunk` logical groups, so neighbouring rows share an L2.

@@ -97,82 +97,252 @@ __global__ __launch_bounds__(256) void k_ilu0_level(int64_t count, const int32_t
 // ---------------------------------------------------------------------------
 struct SweepArgs {
     int64_t ntiles;
-    const int32_t* tile_row_begin;
-    const int32_t* rows;
-    const int32_t* tile_group_begin;
-    const int64_t* group_row_begin;
-    const int64_t* group_ent_begin;
-    const int32_t* gr_slot;
-    const int32_t* gr_len;
-    const int32_t* gr_export;
-    const int32_t* ent_src;
-    unsigned* tickets;      // [0] next ticket, [1] finished tiles
-    int* fault;             // set when a bounded wait runs out
-    int64_t other_nexport;  // size of the other direction's exchange buffer
+    const int32_t* tile_row_begin;    // [ntiles+1] into rows
+    const int32_t* rows;              // [n] global row of each (tile, slot)
+    const int32_t* tile_group_begin;  // [ntiles+1] into the group table
+    const int4* group_tab;            // [ngroups+1]: first group row, first entry, width, imports needed
+    const int2* gr_meta;              // [n]: slot | len << 16, export slot (-1: none)
+    const int32_t* ent_src;           // [nent]: LDS slot, or -(tile-local import + 1)
+    const int32_t* tile_import_begin; // [ntiles+1] into imports
+    const int32_t* imports;           // exchange slot of each import
+    int32_t max_tile_rows;            // LDS rows reserved per tile (imports follow them)
+    int32_t max_tile_imports;         // LDS import slots reserved per tile (group table follows)
+    unsigned* tickets;                // [0] next ticket, [1] finished tiles
+    int* fault;                       // set when a bounded wait runs out
+    int64_t other_nexport;            // size of the other direction's exchange buffer
     int64_t spin_limit;
+    int64_t* trace;                   // optional: [tile][2 + 2 * trace_steps] wall-clock stamps
+    int32_t trace_steps;
 };
 
-// Waits for an exported value.  Bounded: after `limit` polls the wait gives up
-// and raises the fault flag, and every later wait of any workgroup returns at
-// once, so a schedule fault ends the kernel instead of hanging it.
-template <typename VT>
-__device__ inline VT exchange_wait(const VT* p, int* fault, int64_t limit)
+constexpr int kGroupChunk = 256;      // group-table entries staged in LDS at a time
+constexpr int kImportDepth = 8;       // imports each importer lane keeps in flight
+
+__device__ inline int wave_min(int v)
 {
-    VT v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (!Sentinel<VT>::is(v)) return v;
-    for (int64_t it = 0; it < limit; it++) {
-        if (__hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return v;
-        __builtin_amdgcn_s_sleep(2);
-        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!Sentinel<VT>::is(v)) return v;
-    }
-    __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
     return v;
 }
 
-template <typename FV, typename VT, bool UNIT>
-__global__ void k_sweep(SweepArgs a, const FV* __restrict__ ent_val, const FV* __restrict__ gr_diag,
-                        const VT* __restrict__ v, VT* __restrict__ y, VT* xbuf, VT* xbuf_other, const int* skip)
+// The data one lane needs for its row of a level group, loaded one level
+// ahead so that the loads overlap the previous level's arithmetic.
+template <typename FV, int MAXW>
+struct RowPre {
+    int2 meta;
+    FV diag;
+    int32_t src[MAXW];
+    FV val[MAXW];
+};
+
+template <typename FV, bool UNIT, int MAXW>
+__device__ inline void row_prefetch(RowPre<FV, MAXW>& p, const SweepArgs& a, const FV* __restrict__ ent_val,
+                                    const FV* __restrict__ gr_diag, int4 t0, int4 t1, int q)
 {
+    const int m = t1.x - t0.x;
+    if (q >= m) return;
+    p.meta = a.gr_meta[t0.x + q];
+    p.diag = UNIT ? FV(1) : gr_diag[t0.x + q];
+    const int w = t0.z;
+#pragma unroll
+    for (int k = 0; k < MAXW; k++)
+        if (k < w) {
+            const int64_t d = int64_t(t0.y) + int64_t(k) * m + q;
+            p.src[k] = a.ent_src[d];
+            p.val[k] = ent_val[d];
+        }
+}
+
+// One row: rr = v_r; rr -= L_rj * y_j in stored order; y_r = rr * d_r
+// (lower_trisolve.hpp:191-209 / 172-189, upper_trisolve.hpp:159-176).  Own
+// values come from the tile's LDS rows, other tiles' values from the LDS
+// import area filled by the importer wavefront.
+template <typename FV, typename VT, bool UNIT, int MAXW>
+__device__ inline void row_compute(const RowPre<FV, MAXW>& p, const SweepArgs& a, const FV* __restrict__ ent_val,
+                                   VT* lds, const VT* imp, VT* xbuf, int4 t0, int m, int q)
+{
+    const int slot = p.meta.x & 0xFFFF;
+    const int len = int(unsigned(p.meta.x) >> 16);
+    VT rr = lds[slot];
+#pragma unroll
+    for (int k = 0; k < MAXW; k++)
+        if (k < len) {
+            const int32_t src = p.src[k];
+            const VT yv = src >= 0 ? lds[src] : imp[-(src + 1)];
+            rr -= VT(p.val[k]) * yv;
+        }
+    for (int k = MAXW; k < len; k++) {   // rows longer than the prefetch window
+        const int64_t d = int64_t(t0.y) + int64_t(k) * m + q;
+        const int32_t src = a.ent_src[d];
+        const VT yv = src >= 0 ? lds[src] : imp[-(src + 1)];
+        rr -= VT(ent_val[d]) * yv;
+    }
+    const VT out = UNIT ? rr : VT(rr * VT(p.diag));
+    lds[slot] = out;
+    if (p.meta.y >= 0)
+        __hip_atomic_store(xbuf + p.meta.y, Sentinel<VT>::canon(out), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Prefetch depth (levels) of the compute wavefront's register ring.
+template <int MAXW> struct PrefetchDepth { static constexpr int value = MAXW <= 4 ? 8 : (MAXW <= 8 ? 4 : 2); };
+
+// Workgroup = 2 wavefronts.  Wavefront 0 imports the tile's values from
+// earlier tiles (exchange buffer -> LDS) in first-use order, keeping up to
+// 64 * kImportDepth polls in flight, and publishes how many are in LDS.
+// Wavefront 1 walks the tile's levels: it waits (on LDS) until the imports a
+// level needs are present, then computes the level's rows from LDS only; its
+// per-row schedule data is prefetched PrefetchDepth levels ahead into a
+// register ring.
+template <typename FV, typename VT, bool UNIT, int MAXW>
+__global__ __launch_bounds__(128) void k_sweep(SweepArgs a, const FV* __restrict__ ent_val,
+                                               const FV* __restrict__ gr_diag, const VT* __restrict__ v,
+                                               VT* __restrict__ y, VT* xbuf, VT* xbuf_other, const int* skip)
+{
+    constexpr int D = PrefetchDepth<MAXW>::value;
     if (skip && *skip) return;   // the solve this sweep belongs to has finished
     extern __shared__ unsigned char lds_raw[];
     VT* lds = reinterpret_cast<VT*>(lds_raw);
+    VT* imp = lds + a.max_tile_rows;
+    int4* tab = reinterpret_cast<int4*>(imp + a.max_tile_imports);   // whole tile's group table (+D sentinel rows)
     __shared__ int tile_sh;
-    if (threadIdx.x == 0) tile_sh = int(atomicAdd(&a.tickets[0], 1u));
+    __shared__ int progress;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    if (tid == 0) {
+        tile_sh = int(atomicAdd(&a.tickets[0], 1u));
+        progress = 0;
+    }
     __syncthreads();
     const int t = tile_sh;
-    const int nt = blockDim.x;
+    if (a.trace && tid == 0) a.trace[int64_t(t) * (2 + 2 * a.trace_steps)] = int64_t(wall_clock64());
     // the other direction's exchange buffer is free: reset it for its next use
-    for (int64_t x = int64_t(t) * nt + threadIdx.x; x < a.other_nexport; x += a.ntiles * int64_t(nt))
+    for (int64_t x = int64_t(t) * 128 + tid; x < a.other_nexport; x += a.ntiles * int64_t(128))
         xbuf_other[x] = Sentinel<VT>::value();
     const int32_t rb = a.tile_row_begin[t], re = a.tile_row_begin[t + 1];
-    for (int32_t q = rb + threadIdx.x; q < re; q += nt) lds[q - rb] = v[a.rows[q]];
-    __syncthreads();
-    const int32_t g0 = a.tile_group_begin[t], g1 = a.tile_group_begin[t + 1];
-    for (int32_t g = g0; g < g1; g++) {
-        const int64_t b = a.group_row_begin[g];
-        const int64_t m = a.group_row_begin[g + 1] - b;
-        const int64_t eb = a.group_ent_begin[g];
-        for (int64_t q = threadIdx.x; q < m; q += nt) {
-            const int32_t slot = a.gr_slot[b + q];
-            const int32_t len = a.gr_len[b + q];
-            VT rr = lds[slot];
-            for (int32_t k = 0; k < len; k++) {
-                const int64_t d = eb + int64_t(k) * m + q;
-                const int32_t src = a.ent_src[d];
-                const VT yv = src >= 0 ? lds[src] : exchange_wait(xbuf - (src + 1), a.fault, a.spin_limit);
-                rr -= VT(ent_val[d]) * yv;
-            }
-            VT out = UNIT ? rr : VT(rr * VT(gr_diag[b + q]));
-            lds[slot] = out;
-            const int32_t ex = a.gr_export[b + q];
-            if (ex >= 0)
-                __hip_atomic_store(xbuf + ex, Sentinel<VT>::canon(out), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();
+    for (int32_t i0 = rb + tid; i0 < re; i0 += 4 * 128) {   // rows first, then the values: 4 gathers in flight
+        int32_t r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) r[u] = i0 + u * 128 < re ? a.rows[i0 + u * 128] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (r[u] >= 0) lds[i0 + u * 128 - rb] = v[r[u]];
     }
-    for (int32_t q = rb + threadIdx.x; q < re; q += nt) y[a.rows[q]] = lds[q - rb];
-    if (threadIdx.x == 0) {
+    const int32_t g0 = a.tile_group_begin[t], g1 = a.tile_group_begin[t + 1], ng = g1 - g0;
+    for (int i = tid; i <= ng + D; i += 128) {
+        // entries past the tile's last group repeat its end, i.e. empty groups
+        tab[i] = a.group_tab[g0 + min(i, ng)];
+    }
+    const int32_t ib = a.tile_import_begin[t], ni = a.tile_import_begin[t + 1] - ib;
+    __syncthreads();
+
+    if (wave == 0) {
+        // ---- importer ----
+        // Probes only the first missing import while it is not there (one
+        // load per poll, with back-off), then loads a window of up to
+        // 64 * kImportDepth imports at once and publishes the prefix that
+        // arrived.  Waiting tiles so put almost no load on memory.
+        int head = 0;
+        int64_t polls = 0;
+        unsigned nap = 1;
+        while (head < ni) {
+            VT hv = VT(0);
+            if (lane == 0)
+                hv = __hip_atomic_load(xbuf + a.imports[ib + head], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool head_ready = __shfl(int(!Sentinel<VT>::is(hv)), 0, kWave) != 0;
+            if (head_ready) {
+                nap = 1;
+                int32_t xs[kImportDepth];
+                VT val[kImportDepth];
+#pragma unroll
+                for (int u = 0; u < kImportDepth; u++) {
+                    const int idx = head + lane + kWave * u;
+                    xs[u] = idx < ni ? a.imports[ib + idx] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < kImportDepth; u++)
+                    val[u] = xs[u] >= 0 ? __hip_atomic_load(xbuf + xs[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                        : VT(0);
+                int first_missing = ni;   // smallest index of the window that has not arrived
+#pragma unroll
+                for (int u = 0; u < kImportDepth; u++) {
+                    const int idx = head + lane + kWave * u;
+                    if (xs[u] >= 0) {
+                        if (!Sentinel<VT>::is(val[u])) imp[idx] = val[u];
+                        else first_missing = min(first_missing, idx);
+                    }
+                }
+                const int window_end = min(ni, head + kWave * kImportDepth);
+                first_missing = wave_min(min(first_missing, window_end));
+                head = first_missing;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0) __hip_atomic_store(&progress, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                continue;
+            }
+            ++polls;
+            if (polls > a.spin_limit ||
+                ((polls & 63) == 0 && __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                if (lane == 0) __hip_atomic_store(a.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            for (unsigned k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(2);
+            nap = min(nap * 2, 16u);
+        }
+    } else {
+        // ---- compute ----
+        bool failed = false;
+        RowPre<FV, MAXW> ring[D];
+#pragma unroll
+        for (int u = 0; u < D; u++) row_prefetch<FV, UNIT, MAXW>(ring[u], a, ent_val, gr_diag, tab[u], tab[u + 1], lane);
+        for (int32_t g = 0; g < ng; g += D) {
+#pragma unroll
+            for (int u = 0; u < D; u++) {
+                const int gi = g + u;
+                if (gi < ng) {
+                    const int4 t0 = tab[gi];
+                    const int m = tab[gi + 1].x - t0.x;
+                    const RowPre<FV, MAXW> cur = ring[u];
+                    row_prefetch<FV, UNIT, MAXW>(ring[u], a, ent_val, gr_diag, tab[gi + D], tab[gi + D + 1], lane);
+                    // imports this level needs
+                    int64_t polls = 0;
+                    while (!failed &&
+                           __hip_atomic_load(&progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < t0.w) {
+                        ++polls;
+                        if (polls > a.spin_limit ||
+                            ((polls & 63) == 0 && __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                            failed = true;
+                            if (lane == 0) __hip_atomic_store(a.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    if (a.trace && lane == 0 && gi < a.trace_steps)
+                        a.trace[int64_t(t) * (2 + 2 * a.trace_steps) + 2 + 2 * gi] = int64_t(wall_clock64());
+                    if (lane < m) row_compute<FV, VT, UNIT, MAXW>(cur, a, ent_val, lds, imp, xbuf, t0, m, lane);
+                    for (int qq = lane + kWave; qq < m; qq += kWave) {   // levels wider than the wavefront
+                        RowPre<FV, MAXW> extra;
+                        row_prefetch<FV, UNIT, MAXW>(extra, a, ent_val, gr_diag, t0, tab[gi + 1], qq);
+                        row_compute<FV, VT, UNIT, MAXW>(extra, a, ent_val, lds, imp, xbuf, t0, m, qq);
+                    }
+                    if (a.trace && lane == 0 && gi < a.trace_steps)
+                        a.trace[int64_t(t) * (2 + 2 * a.trace_steps) + 3 + 2 * gi] = int64_t(wall_clock64());
+                    // the next level reads these rows from LDS
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (a.trace && tid == 0) a.trace[int64_t(t) * (2 + 2 * a.trace_steps) + 1] = int64_t(wall_clock64());
+    for (int32_t i0 = rb + tid; i0 < re; i0 += 4 * 128) {
+        int32_t r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) r[u] = i0 + u * 128 < re ? a.rows[i0 + u * 128] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (r[u] >= 0) y[r[u]] = lds[i0 + u * 128 - rb];
+    }
+    if (tid == 0) {
         unsigned done = atomicAdd(&a.tickets[1], 1u);
         if (done == unsigned(a.ntiles) - 1) {   // every tile holds a ticket: reset for the next launch
             a.tickets[0] = 0;
@@ -229,6 +399,7 @@ void SweepDev::upload(const SweepPlan& p, const double* fstore_dev, int ftype)
     plan_meta = p;
     plan_meta.rows.clear();   // keep only scalars on the host copy
     plan_meta.ent_src.clear();
+    plan_meta.imports.clear();
     plan_meta.ent_val_pos.clear();
     plan_meta.gr_slot.clear();
     plan_meta.gr_len.clear();
@@ -245,14 +416,31 @@ void SweepDev::upload(const SweepPlan& p, const double* fstore_dev, int ftype)
     up32(tile_row_begin, p.tile_row_begin);
     up32(rows, p.rows);
     up32(tile_group_begin, p.tile_group_begin);
-    up64(group_row_begin, p.group_row_begin);
-    up64(group_ent_begin, p.group_ent_begin);
-    up32(gr_slot, p.gr_slot);
-    up32(gr_len, p.gr_len);
-    up32(gr_export, p.gr_export);
     up32(ent_src, p.ent_src);
+    up32(tile_import_begin, p.tile_import_begin);
+    up32(imports, p.imports);
     up64(ent_val_pos, p.ent_val_pos);
     up64(gr_diag_pos, p.gr_diag_pos);
+    M4G_REQUIRE(p.nent < (int64_t(1) << 31) && p.n < (int64_t(1) << 31), M4G_ERR_INVALID, "sweep too large");
+    std::vector<int4> tab(p.ngroups + 1);
+    for (int64_t g = 0; g <= p.ngroups; g++) {
+        tab[g].x = int(p.group_row_begin[g]);
+        tab[g].y = int(p.group_ent_begin[g]);
+        tab[g].z = g < p.ngroups ? p.group_width[g] : 0;
+        tab[g].w = g < p.ngroups ? p.group_need[g] : 0;
+    }
+    group_tab.reset(tab.size());
+    group_tab.upload(tab.data(), tab.size());
+    std::vector<int2> meta(std::max<int64_t>(p.n, 1));
+    max_len = 0;
+    for (int64_t i = 0; i < p.n; i++) {
+        M4G_REQUIRE(p.gr_slot[i] < 65536 && p.gr_len[i] < 65536, M4G_ERR_INVALID, "sweep row too long");
+        meta[i].x = int(unsigned(p.gr_slot[i]) | (unsigned(p.gr_len[i]) << 16));
+        meta[i].y = p.gr_export[i];
+        max_len = std::max(max_len, int(p.gr_len[i]));
+    }
+    gr_meta.reset(meta.size());
+    gr_meta.upload(meta.data(), meta.size());
     tickets.reset(2);
     M4G_HIP(hipMemsetAsync(tickets.ptr, 0, 2 * sizeof(unsigned), current_stream()));
     fault.reset(1);
@@ -320,29 +508,61 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
     a.tile_row_begin = d.tile_row_begin.ptr;
     a.rows = d.rows.ptr;
     a.tile_group_begin = d.tile_group_begin.ptr;
-    a.group_row_begin = d.group_row_begin.ptr;
-    a.group_ent_begin = d.group_ent_begin.ptr;
-    a.gr_slot = d.gr_slot.ptr;
-    a.gr_len = d.gr_len.ptr;
-    a.gr_export = d.gr_export.ptr;
+    a.group_tab = d.group_tab.ptr;
+    a.gr_meta = d.gr_meta.ptr;
+    a.tile_import_begin = d.tile_import_begin.ptr;
+    a.imports = d.imports.ptr;
+    a.max_tile_rows = int32_t(p.max_tile_rows);
+    a.max_tile_imports = int32_t(p.max_tile_imports);
     a.ent_src = d.ent_src.ptr;
     a.tickets = d.tickets.ptr;
     a.fault = d.fault.ptr;
     a.other_nexport = other.plan_meta.nexport;
     a.spin_limit = spin_limit();
+    a.trace = nullptr;
+    a.trace_steps = 0;
+    static const char* trace_path = std::getenv("M4G_SWEEP_TRACE");
+    static int traced = 0;
+    DevBuf<int64_t> trace_buf;
+    if (trace_path && traced < 4) {
+        a.trace_steps = int32_t(p.max_steps);
+        trace_buf.reset(size_t(p.ntiles) * (2 + 2 * p.max_steps));
+        M4G_HIP(hipMemsetAsync(trace_buf.ptr, 0, trace_buf.count * sizeof(int64_t), s));
+        a.trace = trace_buf.ptr;
+    }
     const FV* ev = std::is_same<FV, double>::value ? (const FV*)d.ent_val64.ptr : (const FV*)d.ent_val32.ptr;
     const FV* dg = std::is_same<FV, double>::value ? (const FV*)d.diag64.ptr : (const FV*)d.diag32.ptr;
-    size_t lds = size_t(p.max_tile_rows) * sizeof(VT);
+    // rows, imports, then the tile's group table (+ prefetch padding), 16-byte aligned
+    size_t lds = size_t(p.max_tile_rows + p.max_tile_imports) * sizeof(VT);
+    lds = (lds + 15) / 16 * 16 + size_t(p.max_steps + 16 + 1) * sizeof(int4);
     if (p.ntiles == 0) return;
-    auto kern = p.unit ? k_sweep<FV, VT, true> : k_sweep<FV, VT, false>;
-    if (lds > 65536 - 64) {
-        // more than 64 KB of dynamic LDS must be requested explicitly
+    auto pick = [&](auto unit_tag) {
+        constexpr bool U = decltype(unit_tag)::value;
+        if (d.max_len <= 4) return k_sweep<FV, VT, U, 4>;
+        if (d.max_len <= 8) return k_sweep<FV, VT, U, 8>;
+        return k_sweep<FV, VT, U, 16>;
+    };
+    auto kern = p.unit ? pick(std::true_type()) : pick(std::false_type());
+    if (lds + 64 > 65536) {
+        // beyond 64 KB of LDS per workgroup (with the static group table) must be requested
         M4G_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     }
-    hipLaunchKernelGGL(kern, dim3(p.ntiles), dim3(p.block), lds, s, a, ev, dg, v, y, (VT*)d.xbuf.ptr,
+    hipLaunchKernelGGL(kern, dim3(p.ntiles), dim3(128), lds, s, a, ev, dg, v, y, (VT*)d.xbuf.ptr,
                        (VT*)other.xbuf.ptr, skip);
     M4G_HIP(hipGetLastError());
+    if (a.trace) {
+        // trace file: header (ntiles, steps) then the stamps of every tile
+        std::vector<int64_t> h(trace_buf.count);
+        trace_buf.download(h.data(), h.size());
+        std::string path = std::string(trace_path) + "." + std::to_string(traced++);
+        if (FILE* f = std::fopen(path.c_str(), "wb")) {
+            int64_t hdr[2] = {p.ntiles, p.max_steps};
+            std::fwrite(hdr, sizeof(hdr), 1, f);
+            std::fwrite(h.data(), sizeof(int64_t), h.size(), f);
+            std::fclose(f);
+        }
+    }
 }
 
 // Forward sweep on `first` then backward on `second`, v -> tmp -> y.
@@ -394,17 +614,34 @@ int64_t sweep_tile_cap(int vtype)
             lds = std::min(v, 160 * 1024);
     }
     const int64_t elem = vtype == M4G_F64 ? 8 : 4;
-    int64_t cap = std::min<int64_t>(vtype == M4G_F64 ? 8192 : 16384, (lds - 256) / elem);
+    int64_t cap = std::min<int64_t>(vtype == M4G_F64 ? 8192 : 16384, (lds - 4400) / elem);
     const char* e = std::getenv("M4G_TILE_ROWS");
     if (e) cap = std::min<int64_t>(cap, std::max<int64_t>(64, std::atoll(e)));
     return cap;
 }
 
-SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap)
+SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int vtype)
 {
-    TileShape shape = choose_tiles(rows, g, cap, env_double("M4G_SWEEP_HOP_US", 1.5),
-                                   env_double("M4G_SWEEP_STEP_US", 0.15));
-    SweepPlan p = build_sweep_plan(rows, g, shape, cap);
+    // a tile's rows and imports share the workgroup's LDS
+    int lds = 65536, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            lds = std::min(v, 160 * 1024);
+    }
+    // sized for double vectors, the widest the sweeps run with
+    const int64_t elem = 8;
+    (void)vtype;
+    SweepPlan p;
+    for (;;) {
+        TileShape shape = choose_tiles(rows, g, cap, env_double("M4G_SWEEP_HOP_US", 1.5),
+                                       env_double("M4G_SWEEP_STEP_US", 0.15));
+        p = build_sweep_plan(rows, g, shape, cap);
+        if ((p.max_tile_rows + p.max_tile_imports) * elem + (p.max_steps + 17) * 16 + 64 <= lds || cap <= 64) break;
+        cap = cap * 3 / 4;
+    }
+    M4G_REQUIRE((p.max_tile_rows + p.max_tile_imports) * elem + (p.max_steps + 17) * 16 + 64 <= lds, M4G_ERR_INVALID,
+                "sweep tile does not fit in LDS");
     if (std::getenv("M4G_VALIDATE_PLANS")) {
         std::string msg = validate_sweep_plan(p, rows);
         M4G_REQUIRE(msg.empty(), M4G_ERR_LOGIC, "sweep plan invalid: " + msg);
@@ -497,11 +734,11 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 M4G_REQUIRE(ic0_rows(n, P->fS, P->fI, lo, up), M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
                 // U = upper(A)
                 P->fstore.reset(std::max<size_t>(a_pos.size(), 1));
-                DevBuf<int64_t> d_apos(std::max<size_t>(a_pos.size(), 1));
-                d_apos.upload(a_pos.data(), a_pos.size());
+                P->a_pos.reset(std::max<size_t>(a_pos.size(), 1));
+                P->a_pos.upload(a_pos.data(), a_pos.size());
                 if (!a_pos.empty())
                     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(a_pos.size())), dim3(256), 0, s,
-                                       int64_t(a_pos.size()), d_apos.ptr, A->values.ptr, P->fstore.ptr, 0.0);
+                                       int64_t(a_pos.size()), P->a_pos.ptr, A->values.ptr, P->fstore.ptr, 0.0);
                 M4G_HIP(hipGetLastError());
                 factorize_levels(*P, lo, *A);
             } else {
@@ -516,12 +753,59 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 factorize_levels(*P, lo, *A);
             }
             int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
-            P->lower.upload(plan_sweep(lo, g, cap), P->fstore.ptr, ftype);
-            P->upper.upload(plan_sweep(up, g, cap), P->fstore.ptr, ftype);
+            P->lower.upload(plan_sweep(lo, g, cap, ftype), P->fstore.ptr, ftype);
+            P->upper.upload(plan_sweep(up, g, cap, ftype), P->fstore.ptr, ftype);
             P->tmp_bytes = 0;
         }
         M4G_HIP(hipStreamSynchronize(s));
         *out = P.release();
+        return M4G_OK;
+    });
+}
+
+int m4g_pc_refactor(m4g_pc P, m4g_matrix A)
+{
+    return guard([&] {
+        M4G_REQUIRE(P && A, M4G_ERR_INVALID, "m4g_pc_refactor: null argument");
+        M4G_REQUIRE(A->nrows == P->n && A->ncols == P->n, M4G_ERR_SIZE, "m4g_pc_refactor: incompatible size");
+        hipStream_t s = current_stream();
+        const int64_t n = P->n;
+        if (P->kind == M4G_PC_DIAGONAL) {
+            if (n)
+                hipLaunchKernelGGL(k_diag_setup, dim3(grid_for(n)), dim3(256), 0, s, n, A->starts.ptr,
+                                   A->indices.ptr, A->values.ptr, P->inv_diag.ptr);
+            M4G_HIP(hipGetLastError());
+        } else if (P->kind == M4G_PC_IC0 || P->kind == M4G_PC_ILU0) {
+            SweepRows lo, up;
+            if (P->kind == M4G_PC_IC0) {
+                std::vector<int64_t> US, UI, apos;
+                M4G_REQUIRE(upper_pattern(n, A->h_starts.data(), A->h_indices.data(), US, UI, apos) && US == P->fS &&
+                                UI == P->fI,
+                            M4G_ERR_INVALID, "m4g_pc_refactor: pattern differs");
+                M4G_REQUIRE(ic0_rows(n, P->fS, P->fI, lo, up), M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
+                const int64_t nu = P->fS[n];
+                if (nu)
+                    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nu)), dim3(256), 0, s, nu, P->a_pos.ptr,
+                                       A->values.ptr, P->fstore.ptr, 0.0);
+            } else {
+                M4G_REQUIRE(A->h_starts == P->fS && A->h_indices == P->fI, M4G_ERR_INVALID,
+                            "m4g_pc_refactor: pattern differs");
+                std::vector<int64_t> dpos;
+                M4G_REQUIRE(ilu0_rows(n, P->fS.data(), P->fI.data(), lo, up, dpos), M4G_ERR_MISSING_DIAGONAL,
+                            "missing diagonal");
+                if (A->nnz)
+                    M4G_HIP(hipMemcpyAsync(P->fstore.ptr, A->values.ptr, A->nnz * sizeof(double),
+                                           hipMemcpyDeviceToDevice, s));
+            }
+            factorize_levels(*P, lo, *A);
+            P->lower.refresh_values(P->fstore.ptr, P->factor_type);
+            P->upper.refresh_values(P->fstore.ptr, P->factor_type);
+            if (P->adj_ready) {
+                P->adj_lower.refresh_values(P->fstore.ptr, P->factor_type);
+                P->adj_upper.refresh_values(P->fstore.ptr, P->factor_type);
+            }
+        }
+        M4G_HIP(hipStreamSynchronize(s));
         return M4G_OK;
     });
 }
@@ -570,8 +854,8 @@ void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipS
             M4G_REQUIRE(ilu0_adjoint_rows(n, P.fS.data(), P.fI.data(), lo, up), M4G_ERR_MISSING_DIAGONAL,
                         "missing diagonal");
             int64_t cap = sweep_tile_cap(P.factor_type);
-            P.adj_lower.upload(plan_sweep(lo, P.grid, cap), P.fstore.ptr, P.factor_type);
-            P.adj_upper.upload(plan_sweep(up, P.grid, cap), P.fstore.ptr, P.factor_type);
+            P.adj_lower.upload(plan_sweep(lo, P.grid, cap, P.factor_type), P.fstore.ptr, P.factor_type);
+            P.adj_upper.upload(plan_sweep(up, P.grid, cap, P.factor_type), P.fstore.ptr, P.factor_type);
             P.adj_ready = true;
         }
         run_sweep_pair(P.adj_lower, P.adj_upper, x, tmp, y, vtype, skip, s);

@@ -9,8 +9,11 @@ namespace m4g {
 // A compiled sweep on the device.
 struct SweepDev {
     SweepPlan plan_meta;                  // scalars and tile/group offsets (large arrays dropped)
-    DevBuf<int32_t> tile_row_begin, rows, tile_group_begin, gr_slot, gr_len, gr_export, ent_src;
-    DevBuf<int64_t> group_row_begin, group_ent_begin, ent_val_pos, gr_diag_pos;
+    DevBuf<int32_t> tile_row_begin, rows, tile_group_begin, ent_src, tile_import_begin, imports;
+    DevBuf<int4> group_tab;               // per group: first group row, first entry, width
+    DevBuf<int2> gr_meta;                 // per group row: slot | len << 16, export slot
+    DevBuf<int64_t> ent_val_pos, gr_diag_pos;
+    int max_len = 0;                      // longest row of the sweep
     DevBuf<double> ent_val64, diag64;
     DevBuf<float> ent_val32, diag32;
     DevBuf<unsigned char> xbuf;           // exchange buffer (double or float entries)
@@ -28,7 +31,7 @@ void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp,
                     const int* skip, hipStream_t s);
 int check_sweep_fault(SweepDev& d);
 int64_t sweep_tile_cap(int vtype);
-SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap);
+SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int vtype);
 
 }  // namespace m4g
 
@@ -41,6 +44,7 @@ struct m4g_pc_s {
     std::vector<int64_t> fS, fI;          // pattern of the storage
     std::vector<int64_t> diag_pos;        // ILU(0): storage index of each diagonal
     m4g::DevBuf<double> fstore;
+    m4g::DevBuf<int64_t> a_pos;           // IC(0): CSR index in A of each upper(A) entry
     m4g::DevBuf<double> inv_diag;         // diagonal PC
     m4g::SweepDev lower, upper;           // solve: lower then upper
     m4g::SweepDev adj_lower, adj_upper;   // ILU(0) adjoint_solve

@@ -368,6 +368,57 @@ int m4g_matrix_set_values(m4g_matrix A, const double* values, int values_device)
     });
 }
 
+int m4g_matrix_transpose(m4g_matrix A, m4g_matrix* out)
+{
+    return guard([&] {
+        M4G_REQUIRE(A && out, M4G_ERR_INVALID, "m4g_matrix_transpose: null argument");
+        auto T = std::make_unique<m4g_matrix_s>();
+        const int64_t m = A->nrows, n = A->ncols, nnz = A->nnz;
+        T->nrows = n;
+        T->ncols = m;
+        T->nnz = nnz;
+        T->h_starts.assign(n + 1, 0);
+        T->h_indices.assign(nnz, 0);
+        std::vector<int64_t> pos(nnz);
+        for (int64_t p = 0; p < nnz; p++) T->h_starts[A->h_indices[p] + 1]++;
+        for (int64_t r = 0; r < n; r++) T->h_starts[r + 1] += T->h_starts[r];
+        std::vector<int64_t> fill(T->h_starts.begin(), T->h_starts.end() - 1);
+        for (int64_t r = 0; r < m; r++)
+            for (int64_t p = A->h_starts[r]; p < A->h_starts[r + 1]; p++) {
+                int64_t d = fill[A->h_indices[p]]++;
+                T->h_indices[d] = r;
+                pos[d] = p;
+            }
+        std::vector<int32_t> s32(T->h_starts.begin(), T->h_starts.end()), i32(T->h_indices.begin(), T->h_indices.end());
+        T->starts.reset(n + 1);
+        T->indices.reset(std::max<int64_t>(nnz, 1));
+        T->values.reset(std::max<int64_t>(nnz, 1));
+        T->starts.upload(s32.data(), n + 1);
+        T->indices.upload(i32.data(), nnz);
+        DevBuf<int64_t> dpos(std::max<int64_t>(nnz, 1));
+        dpos.upload(pos.data(), nnz);
+        launch_gather_values(nnz, dpos.ptr, A->values.ptr, T->values.ptr, current_stream());
+        sell_build(*T);
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+        *out = T.release();
+        return M4G_OK;
+    });
+}
+
+int m4g_pattern_gradient(m4g_matrix A, m4g_vector lambda, m4g_vector x, m4g_vector grad)
+{
+    return guard([&] {
+        M4G_REQUIRE(A && lambda && x && grad, M4G_ERR_INVALID, "m4g_pattern_gradient: null argument");
+        M4G_REQUIRE(lambda->type == M4G_F64 && x->type == M4G_F64 && grad->type == M4G_F64, M4G_ERR_INVALID,
+                    "m4g_pattern_gradient: double vectors");
+        M4G_REQUIRE(lambda->n == A->nrows && x->n == A->ncols && grad->n == A->nnz, M4G_ERR_SIZE,
+                    "m4g_pattern_gradient: incompatible size");
+        launch_pattern_gradient(*A, (const double*)lambda->ptr, (const double*)x->ptr, (double*)grad->ptr,
+                                current_stream());
+        return M4G_OK;
+    });
+}
+
 // ------------------------------- level 1 -----------------------------------
 static void check_f64(m4g_vector v, const char* who)
 {

@@ -216,9 +216,13 @@ TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, doubl
                 double span = w3 * double(std::min(e3, ext3) - 1) + w2 * double(std::min(e2, ext2) - 1) +
                               w1 * double(std::min(e1, ext1) - 1) + 1.0;
                 span = std::min(span, double(nlev));
+                // every global level lies on the critical path whatever the
+                // tiling; tiles add a hop latency at each boundary crossed
                 double per_level = double(rows) / std::max(span, 1.0);
-                double passes = std::max(1.0, std::ceil(per_level / 256.0));
-                double cost = hops * hop_us + span * passes * step_us;
+                // one compute wavefront per tile: rows beyond 64 per level are
+                // extra passes of the level
+                double passes = 1.0 + 0.5 * (std::max(1.0, std::ceil(per_level / 64.0)) - 1.0);
+                double cost = hops * hop_us + double(nlev) * passes * step_us;
                 if (cost < best_cost) {
                     best_cost = cost;
                     best.grid = true;
@@ -393,6 +397,38 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
             }
         }
     }
+    // 6. imports: the values each tile reads from other tiles, numbered per
+    //    tile in order of first use; entries refer to them by tile-local index
+    p.tile_import_begin.assign(p.ntiles + 1, 0);
+    p.group_need.assign(p.ngroups, 0);
+    p.imports.clear();
+    {
+        std::vector<int32_t> local(p.nexport, -1);   // exchange slot -> import of the current tile
+        for (int64_t t = 0; t < p.ntiles; t++) {
+            const int32_t ib = int32_t(p.imports.size());
+            p.tile_import_begin[t] = ib;
+            for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
+                int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
+                for (int64_t q = 0; q < m; q++)
+                    for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
+                        int64_t d = p.group_ent_begin[gi] + k * m + q;
+                        int32_t src = p.ent_src[d];
+                        if (src >= 0) continue;
+                        int32_t xs = -(src + 1);
+                        if (local[xs] < 0) {
+                            local[xs] = int32_t(p.imports.size()) - ib;
+                            p.imports.push_back(xs);
+                        }
+                        p.ent_src[d] = -(local[xs] + 1);
+                    }
+                p.group_need[gi] = int32_t(p.imports.size()) - ib;
+            }
+            for (int64_t i = ib; i < int64_t(p.imports.size()); i++) local[p.imports[i]] = -1;
+            p.max_tile_imports = std::max<int64_t>(p.max_tile_imports, int64_t(p.imports.size()) - ib);
+        }
+        p.tile_import_begin[p.ntiles] = int32_t(p.imports.size());
+        p.nimports = int64_t(p.imports.size());
+    }
     int64_t blk = ((std::min<int64_t>(maxgroup, 1024) + 63) / 64) * 64;
     p.block = int(std::max<int64_t>(64, std::min<int64_t>(blk, 256)));
     return p;
@@ -466,7 +502,10 @@ std::string validate_sweep_plan(const SweepPlan& p, const SweepRows& s)
                     if (grp_of[c] >= gi) return "intra-tile dependency not in an earlier group";
                 } else {
                     if (tile_of[c] >= t) return "dependency on a later tile";
-                    if (x_of[c] < 0 || src != -(x_of[c] + 1)) return "missing export";
+                    int32_t imp = -(src + 1);
+                    if (imp < 0 || p.tile_import_begin[t] + imp >= p.tile_import_begin[t + 1]) return "import range";
+                    if (x_of[c] < 0 || p.imports[p.tile_import_begin[t] + imp] != x_of[c]) return "missing export";
+                    if (imp >= p.group_need[gi]) return "import not counted before its first use";
                 }
             }
         }
@@ -488,7 +527,7 @@ void emulate_sweep(const SweepPlan& p, const FV* f, const VT* v, VT* y)
                 for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
                     int64_t d = p.group_ent_begin[gi] + k * m + q;
                     int32_t src = p.ent_src[d];
-                    VT yv = src >= 0 ? lds[src] : xbuf[-(src + 1)];
+                    VT yv = src >= 0 ? lds[src] : xbuf[p.imports[p.tile_import_begin[t] - src - 1]];
                     rr -= VT(f[p.ent_val_pos[d]]) * yv;
                 }
                 VT out = p.gr_diag_pos[b + q] >= 0 ? VT(rr * VT(f[p.gr_diag_pos[b + q]])) : rr;

@@ -55,7 +55,11 @@ struct SweepPlan {
     std::vector<int32_t> gr_len;          // [n] number of entries
     std::vector<int32_t> gr_export;       // [n] export slot or -1
     std::vector<int64_t> gr_diag_pos;     // [n] factor storage index of the diagonal or -1
-    std::vector<int32_t> ent_src;         // [nent] LDS slot (>= 0) or -(export slot + 1)
+    std::vector<int32_t> ent_src;         // [nent] LDS slot (>= 0) or -(tile-local import + 1)
+    std::vector<int32_t> tile_import_begin;  // [ntiles+1] into imports
+    std::vector<int32_t> imports;         // exchange slot of each import, per tile in first-use order
+    std::vector<int32_t> group_need;      // [ngroups] imports of the tile needed before the group runs
+    int64_t max_tile_imports = 0, nimports = 0;
     std::vector<int64_t> ent_val_pos;     // [nent] factor storage index (-1: padding)
 };
 

@@ -155,8 +155,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_spmv(int64_t n, int64_t nslice
     }
     if (DOT) {
         double blk = block_sum<kSpmvNT>(dotp, scratch);
-        double total;
-        if (last_block_total<kSpmvNT>(blk, partials, counter, &total, scratch)) *out = total;
+        if (threadIdx.x == 0) partials[blockIdx.x] = blk;
     }
 }
 
@@ -214,7 +213,26 @@ void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* 
     hipLaunchKernelGGL((k_sell_spmv<0, true>), dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
                        S.slice_ptr.ptr, S.cols.ptr, S.vals.ptr, p, q, (const double*)nullptr,
                        A.red_partials.ptr, A.red_counter.ptr, out);
+    launch_sum_partials(A.red_partials.ptr, int(grid), out, st);
     M4G_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_pattern_gradient(int64_t n, const int32_t* __restrict__ starts,
+                                                         const int32_t* __restrict__ indices,
+                                                         const double* __restrict__ lambda,
+                                                         const double* __restrict__ x, double* __restrict__ grad)
+{
+    for (int64_t r = blockIdx.x * int64_t(256) + threadIdx.x; r < n; r += int64_t(gridDim.x) * 256) {
+        const double l = lambda[r];
+        for (int32_t p = starts[r]; p < starts[r + 1]; p++) grad[p] = -l * x[indices[p]];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_values(int64_t count, const int64_t* __restrict__ pos,
+                                                       const double* __restrict__ src, double* __restrict__ dst)
+{
+    for (int64_t e = blockIdx.x * int64_t(256) + threadIdx.x; e < count; e += int64_t(gridDim.x) * 256)
+        dst[e] = src[pos[e]];
 }
 
 template <typename T>
@@ -228,6 +246,24 @@ void launch_residual_t(const m4g_matrix_s& A, const T* x, const T* b, T* r, hipS
     M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
     hipLaunchKernelGGL(k_sell_residual<T>, dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
                        S.slice_ptr.ptr, S.cols.ptr, vals, x, b, r);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_pattern_gradient(const m4g_matrix_s& A, const double* lambda, const double* x, double* grad,
+                             hipStream_t s)
+{
+    if (!A.nrows) return;
+    int g = int(std::min<int64_t>((A.nrows + 255) / 256, 16384));
+    hipLaunchKernelGGL(k_pattern_gradient, dim3(g), dim3(256), 0, s, A.nrows, A.starts.ptr, A.indices.ptr, lambda,
+                       x, grad);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_gather_values(int64_t count, const int64_t* pos, const double* src, double* dst, hipStream_t s)
+{
+    if (!count) return;
+    int g = int(std::min<int64_t>((count + 255) / 256, 16384));
+    hipLaunchKernelGGL(k_gather_values, dim3(g), dim3(256), 0, s, count, pos, src, dst);
     M4G_HIP(hipGetLastError());
 }
 

@@ -56,6 +56,12 @@ class Iteration(ctypes.Structure):
                 ("interrupt", ctypes.POINTER(ctypes.c_int))]
 
 
+class RefineInfo(ctypes.Structure):
+    """m4g_refine_info of m4g_refined_solve."""
+    _fields_ = [("outer_iterations", ctypes.c_int), ("inner_iterations", ctypes.c_int),
+                ("resid", ctypes.c_double), ("inner_error", ctypes.c_int)]
+
+
 _SIGS = {
     "m4g_last_error": (ctypes.c_char_p, []),
     "m4g_version": (ctypes.c_char_p, []),
@@ -97,6 +103,11 @@ _SIGS = {
     "m4g_pc_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(PcStats)]),
     "m4g_cg": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
     "m4g_bicgstab": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_pc_refactor": (ctypes.c_int, [_vp, _vp]),
+    "m4g_matrix_transpose": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "m4g_pattern_gradient": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "m4g_refined_solve": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), ctypes.c_double,
+                                         ctypes.c_int, ctypes.POINTER(RefineInfo)]),
     # host-only analysis (mtl4gpu_analysis.h)
     "m4g_analysis_grid": (ctypes.c_int, [_c_i64, _vp, _vp, _c_i64p, _c_i64p, _c_i64p]),
     "m4g_analysis_pc_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_i64, _vp, _vp, _vp, ctypes.c_int,
@@ -210,6 +221,14 @@ class Matrix:
             v = np.ascontiguousarray(values, np.float64)
             _check(_f["m4g_matrix_set_values"](self.h, _ptr(v), 0), "m4g_matrix_set_values")
 
+    def transpose(self):
+        """trans(A) as a new device matrix."""
+        h = ctypes.c_void_p()
+        _check(_f["m4g_matrix_transpose"](self.h, ctypes.byref(h)), "m4g_matrix_transpose")
+        T = Matrix.__new__(Matrix)
+        T.h, T.nrows, T.ncols, T.nnz = h, self.ncols, self.nrows, self.nnz
+        return T
+
     def close(self):
         if getattr(self, "h", None):
             _f["m4g_matrix_destroy"](self.h)
@@ -304,6 +323,10 @@ class PC:
         _check(_f["m4g_pc_download_factor"](self.h, which, _ptr(S), _ptr(I), _ptr(V)), "m4g_pc_download_factor")
         return S, I, V
 
+    def refactor(self, A):
+        """Numeric refactorisation for A's current values (same pattern)."""
+        _check(_f["m4g_pc_refactor"](self.h, A.h), "m4g_pc_refactor")
+
     def stats(self):
         st = PcStats()
         _check(_f["m4g_pc_stats_get"](self.h, ctypes.byref(st)), "m4g_pc_stats_get")
@@ -366,6 +389,11 @@ def spmv_dot(A, p, q):
     return _scalar(_f["m4g_spmv_dot"], "m4g_spmv_dot", A.h, p.h, q.h)
 
 
+def pattern_gradient(A, lam, x, grad):
+    """grad[p] = -lam[row(p)] * x[col(p)] on A's stored pattern."""
+    _check(_f["m4g_pattern_gradient"](A.h, lam.h, x.h, grad.h), "m4g_pattern_gradient")
+
+
 def make_iteration(max_iter, rtol, atol=0.0, norm_r0=0.0, history=True):
     it = Iteration()
     it.max_iter, it.rtol, it.atol, it.norm_r0 = int(max_iter), float(rtol), float(atol), float(norm_r0)
@@ -391,6 +419,15 @@ def cg(A, x, b, P, it):
 def bicgstab(A, x, b, P, it):
     """itl::bicgstab(A, x, b, P, iter); returns the iteration's error code."""
     return _solve(_f["m4g_bicgstab"], "m4g_bicgstab", A, x, b, P, it)
+
+
+def refined_solve(method, A, x, b, P, it, inner_rtol, inner_max_iter=10000):
+    """FP32-inner / FP64-refined solve; returns (error code, RefineInfo)."""
+    info = RefineInfo()
+    st = _f["m4g_refined_solve"](method, A.h, x.h, b.h, P.h if P is not None else None, ctypes.byref(it),
+                                 float(inner_rtol), int(inner_max_iter), ctypes.byref(info))
+    _check(st, "m4g_refined_solve", allow=(OK, MAX_ITER))
+    return st, info
 
 
 # ---------------------------------------------------------------------------
