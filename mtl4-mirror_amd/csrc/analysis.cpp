@@ -52,7 +52,8 @@ int m4g_analysis_pc_solve(int kind, int adjoint, int64_t n, const int64_t* S, co
             }
         }
         GridInfo g = infer_grid(n, S, I);
-        TileShape sl = choose_tiles(lo, g, cap, 1.5, 0.15), su = choose_tiles(up, g, cap, 1.5, 0.15);
+        TileShape sl = choose_tiles(lo, g, cap, kSweepHopUs, kSweepStepUs),
+                  su = choose_tiles(up, g, cap, kSweepHopUs, kSweepStepUs);
         SweepPlan pl = build_sweep_plan(lo, g, sl, cap), pu = build_sweep_plan(up, g, su, cap);
         std::string m1 = validate_sweep_plan(pl, lo), m2 = validate_sweep_plan(pu, up);
         if (!m1.empty() || !m2.empty()) {

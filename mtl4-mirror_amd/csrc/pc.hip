@@ -95,28 +95,96 @@ __global__ __launch_bounds__(256) void k_ilu0_level(int64_t count, const int32_t
 // ---------------------------------------------------------------------------
 // sweep kernel
 // ---------------------------------------------------------------------------
+// Packed record of one piece (a run of at most W entries of a row).  The
+// record is a whole number of 16-byte chunks; the records of a group of m
+// pieces are stored chunk-interleaved (chunk c of piece q at c*m + q), so
+// each chunk load of a wavefront reads one contiguous span.
+template <typename FV, int W>
+struct alignas(16) SweepRec {
+    uint32_t row;       // LDS byte offset of the row's value
+    int32_t need;       // imports of the tile the piece's group needs (same for the whole group)
+    uint32_t src[W];    // LDS byte offset of each entry's value (padding: the zero)
+    FV diag;            // inverse diagonal; 1 for a unit diagonal or a piece before the last
+    FV val[W];          // 0 for padding
+};
+template <typename FV, int W> constexpr int kRecChunks = int(sizeof(SweepRec<FV, W>) / 16);
+// Records reach the compute wavefront through a ring of 2*B group slots in
+// LDS filled by two loader wavefronts: loader k writes the batches of B
+// groups b = k, k+2, ... into ring half k, with the next batch's loads in
+// flight meanwhile (two register buffers of B*C chunk loads, within the 63
+// loads a wavefront can count).
+template <typename FV, int W> constexpr int kLoadBatch =
+    kRecChunks<FV, W> <= 3 ? 8 : (kRecChunks<FV, W> <= 4 ? 6 : (kRecChunks<FV, W> <= 6 ? 4 : 2));
+
+// Host-side mirror of the record size (bytes) for a factor type and width.
+inline int rec_chunks(int ftype, int width)
+{
+    const int fb = ftype == M4G_F64 ? 8 : 4;
+    int off = 8 + 4 * width;                 // row, need, src[]
+    off = (off + fb - 1) / fb * fb;          // diag alignment
+    off += fb * (1 + width);                 // diag, val[]
+    return (off + 15) / 16;
+}
+inline int load_batch(int chunks) { return chunks <= 3 ? 8 : (chunks <= 4 ? 6 : (chunks <= 6 ? 4 : 2)); }
+
+template <typename FV, int W>
+union RecBits {
+    SweepRec<FV, W> r;
+    int4 c[kRecChunks<FV, W>];
+};
+
 struct SweepArgs {
     int64_t ntiles;
     const int32_t* tile_row_begin;    // [ntiles+1] into rows
-    const int32_t* rows;              // [n] global row of each (tile, slot)
+    const int32_t* rows;              // [n] rows of each tile, natural order
+    const int32_t* row_slot;          // [n] LDS slot of each entry of rows
     const int32_t* tile_group_begin;  // [ntiles+1] into the group table
-    const int4* group_tab;            // [ngroups+1]: first group row, first entry, width, imports needed
-    const int2* gr_meta;              // [n]: slot | len << 16, export slot (-1: none)
-    const int32_t* ent_src;           // [nent]: LDS slot, or -(tile-local import + 1)
+    const int4* group_tab;            // [ngroups+1]: first piece, imports needed, exports before, 0
     const int32_t* tile_import_begin; // [ntiles+1] into imports
     const int32_t* imports;           // exchange slot of each import
+    const int32_t* tile_export_begin; // [ntiles+1] exchange slots of each tile's exports
+    const int32_t* xlist;             // [nexport] LDS slot of the row behind each export
     int32_t max_tile_rows;            // LDS rows reserved per tile (imports follow them)
-    int32_t max_tile_imports;         // LDS import slots reserved per tile (group table follows)
+    int32_t max_tile_imports;         // LDS import slots reserved per tile
+    int32_t max_tile_exports;
+    int32_t max_steps;                // most groups of a tile
     unsigned* tickets;                // [0] next ticket, [1] finished tiles
     int* fault;                       // set when a bounded wait runs out
     int64_t other_nexport;            // size of the other direction's exchange buffer
     int64_t spin_limit;
-    int64_t* trace;                   // optional: [tile][2 + 2 * trace_steps] wall-clock stamps
-    int32_t trace_steps;
+    int64_t* trace;                   // optional: [tile][kTraceWords] wall-clock stamps (100 MHz)
 };
 
-constexpr int kGroupChunk = 256;      // group-table entries staged in LDS at a time
-constexpr int kImportDepth = 8;       // imports each importer lane keeps in flight
+// Workgroup roles: wavefront 0 imports, 1 computes, 2 and 3 load records,
+// 4 exports; all five move rows between HBM and LDS at the start and end.
+constexpr int kSweepWaves = 5;
+constexpr int kSweepThreads = kSweepWaves * kWave;
+constexpr int kImportDepth = 4;       // imports each importer lane polls per window
+constexpr int kFillUnroll = 26;       // row values each thread moves at once between HBM and LDS
+constexpr int kFillLists = 8;         // import and export list entries per thread in the first fill pass
+// Trace words per tile: start, compute end, compute time, polls waiting for
+// records, polls waiting for imports, fill end, first group start, drain end.
+constexpr int kTraceWords = 8;
+
+// Byte offsets of the dynamic LDS regions of a sweep workgroup: header (the
+// workgroup's counters), values (rows, imports, one zero), group table, export
+// list (16-bit slots), record ring (2 * B group slots).  The workgroup has no
+// static LDS, so the byte offsets in the records address LDS directly.
+constexpr int kSweepHeader = 64;
+struct SweepLds {
+    size_t vals, tab, xlist, ring, total;
+};
+__host__ __device__ inline SweepLds sweep_lds_layout(int rows, int imports, int steps, int exports, int vbytes,
+                                                     int C, int B)
+{
+    SweepLds l;
+    l.vals = kSweepHeader;
+    l.tab = l.vals + (size_t(rows + imports + 1) * vbytes + 15) / 16 * 16;
+    l.xlist = l.tab + size_t(steps + 3 * B + 2) * 16;
+    l.ring = l.xlist + (size_t(exports) * 2 + 15) / 16 * 16;
+    l.total = l.ring + size_t(2 * B * C * kWave) * 16;
+    return l;
+}
 
 __device__ inline int wave_min(int v)
 {
@@ -125,153 +193,230 @@ __device__ inline int wave_min(int v)
     return v;
 }
 
-// The data one lane needs for its row of a level group, loaded one level
-// ahead so that the loads overlap the previous level's arithmetic.
-template <typename FV, int MAXW>
-struct RowPre {
-    int2 meta;
-    FV diag;
-    int32_t src[MAXW];
-    FV val[MAXW];
-};
+// Native 4 x 32-bit vector: copies of it stay in registers (HIP's int4 is a
+// struct, whose copies from global memory become private-memory memcpys).
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 
-template <typename FV, bool UNIT, int MAXW>
-__device__ inline void row_prefetch(RowPre<FV, MAXW>& p, const SweepArgs& a, const FV* __restrict__ ent_val,
-                                    const FV* __restrict__ gr_diag, int4 t0, int4 t1, int q)
+// Loads the records of batch b (groups b*B .. b*B+B-1 of the tile) into
+// registers: lane q takes piece q of each group.  Groups past the tile's end
+// are empty (their table entries repeat its end) and read the record at the
+// end; lanes past a group's size reread its last record.  So every load is
+// issued and in bounds.
+template <int C, int B>
+__device__ __attribute__((always_inline)) inline void load_batch_regs(i32x4 (&buf)[B][C],
+                                                                      const i32x4* __restrict__ recs, const int4* tab,
+                                                                      int b, int lane)
 {
-    const int m = t1.x - t0.x;
-    if (q >= m) return;
-    p.meta = a.gr_meta[t0.x + q];
-    p.diag = UNIT ? FV(1) : gr_diag[t0.x + q];
-    const int w = t0.z;
 #pragma unroll
-    for (int k = 0; k < MAXW; k++)
-        if (k < w) {
-            const int64_t d = int64_t(t0.y) + int64_t(k) * m + q;
-            p.src[k] = a.ent_src[d];
-            p.val[k] = ent_val[d];
-        }
-}
-
-// One row: rr = v_r; rr -= L_rj * y_j in stored order; y_r = rr * d_r
-// (lower_trisolve.hpp:191-209 / 172-189, upper_trisolve.hpp:159-176).  Own
-// values come from the tile's LDS rows, other tiles' values from the LDS
-// import area filled by the importer wavefront.
-template <typename FV, typename VT, bool UNIT, int MAXW>
-__device__ inline void row_compute(const RowPre<FV, MAXW>& p, const SweepArgs& a, const FV* __restrict__ ent_val,
-                                   VT* lds, const VT* imp, VT* xbuf, int4 t0, int m, int q)
-{
-    const int slot = p.meta.x & 0xFFFF;
-    const int len = int(unsigned(p.meta.x) >> 16);
-    VT rr = lds[slot];
+    for (int j = 0; j < B; j++) {
+        const int gi = b * B + j;
+        const int first = tab[gi].x, m = tab[gi + 1].x - first;
+        const int q = max(0, min(lane, m - 1));
+        const i32x4* base = recs + int64_t(first) * C + q;
 #pragma unroll
-    for (int k = 0; k < MAXW; k++)
-        if (k < len) {
-            const int32_t src = p.src[k];
-            const VT yv = src >= 0 ? lds[src] : imp[-(src + 1)];
-            rr -= VT(p.val[k]) * yv;
-        }
-    for (int k = MAXW; k < len; k++) {   // rows longer than the prefetch window
-        const int64_t d = int64_t(t0.y) + int64_t(k) * m + q;
-        const int32_t src = a.ent_src[d];
-        const VT yv = src >= 0 ? lds[src] : imp[-(src + 1)];
-        rr -= VT(ent_val[d]) * yv;
+        for (int c = 0; c < C; c++) buf[j][c] = base[c * m];
     }
-    const VT out = UNIT ? rr : VT(rr * VT(p.diag));
-    lds[slot] = out;
-    if (p.meta.y >= 0)
-        __hip_atomic_store(xbuf + p.meta.y, Sentinel<VT>::canon(out), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Prefetch depth (levels) of the compute wavefront's register ring.
-template <int MAXW> struct PrefetchDepth { static constexpr int value = MAXW <= 4 ? 8 : (MAXW <= 8 ? 4 : 2); };
-
-// Workgroup = 2 wavefronts.  Wavefront 0 imports the tile's values from
-// earlier tiles (exchange buffer -> LDS) in first-use order, keeping up to
-// 64 * kImportDepth polls in flight, and publishes how many are in LDS.
-// Wavefront 1 walks the tile's levels: it waits (on LDS) until the imports a
-// level needs are present, then computes the level's rows from LDS only; its
-// per-row schedule data is prefetched PrefetchDepth levels ahead into a
-// register ring.
-template <typename FV, typename VT, bool UNIT, int MAXW>
-__global__ __launch_bounds__(128) void k_sweep(SweepArgs a, const FV* __restrict__ ent_val,
-                                               const FV* __restrict__ gr_diag, const VT* __restrict__ v,
-                                               VT* __restrict__ y, VT* xbuf, VT* xbuf_other, const int* skip)
+// Writes a batch into its ring half and publishes it.
+template <int C, int B>
+__device__ __attribute__((always_inline)) inline void store_batch_lds(const i32x4 (&buf)[B][C], i32x4* ring,
+                                                                      int half, int b, int* half_batch, int lane)
 {
-    constexpr int D = PrefetchDepth<MAXW>::value;
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+        i32x4* slot = ring + (half * B + j) * (C * kWave);
+#pragma unroll
+        for (int c = 0; c < C; c++) slot[c * kWave + lane] = buf[j][c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(half_batch + half, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// LDS at a byte offset.  The sweep workgroup has no static LDS, so its
+// dynamic LDS starts at LDS address 0 and record offsets are addresses.
+template <typename T>
+__device__ inline __attribute__((address_space(3))) T* lds_at(uint32_t off)
+{
+    return reinterpret_cast<__attribute__((address_space(3))) T*>(static_cast<uintptr_t>(off));
+}
+
+__device__ inline void wait_lds_at_least(int* p, int target)
+{
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// One workgroup per tile (ticket order), five wavefronts with fixed roles,
+// communicating through LDS counters:
+//  0  importer: polls the tile's imports from earlier tiles (exchange buffer
+//     -> LDS) in first-use order; publishes how many have arrived;
+//  1  compute: walks the tile's groups from LDS alone; once a group's
+//     records and the imports it needs are present, computes one piece per
+//     lane:  rr = (slot); rr -= L_rj * y_j in stored order; rr *= d_r
+//     (lower_trisolve.hpp:191-209 / 172-189, upper_trisolve.hpp:159-176);
+//     publishes how many groups are finished;
+//  2,3 loaders: stream the tile's piece records into the LDS ring;
+//  4  exporter: stores the finished rows that later tiles read into the
+//     exchange buffer.
+// The compute wavefront so issues no global memory operation: its pace is
+// set by LDS and arithmetic latency only.
+template <typename FV, typename VT, bool UNIT, int W>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4* __restrict__ recs,
+                                                         const VT* __restrict__ v, VT* __restrict__ y, VT* xbuf,
+                                                         VT* xbuf_other, const int* skip)
+{
+    constexpr int C = kRecChunks<FV, W>;
+    constexpr int B = kLoadBatch<FV, W>;
     if (skip && *skip) return;   // the solve this sweep belongs to has finished
     extern __shared__ unsigned char lds_raw[];
-    VT* lds = reinterpret_cast<VT*>(lds_raw);
+    // records address LDS from 0: the dynamic segment must start there (this
+    // kernel has no static LDS; the compiler folds the test away)
+    if (static_cast<uint32_t>(reinterpret_cast<uintptr_t>(
+            (__attribute__((address_space(3))) unsigned char*)(lds_raw))) != 0) {
+        if (threadIdx.x == 0) __hip_atomic_store(a.fault, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const SweepLds L = sweep_lds_layout(a.max_tile_rows, a.max_tile_imports, a.max_steps, a.max_tile_exports,
+                                        int(sizeof(VT)), C, B);
+    VT* lds = reinterpret_cast<VT*>(lds_raw + L.vals);   // rows, imports, then a zero that padding entries read
     VT* imp = lds + a.max_tile_rows;
-    int4* tab = reinterpret_cast<int4*>(imp + a.max_tile_imports);   // whole tile's group table (+D sentinel rows)
-    __shared__ int tile_sh;
-    __shared__ int progress;
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    int4* tab = reinterpret_cast<int4*>(lds_raw + L.tab);
+    uint16_t* xl = reinterpret_cast<uint16_t*>(lds_raw + L.xlist);
+    int4* ring = reinterpret_cast<int4*>(lds_raw + L.ring);
+    int* hdr = reinterpret_cast<int*>(lds_raw);
+    int& tile_sh = hdr[0];
+    int& progress = hdr[1];       // imports in LDS
+    int& consumed = hdr[2];       // groups whose records the compute wavefront no longer needs
+    int& finished = hdr[3];       // groups the compute wavefront has finished
+    int* half_batch = hdr + 4;    // batch held by each ring half
+    const int tid = threadIdx.x, lane = tid % kWave;
+    // the role of a wavefront is uniform: keep its branches scalar, so the
+    // roles' code is not linearised into one another
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     if (tid == 0) {
         tile_sh = int(atomicAdd(&a.tickets[0], 1u));
         progress = 0;
+        consumed = 0;
+        finished = 0;
+        half_batch[0] = -1;
+        half_batch[1] = -1;
+        lds[a.max_tile_rows + a.max_tile_imports] = VT(0);
     }
     __syncthreads();
     const int t = tile_sh;
-    if (a.trace && tid == 0) a.trace[int64_t(t) * (2 + 2 * a.trace_steps)] = int64_t(wall_clock64());
+    int64_t* trace = a.trace ? a.trace + int64_t(t) * kTraceWords : nullptr;
+    if (trace && tid == 0) trace[0] = int64_t(wall_clock64());
     // the other direction's exchange buffer is free: reset it for its next use
-    for (int64_t x = int64_t(t) * 128 + tid; x < a.other_nexport; x += a.ntiles * int64_t(128))
+    for (int64_t x = int64_t(t) * kSweepThreads + tid; x < a.other_nexport; x += a.ntiles * int64_t(kSweepThreads))
         xbuf_other[x] = Sentinel<VT>::value();
     const int32_t rb = a.tile_row_begin[t], re = a.tile_row_begin[t + 1];
-    for (int32_t i0 = rb + tid; i0 < re; i0 += 4 * 128) {   // rows first, then the values: 4 gathers in flight
-        int32_t r[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) r[u] = i0 + u * 128 < re ? a.rows[i0 + u * 128] : -1;
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (r[u] >= 0) lds[i0 + u * 128 - rb] = v[r[u]];
-    }
-    const int32_t g0 = a.tile_group_begin[t], g1 = a.tile_group_begin[t + 1], ng = g1 - g0;
-    for (int i = tid; i <= ng + D; i += 128) {
-        // entries past the tile's last group repeat its end, i.e. empty groups
-        tab[i] = a.group_tab[g0 + min(i, ng)];
-    }
     const int32_t ib = a.tile_import_begin[t], ni = a.tile_import_begin[t + 1] - ib;
+    const int32_t xb = a.tile_export_begin[t], nx = a.tile_export_begin[t + 1] - xb;
+    const int32_t g0 = a.tile_group_begin[t], ng = a.tile_group_begin[t + 1] - g0;
+    const int nb = (ng + B - 1) / B;   // batches
+    const int ntab = (nb + 2) * B + 1;
+    // Fill LDS in two round trips: row indices and slots, import slots,
+    // export slots and group table entries of the first pass together, then
+    // the row values.  Table entries past the tile's last group are empty
+    // groups that need nothing and follow every export; each import slot
+    // holds its exchange slot until the value arrives.
+    {
+        int32_t r[kFillUnroll], sl[kFillUnroll], xs[kFillLists], es[kFillLists];
+        int4 e = make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; u++) {
+            const int32_t i = rb + tid + u * kSweepThreads;
+            r[u] = i < re ? a.rows[i] : -1;
+            sl[u] = i < re ? a.row_slot[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kFillLists; u++) {
+            const int i = tid + u * kSweepThreads;
+            xs[u] = i < ni ? a.imports[ib + i] : -1;
+            es[u] = i < nx ? a.xlist[xb + i] : -1;
+        }
+        if (tid < ntab) e = a.group_tab[g0 + min(tid, ng)];
+        VT x[kFillUnroll];
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; u++) x[u] = r[u] >= 0 ? v[r[u]] : VT(0);
+#pragma unroll
+        for (int u = 0; u < kFillLists; u++) {
+            if (xs[u] >= 0) reinterpret_cast<int32_t*>(imp + tid + u * kSweepThreads)[0] = xs[u];
+            if (es[u] >= 0) xl[tid + u * kSweepThreads] = uint16_t(es[u]);
+        }
+        if (tid < ntab) {
+            if (tid >= ng) e = make_int4(e.x, 0, nx, 0);
+            tab[tid] = e;
+        }
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; u++)
+            if (r[u] >= 0) lds[sl[u]] = x[u];
+    }
+    // tiles larger than one pass
+    for (int32_t i0 = rb + tid + kFillUnroll * kSweepThreads; i0 < re; i0 += kFillUnroll * kSweepThreads) {
+        int32_t r[kFillUnroll], sl[kFillUnroll];
+        VT x[kFillUnroll];
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; u++) {
+            const int32_t i = i0 + u * kSweepThreads;
+            r[u] = i < re ? a.rows[i] : -1;
+            sl[u] = i < re ? a.row_slot[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; u++) x[u] = r[u] >= 0 ? v[r[u]] : VT(0);
+#pragma unroll
+        for (int u = 0; u < kFillUnroll; u++)
+            if (r[u] >= 0) lds[sl[u]] = x[u];
+    }
+    for (int i = tid + kFillLists * kSweepThreads; i < ni; i += kSweepThreads)
+        reinterpret_cast<int32_t*>(imp + i)[0] = a.imports[ib + i];
+    for (int i = tid + kFillLists * kSweepThreads; i < nx; i += kSweepThreads) xl[i] = uint16_t(a.xlist[xb + i]);
+    for (int i = tid + kSweepThreads; i < ntab; i += kSweepThreads) {
+        int4 e = a.group_tab[g0 + min(i, ng)];
+        if (i >= ng) e = make_int4(e.x, 0, nx, 0);
+        tab[i] = e;
+    }
     __syncthreads();
+    if (trace && tid == 0) trace[5] = int64_t(wall_clock64());
 
     if (wave == 0) {
         // ---- importer ----
-        // Probes only the first missing import while it is not there (one
-        // load per poll, with back-off), then loads a window of up to
-        // 64 * kImportDepth imports at once and publishes the prefix that
-        // arrived.  Waiting tiles so put almost no load on memory.
+        // Polls the next imports of the tile, in first-use order, straight
+        // from the exchange buffer into LDS and publishes the prefix that has
+        // arrived: a window of 64 * kImportDepth after progress, 64 while
+        // waiting (fewer loads on memory that is being written).
+        const int32_t* slot_of = reinterpret_cast<const int32_t*>(imp);
         int head = 0;
         int64_t polls = 0;
-        unsigned nap = 1;
+        bool wide = true;
         while (head < ni) {
-            VT hv = VT(0);
-            if (lane == 0)
-                hv = __hip_atomic_load(xbuf + a.imports[ib + head], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const bool head_ready = __shfl(int(!Sentinel<VT>::is(hv)), 0, kWave) != 0;
-            if (head_ready) {
-                nap = 1;
-                int32_t xs[kImportDepth];
-                VT val[kImportDepth];
+            int32_t xs[kImportDepth];
+            VT val[kImportDepth];
 #pragma unroll
-                for (int u = 0; u < kImportDepth; u++) {
-                    const int idx = head + lane + kWave * u;
-                    xs[u] = idx < ni ? a.imports[ib + idx] : -1;
-                }
+            for (int u = 0; u < kImportDepth; u++) {
+                const int idx = head + lane + kWave * u;
+                xs[u] = idx < ni && (u == 0 || wide) ? slot_of[idx * (sizeof(VT) / 4)] : -1;
+            }
 #pragma unroll
-                for (int u = 0; u < kImportDepth; u++)
-                    val[u] = xs[u] >= 0 ? __hip_atomic_load(xbuf + xs[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                        : VT(0);
-                int first_missing = ni;   // smallest index of the window that has not arrived
+            for (int u = 0; u < kImportDepth; u++)
+                val[u] = xs[u] >= 0 ? __hip_atomic_load(xbuf + xs[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : VT(0);
+            int first_missing = ni;   // smallest index of the window that has not arrived
 #pragma unroll
-                for (int u = 0; u < kImportDepth; u++) {
-                    const int idx = head + lane + kWave * u;
-                    if (xs[u] >= 0) {
-                        if (!Sentinel<VT>::is(val[u])) imp[idx] = val[u];
-                        else first_missing = min(first_missing, idx);
-                    }
-                }
-                const int window_end = min(ni, head + kWave * kImportDepth);
-                first_missing = wave_min(min(first_missing, window_end));
+            for (int u = 0; u < kImportDepth; u++) {
+                const int idx = head + lane + kWave * u;
+                if (xs[u] >= 0 && Sentinel<VT>::is(val[u])) first_missing = min(first_missing, idx);
+            }
+            const int window_end = min(ni, head + kWave * (wide ? kImportDepth : 1));
+            first_missing = wave_min(min(first_missing, window_end));
+#pragma unroll
+            for (int u = 0; u < kImportDepth; u++) {
+                const int idx = head + lane + kWave * u;
+                if (xs[u] >= 0 && idx < first_missing) imp[idx] = val[u];
+            }
+            wide = first_missing > head;
+            if (wide) {
                 head = first_missing;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 if (lane == 0) __hip_atomic_store(&progress, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -280,68 +425,142 @@ __global__ __launch_bounds__(128) void k_sweep(SweepArgs a, const FV* __restrict
             ++polls;
             if (polls > a.spin_limit ||
                 ((polls & 63) == 0 && __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                if (lane == 0) __hip_atomic_store(a.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // give up: flag the fault and release the compute wavefront
+                // (its results are then meaningless; the host reports the fault)
+                if (lane == 0) {
+                    __hip_atomic_store(a.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&progress, ni, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
                 break;
             }
-            for (unsigned k = 0; k < nap; k++) __builtin_amdgcn_s_sleep(2);
-            nap = min(nap * 2, 16u);
+            __builtin_amdgcn_s_sleep(2);
         }
-    } else {
+    } else if (wave == 1) {
         // ---- compute ----
-        bool failed = false;
-        RowPre<FV, MAXW> ring[D];
+        // Batch by batch: wait for the batch's records in the ring, then its
+        // groups unrolled.  The next group's record is read from LDS while
+        // the current group computes.  Lanes past a group's size hold a copy
+        // of its last record: they compute the same value and write it to the
+        // same slot, so no lane is masked off.
+        const i32x4* ringv = reinterpret_cast<const i32x4*>(ring);
+        // (No clock reads inside the loop: scalar memory operations there
+        // would make the compiler wait for every LDS read; waits are counted
+        // in polls instead.)
+        int seen = 0;    // imports known to be in LDS
+        int64_t polls_ring = 0, polls_imp = 0, w_begin = trace ? int64_t(wall_clock64()) : 0;
+        for (int b = 0; b < nb; b++) {
+            __hip_atomic_store(&consumed, b * B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            while (__hip_atomic_load(half_batch + (b & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < b) {
+                __builtin_amdgcn_s_sleep(1);
+                polls_ring++;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const i32x4* rs = ringv + (b & 1) * (B * C * kWave) + lane;
+            i32x4 rc[C];
 #pragma unroll
-        for (int u = 0; u < D; u++) row_prefetch<FV, UNIT, MAXW>(ring[u], a, ent_val, gr_diag, tab[u], tab[u + 1], lane);
-        for (int32_t g = 0; g < ng; g += D) {
+            for (int c = 0; c < C; c++) rc[c] = rs[c * kWave];
 #pragma unroll
-            for (int u = 0; u < D; u++) {
-                const int gi = g + u;
-                if (gi < ng) {
-                    const int4 t0 = tab[gi];
-                    const int m = tab[gi + 1].x - t0.x;
-                    const RowPre<FV, MAXW> cur = ring[u];
-                    row_prefetch<FV, UNIT, MAXW>(ring[u], a, ent_val, gr_diag, tab[gi + D], tab[gi + D + 1], lane);
-                    // imports this level needs
-                    int64_t polls = 0;
-                    while (!failed &&
-                           __hip_atomic_load(&progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < t0.w) {
-                        ++polls;
-                        if (polls > a.spin_limit ||
-                            ((polls & 63) == 0 && __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                            failed = true;
-                            if (lane == 0) __hip_atomic_store(a.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        }
+            for (int j = 0; j < B; j++) {
+                if (b * B + j >= ng) break;
+                // the whole record stays allocated until here (its padding
+                // bytes too), so no temporary reuses registers a pending
+                // read is still writing
+#pragma unroll
+                for (int c = 0; c < C; c++) asm volatile("" ::"v"(rc[c]));
+                SweepRec<FV, W> cur;
+                __builtin_memcpy(&cur, rc, sizeof(cur));
+                const int need = __builtin_amdgcn_readfirstlane(cur.need);
+                if (need > seen) {
+                    while ((seen = __hip_atomic_load(&progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) <
+                           need) {
                         __builtin_amdgcn_s_sleep(1);
+                        polls_imp++;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    if (a.trace && lane == 0 && gi < a.trace_steps)
-                        a.trace[int64_t(t) * (2 + 2 * a.trace_steps) + 2 + 2 * gi] = int64_t(wall_clock64());
-                    if (lane < m) row_compute<FV, VT, UNIT, MAXW>(cur, a, ent_val, lds, imp, xbuf, t0, m, lane);
-                    for (int qq = lane + kWave; qq < m; qq += kWave) {   // levels wider than the wavefront
-                        RowPre<FV, MAXW> extra;
-                        row_prefetch<FV, UNIT, MAXW>(extra, a, ent_val, gr_diag, t0, tab[gi + 1], qq);
-                        row_compute<FV, VT, UNIT, MAXW>(extra, a, ent_val, lds, imp, xbuf, t0, m, qq);
-                    }
-                    if (a.trace && lane == 0 && gi < a.trace_steps)
-                        a.trace[int64_t(t) * (2 + 2 * a.trace_steps) + 3 + 2 * gi] = int64_t(wall_clock64());
-                    // the next level reads these rows from LDS
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
+                VT yv[W];
+#pragma unroll
+                for (int k = 0; k < W; k++) yv[k] = *lds_at<VT>(cur.src[k]);
+                VT rr = *lds_at<VT>(cur.row);
+                // the next group's record, issued after this group's reads
+                // so that waiting for those does not wait for it
+                __builtin_amdgcn_sched_barrier(0);
+                if (j + 1 < B) {
+#pragma unroll
+                    for (int c = 0; c < C; c++) rc[c] = rs[((j + 1) * C + c) * kWave];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                // rr -= L_rj * y_j in stored order; padding entries subtract 0 * (+0)
+#pragma unroll
+                for (int k = 0; k < W; k++) rr = rr - VT(cur.val[k]) * yv[k];
+                if (!UNIT) rr = rr * VT(cur.diag);
+                *lds_at<VT>(cur.row) = rr;
+                // LDS serves a wavefront's operations in order: the exporter
+                // that sees this count also sees the row written above
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __hip_atomic_store(&finished, b * B + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
+        if (trace && lane == 0) {
+            trace[2] = int64_t(wall_clock64()) - w_begin;
+            trace[3] = polls_ring;
+            trace[4] = polls_imp;
+            trace[6] = w_begin;
+        }
+    } else if (wave == 4) {
+        // ---- exporter ----
+        // Follows the compute wavefront and stores each finished exported row
+        // (in finish order, so exchange slots xb, xb+1, ...) for later tiles.
+        VT* xout = xbuf + xb;
+        int e = 0;
+        while (e < nx) {
+            const int fin = __hip_atomic_load(&finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int avail = tab[fin].z;
+            if (avail == e) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            for (int i = e + lane; i < avail; i += kWave)
+                __hip_atomic_store(xout + i, Sentinel<VT>::canon(lds[xl[i]]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            e = avail;
+        }
+    } else {
+        // ---- loaders: wavefront 2 fills ring half 0, wavefront 3 half 1 ----
+        const int k = wave - 2;
+        const i32x4* rv = reinterpret_cast<const i32x4*>(recs);
+        i32x4* rr = reinterpret_cast<i32x4*>(ring);
+        i32x4 bufA[B][C], bufB[B][C];
+        load_batch_regs<C, B>(bufA, rv, tab, k, lane);
+        for (int b = k; b < nb; b += 4) {
+            load_batch_regs<C, B>(bufB, rv, tab, b + 2, lane);
+            wait_lds_at_least(&consumed, (b - 1) * B);   // batch b - 2 (same half) is finished
+            store_batch_lds<C, B>(bufA, rr, k, b, half_batch, lane);
+            if (b + 2 >= nb) break;
+            load_batch_regs<C, B>(bufA, rv, tab, b + 4, lane);
+            wait_lds_at_least(&consumed, (b + 1) * B);
+            store_batch_lds<C, B>(bufB, rr, k, b + 2, half_batch, lane);
+        }
+        // drain the batch still in flight here, so the compiler does not
+        // carry it into the other roles' code at the join below
+        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
-    if (a.trace && tid == 0) a.trace[int64_t(t) * (2 + 2 * a.trace_steps) + 1] = int64_t(wall_clock64());
-    for (int32_t i0 = rb + tid; i0 < re; i0 += 4 * 128) {
-        int32_t r[4];
+    if (trace && tid == 0) trace[1] = int64_t(wall_clock64());
+    for (int32_t i0 = rb + tid; i0 < re; i0 += kFillUnroll * kSweepThreads) {
+        int32_t r[kFillUnroll], sl[kFillUnroll];
 #pragma unroll
-        for (int u = 0; u < 4; u++) r[u] = i0 + u * 128 < re ? a.rows[i0 + u * 128] : -1;
+        for (int u = 0; u < kFillUnroll; u++) {
+            const int32_t i = i0 + u * kSweepThreads;
+            r[u] = i < re ? a.rows[i] : -1;
+            sl[u] = i < re ? a.row_slot[i] : 0;
+        }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (r[u] >= 0) y[r[u]] = lds[i0 + u * 128 - rb];
+        for (int u = 0; u < kFillUnroll; u++)
+            if (r[u] >= 0) y[r[u]] = lds[sl[u]];
     }
+    if (trace && tid == 0) trace[7] = int64_t(wall_clock64());
     if (tid == 0) {
         unsigned done = atomicAdd(&a.tickets[1], 1u);
         if (done == unsigned(a.ntiles) - 1) {   // every tile holds a ticket: reset for the next launch
@@ -349,6 +568,39 @@ __global__ __launch_bounds__(128) void k_sweep(SweepArgs a, const FV* __restrict
             a.tickets[1] = 0;
         }
     }
+}
+
+// Packs the records of one group per workgroup from the plan's position
+// arrays and the factor storage, with LDS byte offsets for vector entries of
+// vbytes bytes.
+template <typename FV, int W>
+__global__ __launch_bounds__(64) void k_build_recs(const int4* __restrict__ group_tab, const int2* __restrict__ meta,
+                                                   const int32_t* __restrict__ src,
+                                                   const int64_t* __restrict__ val_pos,
+                                                   const int64_t* __restrict__ diag_pos, const double* __restrict__ f,
+                                                   int vbytes, int4* __restrict__ recs)
+{
+    constexpr int C = kRecChunks<FV, W>;
+    const int4 g = group_tab[blockIdx.x];
+    const int first = g.x, m = group_tab[blockIdx.x + 1].x - first;
+    const int q = threadIdx.x;
+    if (q >= m) return;
+    const int64_t pc = int64_t(first) + q;
+    RecBits<FV, W> u;
+#pragma unroll
+    for (int c = 0; c < C; c++) u.c[c] = make_int4(0, 0, 0, 0);
+    u.r.row = uint32_t(kSweepHeader) + uint32_t(meta[pc].x & 0xFFFF) * uint32_t(vbytes);
+    u.r.need = g.y;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        u.r.src[k] = uint32_t(kSweepHeader) + uint32_t(src[pc * W + k]) * uint32_t(vbytes);
+        const int64_t vp = val_pos[pc * W + k];
+        u.r.val[k] = vp >= 0 ? FV(f[vp]) : FV(0);
+    }
+    const int64_t dp = diag_pos[pc];
+    u.r.diag = dp >= 0 ? FV(f[dp]) : FV(1);
+#pragma unroll
+    for (int c = 0; c < C; c++) recs[int64_t(first) * C + int64_t(c) * m + q] = u.c[c];
 }
 
 template <typename T>
@@ -396,15 +648,8 @@ static int grid_for(int64_t n, int64_t cap = 16384)
 // ---------------------------------------------------------------------------
 void SweepDev::upload(const SweepPlan& p, const double* fstore_dev, int ftype)
 {
-    plan_meta = p;
-    plan_meta.rows.clear();   // keep only scalars on the host copy
-    plan_meta.ent_src.clear();
-    plan_meta.imports.clear();
-    plan_meta.ent_val_pos.clear();
-    plan_meta.gr_slot.clear();
-    plan_meta.gr_len.clear();
-    plan_meta.gr_export.clear();
-    plan_meta.gr_diag_pos.clear();
+    M4G_REQUIRE(p.npieces * int64_t(p.width) < (int64_t(1) << 31) && p.n < (int64_t(1) << 31), M4G_ERR_INVALID,
+                "sweep too large");
     auto up32 = [](DevBuf<int32_t>& d, const std::vector<int32_t>& h) {
         d.reset(std::max<size_t>(h.size(), 1));
         d.upload(h.data(), h.size());
@@ -415,64 +660,92 @@ void SweepDev::upload(const SweepPlan& p, const double* fstore_dev, int ftype)
     };
     up32(tile_row_begin, p.tile_row_begin);
     up32(rows, p.rows);
+    up32(row_slot, p.row_slot);
     up32(tile_group_begin, p.tile_group_begin);
-    up32(ent_src, p.ent_src);
     up32(tile_import_begin, p.tile_import_begin);
     up32(imports, p.imports);
-    up64(ent_val_pos, p.ent_val_pos);
-    up64(gr_diag_pos, p.gr_diag_pos);
-    M4G_REQUIRE(p.nent < (int64_t(1) << 31) && p.n < (int64_t(1) << 31), M4G_ERR_INVALID, "sweep too large");
+    up32(tile_export_begin, p.tile_export_begin);
+    up32(xlist, p.xlist);
+    up32(pc_src, p.pc_src);
+    up64(pc_val_pos, p.pc_val_pos);
+    up64(pc_diag_pos, p.pc_diag_pos);
     std::vector<int4> tab(p.ngroups + 1);
     for (int64_t g = 0; g <= p.ngroups; g++) {
-        tab[g].x = int(p.group_row_begin[g]);
-        tab[g].y = int(p.group_ent_begin[g]);
-        tab[g].z = g < p.ngroups ? p.group_width[g] : 0;
-        tab[g].w = g < p.ngroups ? p.group_need[g] : 0;
+        tab[g].x = p.group_piece_begin[g];
+        tab[g].y = g < p.ngroups ? p.group_need[g] : 0;
+        tab[g].z = g < p.ngroups ? p.group_xbefore[g] : 0;
+        tab[g].w = 0;
     }
     group_tab.reset(tab.size());
     group_tab.upload(tab.data(), tab.size());
-    std::vector<int2> meta(std::max<int64_t>(p.n, 1));
-    max_len = 0;
-    for (int64_t i = 0; i < p.n; i++) {
-        M4G_REQUIRE(p.gr_slot[i] < 65536 && p.gr_len[i] < 65536, M4G_ERR_INVALID, "sweep row too long");
-        meta[i].x = int(unsigned(p.gr_slot[i]) | (unsigned(p.gr_len[i]) << 16));
-        meta[i].y = p.gr_export[i];
-        max_len = std::max(max_len, int(p.gr_len[i]));
+    std::vector<int2> meta(std::max<int64_t>(p.npieces, 1));
+    for (int64_t i = 0; i < p.npieces; i++) {
+        M4G_REQUIRE(p.pc_slot[i] < 65536 && p.pc_len[i] <= p.width, M4G_ERR_INVALID, "sweep piece out of range");
+        meta[i].x = int(unsigned(p.pc_slot[i]) | (unsigned(p.pc_len[i]) << 16));
+        meta[i].y = p.pc_export[i];
     }
-    gr_meta.reset(meta.size());
-    gr_meta.upload(meta.data(), meta.size());
+    pc_meta.reset(meta.size());
+    pc_meta.upload(meta.data(), meta.size());
     tickets.reset(2);
     M4G_HIP(hipMemsetAsync(tickets.ptr, 0, 2 * sizeof(unsigned), current_stream()));
     fault.reset(1);
     M4G_HIP(hipMemsetAsync(fault.ptr, 0, sizeof(int), current_stream()));
+    plan_meta = p;   // keep only scalars and offsets on the host copy
+    plan_meta.rows.clear();
+    plan_meta.row_slot.clear();
+    plan_meta.pc_slot.clear();
+    plan_meta.pc_len.clear();
+    plan_meta.pc_export.clear();
+    plan_meta.pc_diag_pos.clear();
+    plan_meta.pc_src.clear();
+    plan_meta.pc_val_pos.clear();
+    plan_meta.imports.clear();
+    plan_meta.group_piece_begin.clear();
+    plan_meta.group_need.clear();
+    plan_meta.group_xbefore.clear();
+    plan_meta.xlist.clear();
     refresh_values(fstore_dev, ftype);
+}
+
+template <typename FV, int W>
+static void build_recs_t(SweepDev& d, int vbytes, hipStream_t s)
+{
+    constexpr int C = kRecChunks<FV, W>;
+    const SweepPlan& p = d.plan_meta;
+    d.recs.reset(size_t(p.npieces + 1) * C);   // one record of padding: the ring reads past the last group
+    M4G_HIP(hipMemsetAsync(d.recs.ptr, 0, d.recs.count * sizeof(int4), s));
+    if (p.ngroups)
+        hipLaunchKernelGGL((k_build_recs<FV, W>), dim3(p.ngroups), dim3(64), 0, s, d.group_tab.ptr, d.pc_meta.ptr,
+                           d.pc_src.ptr, d.pc_val_pos.ptr, d.pc_diag_pos.ptr, d.fsrc, vbytes, d.recs.ptr);
+    M4G_HIP(hipGetLastError());
+}
+
+template <typename FV>
+static void build_recs_w(SweepDev& d, int vbytes, hipStream_t s)
+{
+    switch (d.plan_meta.width) {
+    case 3: build_recs_t<FV, 3>(d, vbytes, s); break;
+    case 4: build_recs_t<FV, 4>(d, vbytes, s); break;
+    case 8: build_recs_t<FV, 8>(d, vbytes, s); break;
+    case 16: build_recs_t<FV, 16>(d, vbytes, s); break;
+    default: throw Error(M4G_ERR_LOGIC, "unsupported sweep piece width");
+    }
 }
 
 void SweepDev::refresh_values(const double* f, int ftype)
 {
+    // records are packed at the next launch, for that launch's vector type
     factor_type = ftype;
-    hipStream_t s = current_stream();
-    const int64_t nent = plan_meta.nent, n = plan_meta.n;
-    if (ftype == M4G_F64) {
-        ent_val64.reset(std::max<int64_t>(nent, 1));
-        diag64.reset(std::max<int64_t>(n, 1));
-        if (nent)
-            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nent)), dim3(256), 0, s, nent, ent_val_pos.ptr, f,
-                               ent_val64.ptr, 0.0);
-        if (n)
-            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, s, n, gr_diag_pos.ptr, f,
-                               diag64.ptr, 1.0);
-    } else {
-        ent_val32.reset(std::max<int64_t>(nent, 1));
-        diag32.reset(std::max<int64_t>(n, 1));
-        if (nent)
-            hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(nent)), dim3(256), 0, s, nent, ent_val_pos.ptr, f,
-                               ent_val32.ptr, 0.0f);
-        if (n)
-            hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, s, n, gr_diag_pos.ptr, f,
-                               diag32.ptr, 1.0f);
-    }
-    M4G_HIP(hipGetLastError());
+    fsrc = f;
+    recs_vbytes = 0;
+}
+
+void SweepDev::ensure_recs(int vbytes, hipStream_t s)
+{
+    if (recs_vbytes == vbytes) return;
+    if (factor_type == M4G_F64) build_recs_w<double>(*this, vbytes, s);
+    else build_recs_w<float>(*this, vbytes, s);
+    recs_vbytes = vbytes;
 }
 
 void SweepDev::ensure_exchange(int vtype)
@@ -499,65 +772,89 @@ static int64_t spin_limit()
     return lim;
 }
 
+// Dynamic LDS of a sweep launch (layout in sweep_lds_layout).
+static size_t sweep_lds_bytes(const SweepPlan& p, size_t vbytes, int ftype)
+{
+    const int C = rec_chunks(ftype, p.width);
+    return sweep_lds_layout(int(p.max_tile_rows), int(p.max_tile_imports), int(p.max_steps), int(p.max_tile_exports),
+                            int(vbytes), C, load_batch(C))
+        .total;
+}
+
+template <typename FV, typename VT, bool UNIT, int W>
+static void launch_sweep_w(SweepDev& d, const SweepArgs& a, const VT* v, VT* y, VT* xo, const int* skip,
+                           hipStream_t s)
+{
+    const SweepPlan& p = d.plan_meta;
+    static_assert(sizeof(SweepRec<FV, W>) % 16 == 0, "record size");
+    auto kern = k_sweep<FV, VT, UNIT, W>;
+    constexpr int ftype = std::is_same<FV, double>::value ? M4G_F64 : M4G_F32;
+    const size_t lds = sweep_lds_bytes(p, sizeof(VT), ftype);
+    const bool layout_ok = rec_chunks(ftype, W) == kRecChunks<FV, W>;
+    M4G_REQUIRE(layout_ok, M4G_ERR_LOGIC, "record layout mismatch");
+    // static shared words (ticket, progress) come on top of the dynamic part
+    if (lds + 64 > 65536)
+        M4G_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds)));
+    hipLaunchKernelGGL(kern, dim3(p.ntiles), dim3(kSweepThreads), lds, s, a, d.recs.ptr, v, y, (VT*)d.xbuf.ptr, xo,
+                       skip);
+}
+
 template <typename FV, typename VT>
 static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, const int* skip, hipStream_t s)
 {
     const SweepPlan& p = d.plan_meta;
+    if (p.ntiles == 0) return;
+    d.ensure_recs(int(sizeof(VT)), s);
     SweepArgs a;
     a.ntiles = p.ntiles;
     a.tile_row_begin = d.tile_row_begin.ptr;
     a.rows = d.rows.ptr;
+    a.row_slot = d.row_slot.ptr;
     a.tile_group_begin = d.tile_group_begin.ptr;
     a.group_tab = d.group_tab.ptr;
-    a.gr_meta = d.gr_meta.ptr;
     a.tile_import_begin = d.tile_import_begin.ptr;
     a.imports = d.imports.ptr;
+    a.tile_export_begin = d.tile_export_begin.ptr;
+    a.xlist = d.xlist.ptr;
+    a.max_tile_exports = int32_t(p.max_tile_exports);
     a.max_tile_rows = int32_t(p.max_tile_rows);
     a.max_tile_imports = int32_t(p.max_tile_imports);
-    a.ent_src = d.ent_src.ptr;
+    a.max_steps = int32_t(p.max_steps);
     a.tickets = d.tickets.ptr;
     a.fault = d.fault.ptr;
     a.other_nexport = other.plan_meta.nexport;
     a.spin_limit = spin_limit();
     a.trace = nullptr;
-    a.trace_steps = 0;
     static const char* trace_path = std::getenv("M4G_SWEEP_TRACE");
     static int traced = 0;
     DevBuf<int64_t> trace_buf;
     if (trace_path && traced < 4) {
-        a.trace_steps = int32_t(p.max_steps);
-        trace_buf.reset(size_t(p.ntiles) * (2 + 2 * p.max_steps));
+        trace_buf.reset(size_t(p.ntiles) * kTraceWords);
         M4G_HIP(hipMemsetAsync(trace_buf.ptr, 0, trace_buf.count * sizeof(int64_t), s));
         a.trace = trace_buf.ptr;
     }
-    const FV* ev = std::is_same<FV, double>::value ? (const FV*)d.ent_val64.ptr : (const FV*)d.ent_val32.ptr;
-    const FV* dg = std::is_same<FV, double>::value ? (const FV*)d.diag64.ptr : (const FV*)d.diag32.ptr;
-    // rows, imports, then the tile's group table (+ prefetch padding), 16-byte aligned
-    size_t lds = size_t(p.max_tile_rows + p.max_tile_imports) * sizeof(VT);
-    lds = (lds + 15) / 16 * 16 + size_t(p.max_steps + 16 + 1) * sizeof(int4);
-    if (p.ntiles == 0) return;
-    auto pick = [&](auto unit_tag) {
+    VT* xo = (VT*)other.xbuf.ptr;
+    auto go = [&](auto unit_tag) {
         constexpr bool U = decltype(unit_tag)::value;
-        if (d.max_len <= 4) return k_sweep<FV, VT, U, 4>;
-        if (d.max_len <= 8) return k_sweep<FV, VT, U, 8>;
-        return k_sweep<FV, VT, U, 16>;
+        switch (p.width) {
+        case 3: launch_sweep_w<FV, VT, U, 3>(d, a, v, y, xo, skip, s); break;
+        case 4: launch_sweep_w<FV, VT, U, 4>(d, a, v, y, xo, skip, s); break;
+        case 8: launch_sweep_w<FV, VT, U, 8>(d, a, v, y, xo, skip, s); break;
+        case 16: launch_sweep_w<FV, VT, U, 16>(d, a, v, y, xo, skip, s); break;
+        default: throw Error(M4G_ERR_LOGIC, "unsupported sweep piece width");
+        }
     };
-    auto kern = p.unit ? pick(std::true_type()) : pick(std::false_type());
-    if (lds + 64 > 65536) {
-        // beyond 64 KB of LDS per workgroup (with the static group table) must be requested
-        M4G_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    }
-    hipLaunchKernelGGL(kern, dim3(p.ntiles), dim3(128), lds, s, a, ev, dg, v, y, (VT*)d.xbuf.ptr,
-                       (VT*)other.xbuf.ptr, skip);
+    if (p.unit) go(std::true_type());
+    else go(std::false_type());
     M4G_HIP(hipGetLastError());
     if (a.trace) {
-        // trace file: header (ntiles, steps) then the stamps of every tile
+        // trace file: header (ntiles, groups, words) then the stamps of every tile
         std::vector<int64_t> h(trace_buf.count);
         trace_buf.download(h.data(), h.size());
         std::string path = std::string(trace_path) + "." + std::to_string(traced++);
         if (FILE* f = std::fopen(path.c_str(), "wb")) {
-            int64_t hdr[2] = {p.ntiles, p.max_steps};
+            int64_t hdr[3] = {p.ntiles, p.max_steps, kTraceWords};
             std::fwrite(hdr, sizeof(hdr), 1, f);
             std::fwrite(h.data(), sizeof(int64_t), h.size(), f);
             std::fclose(f);
@@ -620,28 +917,26 @@ int64_t sweep_tile_cap(int vtype)
     return cap;
 }
 
-SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int vtype)
+SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype)
 {
-    // a tile's rows and imports share the workgroup's LDS
+    // a tile's rows, imports, group table and record ring share the
+    // workgroup's LDS; sized for double vectors, the widest the sweeps use
     int lds = 65536, dev = 0;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
             lds = std::min(v, 160 * 1024);
     }
-    // sized for double vectors, the widest the sweeps run with
-    const int64_t elem = 8;
-    (void)vtype;
+    const size_t budget = size_t(lds) - 64;   // static shared words
     SweepPlan p;
     for (;;) {
-        TileShape shape = choose_tiles(rows, g, cap, env_double("M4G_SWEEP_HOP_US", 1.5),
-                                       env_double("M4G_SWEEP_STEP_US", 0.15));
+        TileShape shape = choose_tiles(rows, g, cap, env_double("M4G_SWEEP_HOP_US", kSweepHopUs),
+                                       env_double("M4G_SWEEP_STEP_US", kSweepStepUs));
         p = build_sweep_plan(rows, g, shape, cap);
-        if ((p.max_tile_rows + p.max_tile_imports) * elem + (p.max_steps + 17) * 16 + 64 <= lds || cap <= 64) break;
+        if (sweep_lds_bytes(p, 8, ftype) <= budget || cap <= 64) break;
         cap = cap * 3 / 4;
     }
-    M4G_REQUIRE((p.max_tile_rows + p.max_tile_imports) * elem + (p.max_steps + 17) * 16 + 64 <= lds, M4G_ERR_INVALID,
-                "sweep tile does not fit in LDS");
+    M4G_REQUIRE(sweep_lds_bytes(p, 8, ftype) <= budget, M4G_ERR_INVALID, "sweep tile does not fit in LDS");
     if (std::getenv("M4G_VALIDATE_PLANS")) {
         std::string msg = validate_sweep_plan(p, rows);
         M4G_REQUIRE(msg.empty(), M4G_ERR_LOGIC, "sweep plan invalid: " + msg);

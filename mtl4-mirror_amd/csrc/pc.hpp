@@ -6,16 +6,20 @@
 
 namespace m4g {
 
-// A compiled sweep on the device.
+// A compiled sweep on the device.  Each piece of the plan becomes one packed
+// record (layout in pc.hip); records are rebuilt from the position arrays when
+// the factor values change.
 struct SweepDev {
-    SweepPlan plan_meta;                  // scalars and tile/group offsets (large arrays dropped)
-    DevBuf<int32_t> tile_row_begin, rows, tile_group_begin, ent_src, tile_import_begin, imports;
-    DevBuf<int4> group_tab;               // per group: first group row, first entry, width
-    DevBuf<int2> gr_meta;                 // per group row: slot | len << 16, export slot
-    DevBuf<int64_t> ent_val_pos, gr_diag_pos;
-    int max_len = 0;                      // longest row of the sweep
-    DevBuf<double> ent_val64, diag64;
-    DevBuf<float> ent_val32, diag32;
+    SweepPlan plan_meta;                  // scalars and tile offsets (piece arrays dropped)
+    DevBuf<int32_t> tile_row_begin, rows, row_slot, tile_group_begin, tile_import_begin, imports;
+    DevBuf<int32_t> tile_export_begin, xlist;
+    DevBuf<int4> group_tab;               // per group (+1): first piece, imports needed, exports before
+    DevBuf<int2> pc_meta;                 // per piece: slot | len << 16, export slot
+    DevBuf<int32_t> pc_src;               // per piece and entry: LDS index
+    DevBuf<int64_t> pc_val_pos, pc_diag_pos;
+    DevBuf<int4> recs;                    // packed records for factor_type and recs_vbytes
+    const double* fsrc = nullptr;         // factor storage the records are packed from
+    int recs_vbytes = 0;                  // vector entry size the records address (0: stale)
     DevBuf<unsigned char> xbuf;           // exchange buffer (double or float entries)
     int xbuf_type = -1;
     DevBuf<unsigned> tickets;
@@ -24,6 +28,7 @@ struct SweepDev {
 
     void upload(const SweepPlan& p, const double* fstore_dev, int ftype);
     void refresh_values(const double* fstore_dev, int ftype);
+    void ensure_recs(int vbytes, hipStream_t s);
     void ensure_exchange(int vtype);
 };
 
@@ -31,7 +36,7 @@ void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp,
                     const int* skip, hipStream_t s);
 int check_sweep_fault(SweepDev& d);
 int64_t sweep_tile_cap(int vtype);
-SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int vtype);
+SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype);
 
 }  // namespace m4g
 

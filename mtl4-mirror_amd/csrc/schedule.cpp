@@ -2,7 +2,8 @@
 //
 // A sweep (forward on L or backward on U) is split into tiles of rows.  Each
 // tile is executed by one workgroup that keeps the tile's vector entries in
-// LDS and walks the tile's local dependency levels, one LDS barrier per level.
+// LDS and walks the tile's local dependency levels in groups of at most 64
+// rows, one wavefront pass per group.
 // Values needed by later tiles are exported through a device exchange buffer.
 // Tiles are numbered in a topological order of the tile graph and handed to
 // workgroups by an atomic ticket, so every wait of a tile is on a tile with a
@@ -168,6 +169,26 @@ static inline void grid_coords(const GridInfo& g, int64_t r, int64_t& i, int64_t
     k = r / (g.nx * g.ny);
 }
 
+// Wavefront passes per level of a box tile of extents x1, x2, x3 in which
+// cell (t1, t2, t3) sits on local level w1*t1 + w2*t2 + w3*t3: a level of
+// more than 64 cells takes several passes.  Returns passes / levels.
+static double passes_per_level(int64_t x1, int64_t x2, int64_t x3, int64_t w1, int64_t w2, int64_t w3)
+{
+    std::vector<int64_t> cnt(size_t(w1 * (x1 - 1) + w2 * (x2 - 1) + w3 * (x3 - 1) + 1), 0);
+    for (int64_t t1 = 0; t1 < x1; t1++)
+        for (int64_t t2 = 0; t2 < x2; t2++) {
+            const int64_t base = w1 * t1 + w2 * t2;
+            for (int64_t t3 = 0; t3 < x3; t3++) cnt[size_t(base + w3 * t3)]++;
+        }
+    int64_t groups = 0, levels = 0;
+    for (int64_t c : cnt)
+        if (c > 0) {
+            groups += (c + 63) / 64;
+            levels++;
+        }
+    return levels ? double(groups) / double(levels) : 1.0;
+}
+
 TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, double hop_us, double step_us)
 {
     TileShape best;
@@ -213,15 +234,12 @@ TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, doubl
                 if (rows > cap) continue;
                 int64_t n1 = (ext1 + e1 - 1) / e1, n2 = (ext2 + e2 - 1) / e2, n3 = (ext3 + e3 - 1) / e3;
                 double hops = double(n1 - 1 + n2 - 1 + n3 - 1);
-                double span = w3 * double(std::min(e3, ext3) - 1) + w2 * double(std::min(e2, ext2) - 1) +
-                              w1 * double(std::min(e1, ext1) - 1) + 1.0;
-                span = std::min(span, double(nlev));
                 // every global level lies on the critical path whatever the
-                // tiling; tiles add a hop latency at each boundary crossed
-                double per_level = double(rows) / std::max(span, 1.0);
-                // one compute wavefront per tile: rows beyond 64 per level are
-                // extra passes of the level
-                double passes = 1.0 + 0.5 * (std::max(1.0, std::ceil(per_level / 64.0)) - 1.0);
+                // tiling, at the pace of the tile's wavefront passes per level;
+                // tiles add a hop latency at each boundary crossed
+                double passes = passes_per_level(std::min(e1, ext1), std::min(e2, ext2), std::min(e3, ext3),
+                                                 int64_t(std::llround(w1)), int64_t(std::llround(w2)),
+                                                 int64_t(std::llround(w3)));
                 double cost = hops * hop_us + double(nlev) * passes * step_us;
                 if (cost < best_cost) {
                     best_cost = cost;
@@ -243,11 +261,25 @@ TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, doubl
 // ---------------------------------------------------------------------------
 // plan construction
 // ---------------------------------------------------------------------------
-SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t cap)
+int sweep_piece_width(int64_t max_len)
+{
+    for (int w : {3, 4, 8})
+        if (max_len <= w) return w;
+    return 16;
+}
+
+SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t cap, int width)
 {
     SweepPlan p;
     const int64_t n = s.n;
     p.n = n;
+    if (width <= 0 && std::getenv("M4G_SWEEP_WIDTH")) width = std::atoi(std::getenv("M4G_SWEEP_WIDTH"));
+    if (width <= 0) {
+        int64_t max_len = 0;
+        for (int64_t r = 0; r < n; r++) max_len = std::max(max_len, s.ptr[r + 1] - s.ptr[r]);
+        width = sweep_piece_width(max_len);
+    }
+    p.width = width;
     p.lower = s.lower;
     p.unit = true;
     for (int64_t r = 0; r < n; r++)
@@ -299,106 +331,131 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
         p.max_tile_rows = std::max<int64_t>(p.max_tile_rows, p.tile_row_begin[t + 1] - p.tile_row_begin[t]);
     if (p.max_tile_rows > cap) throw std::runtime_error("sweep tile exceeds LDS capacity");
 
-    // 3. local levels and exports
-    std::vector<int32_t> llev(n, 0);
+    // 3. pieces and their local levels; rows read by other tiles are exported
+    const int W = p.width;
+    struct Piece {
+        int32_t row, k0, len, level;
+    };
+    std::vector<Piece> pieces;
+    pieces.reserve(size_t(n) + size_t(n) / 4);
+    std::vector<int32_t> llev(n, 0);   // local level of the row's last piece
     std::vector<char> exported(n, 0);
     for (int64_t q = 0; q < n; q++) {
         int64_t r = s.lower ? q : n - 1 - q;
-        int32_t l = 0;
-        for (int64_t e = s.ptr[r]; e < s.ptr[r + 1]; e++) {
-            int64_t c = s.col[e];
-            if (tile[c] == tile[r]) l = std::max(l, llev[c] + 1);
-            else exported[c] = 1;
-        }
-        llev[r] = l;
-    }
-    // export slots: contiguous per tile, in slot order
-    std::vector<int32_t> xslot(n, -1);
-    p.tile_export_begin.assign(p.ntiles + 1, 0);
-    {
-        int32_t next = 0;
-        for (int64_t t = 0; t < p.ntiles; t++) {
-            p.tile_export_begin[t] = next;
-            for (int32_t q = p.tile_row_begin[t]; q < p.tile_row_begin[t + 1]; q++) {
-                int32_t r = p.rows[q];
-                if (exported[r]) xslot[r] = next++;
+        const int64_t len = s.ptr[r + 1] - s.ptr[r];
+        int32_t prev = -1;
+        for (int64_t k0 = 0; k0 == 0 || k0 < len; k0 += W) {
+            const int64_t k1 = std::min<int64_t>(len, k0 + W);
+            int32_t l = prev + 1;
+            for (int64_t e = s.ptr[r] + k0; e < s.ptr[r] + k1; e++) {
+                int64_t c = s.col[e];
+                if (tile[c] == tile[r]) l = std::max(l, llev[c] + 1);
+                else exported[c] = 1;
             }
+            pieces.push_back({int32_t(r), int32_t(k0), int32_t(k1 - k0), l});
+            prev = l;
         }
-        p.tile_export_begin[p.ntiles] = next;
-        p.nexport = next;
+        llev[r] = prev;
     }
-
-    // 4. level groups per tile: rows ordered by (tile, local level, processing order)
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
+    p.npieces = int64_t(pieces.size());
+    if (p.npieces >= (int64_t(1) << 31) / std::max(W, 1)) throw std::runtime_error("sweep too large");
+    // 4. groups: pieces ordered by (tile, local level, processing order),
+    //    each run of one (tile, level) cut into groups of at most 64
+    std::vector<int32_t> order(p.npieces);
+    std::iota(order.begin(), order.end(), 0);   // pieces were made in processing order
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        if (tile[x] != tile[y]) return tile[x] < tile[y];
-        if (llev[x] != llev[y]) return llev[x] < llev[y];
-        return s.lower ? x < y : x > y;
+        const Piece &a = pieces[x], &b = pieces[y];
+        if (tile[a.row] != tile[b.row]) return tile[a.row] < tile[b.row];
+        return a.level < b.level;
     });
-    p.tile_group_begin.assign(p.ntiles + 1, 0);
-    p.group_row_begin.clear();
-    p.group_width.clear();
-    p.gr_slot.resize(n);
-    p.gr_len.resize(n);
-    p.gr_export.resize(n);
-    p.gr_diag_pos.resize(n);
-    int64_t maxgroup = 1;
-    for (int64_t q = 0; q < n;) {
-        int32_t r0 = order[q];
+    p.group_piece_begin.clear();
+    for (int64_t q = 0; q < p.npieces;) {
+        const Piece& h = pieces[order[q]];
         int64_t q1 = q;
-        int32_t width = 0;
-        while (q1 < n && tile[order[q1]] == tile[r0] && llev[order[q1]] == llev[r0]) {
-            int32_t r = order[q1];
-            width = std::max<int32_t>(width, int32_t(s.ptr[r + 1] - s.ptr[r]));
+        while (q1 < p.npieces && q1 - q < 64 && tile[pieces[order[q1]].row] == tile[h.row] &&
+               pieces[order[q1]].level == h.level)
             q1++;
-        }
-        p.group_row_begin.push_back(q);
-        p.group_width.push_back(width);
-        maxgroup = std::max<int64_t>(maxgroup, q1 - q);
+        p.group_piece_begin.push_back(int32_t(q));
         q = q1;
     }
-    p.ngroups = int64_t(p.group_width.size());
-    p.group_row_begin.push_back(n);
-    // tiles are contiguous in order[]; fix group ranges per tile
-    {
-        std::vector<int32_t> tg(p.ntiles + 1, int32_t(p.ngroups));
-        for (int64_t gi = p.ngroups - 1; gi >= 0; gi--) tg[tile[order[p.group_row_begin[gi]]]] = int32_t(gi);
-        for (int64_t t = p.ntiles - 1; t >= 0; t--)
-            if (p.tile_row_begin[t + 1] == p.tile_row_begin[t]) tg[t] = tg[t + 1];
-        p.tile_group_begin.assign(tg.begin(), tg.end());
-        p.tile_group_begin[p.ntiles] = int32_t(p.ngroups);
-    }
+    p.ngroups = int64_t(p.group_piece_begin.size());
+    p.group_piece_begin.push_back(int32_t(p.npieces));
+    p.tile_group_begin.assign(p.ntiles + 1, int32_t(p.ngroups));
+    for (int64_t gi = p.ngroups - 1; gi >= 0; gi--)
+        p.tile_group_begin[tile[pieces[order[p.group_piece_begin[gi]]].row]] = int32_t(gi);
+    for (int64_t t = p.ntiles - 1; t >= 0; t--)   // tiles without rows: empty group range
+        if (p.tile_row_begin[t + 1] == p.tile_row_begin[t]) p.tile_group_begin[t] = p.tile_group_begin[t + 1];
     for (int64_t t = 0; t < p.ntiles; t++)
         p.max_steps = std::max<int64_t>(p.max_steps, p.tile_group_begin[t + 1] - p.tile_group_begin[t]);
-
-    // 5. entries (ELL inside each group)
-    p.group_ent_begin.assign(p.ngroups + 1, 0);
-    for (int64_t gi = 0; gi < p.ngroups; gi++) {
-        int64_t m = p.group_row_begin[gi + 1] - p.group_row_begin[gi];
-        p.group_ent_begin[gi + 1] = p.group_ent_begin[gi] + m * p.group_width[gi];
-    }
-    p.nent = p.group_ent_begin[p.ngroups];
-    p.ent_src.assign(p.nent, 0);
-    p.ent_val_pos.assign(p.nent, -1);
-    for (int64_t gi = 0; gi < p.ngroups; gi++) {
-        int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
-        for (int64_t q = 0; q < m; q++) {
-            int32_t r = order[b + q];
-            p.gr_slot[b + q] = slot[r];
-            p.gr_len[b + q] = int32_t(s.ptr[r + 1] - s.ptr[r]);
-            p.gr_export[b + q] = xslot[r];
-            p.gr_diag_pos[b + q] = s.diag_pos[r];
-            for (int64_t k = 0; k < s.ptr[r + 1] - s.ptr[r]; k++) {
-                int64_t e = s.ptr[r] + k, c = s.col[e];
-                int64_t d = p.group_ent_begin[gi] + k * m + q;
-                p.ent_src[d] = tile[c] == tile[r] ? slot[c] : -(xslot[c] + 1);
-                p.ent_val_pos[d] = s.val_pos[e];
+    // LDS slots in (group, lane) order: the rows of a group, and on a grid
+    // the rows they read from the group before, sit in consecutive slots, so
+    // a wavefront's LDS accesses fall in distinct banks
+    {
+        std::vector<int32_t> next(p.ntiles, 0);
+        std::vector<char> placed(n, 0);
+        for (int64_t q = 0; q < p.npieces; q++) {
+            const int32_t r = pieces[order[q]].row;
+            if (!placed[r]) {
+                placed[r] = 1;
+                slot[r] = next[tile[r]]++;
             }
+        }
+        p.row_slot.resize(n);
+        for (int64_t q = 0; q < n; q++) p.row_slot[q] = slot[p.rows[q]];
+    }
+    // export slots: contiguous per tile, in the order the rows finish (group
+    // order of their last pieces); each tile's list of exported LDS slots, and
+    // per group the tile's exports finished before it
+    std::vector<int32_t> xslot(n, -1);
+    p.tile_export_begin.assign(p.ntiles + 1, 0);
+    p.xlist.clear();
+    p.group_xbefore.assign(p.ngroups, 0);
+    for (int64_t t = 0; t < p.ntiles; t++) {
+        const int32_t xb = int32_t(p.xlist.size());
+        p.tile_export_begin[t] = xb;
+        for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
+            p.group_xbefore[gi] = int32_t(p.xlist.size()) - xb;
+            for (int32_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++) {
+                const Piece& pc = pieces[order[q]];
+                const int32_t r = pc.row;
+                if (exported[r] && pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r]) {
+                    xslot[r] = int32_t(p.xlist.size());
+                    p.xlist.push_back(slot[r]);
+                }
+            }
+        }
+        p.max_tile_exports = std::max<int64_t>(p.max_tile_exports, int64_t(p.xlist.size()) - xb);
+    }
+    p.tile_export_begin[p.ntiles] = int32_t(p.xlist.size());
+    p.nexport = int64_t(p.xlist.size());
+
+    // 5. piece records in group order; sources still as slot / -(export + 1)
+    p.pc_slot.resize(p.npieces);
+    p.pc_len.resize(p.npieces);
+    p.pc_export.resize(p.npieces);
+    p.pc_diag_pos.resize(p.npieces);
+    p.pc_src.assign(size_t(p.npieces) * W, 0);
+    p.pc_val_pos.assign(size_t(p.npieces) * W, -1);
+    for (int64_t q = 0; q < p.npieces; q++) {
+        const Piece& pc = pieces[order[q]];
+        const int32_t r = pc.row;
+        const bool last = pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r];
+        p.pc_slot[q] = slot[r];
+        p.pc_len[q] = pc.len;
+        p.pc_export[q] = last ? xslot[r] : -1;
+        p.pc_diag_pos[q] = last ? s.diag_pos[r] : -1;
+        for (int32_t k = 0; k < W; k++) {
+            if (k >= pc.len) {
+                p.pc_src[q * W + k] = -1;   // padding: the zero slot, set below
+                continue;
+            }
+            const int64_t e = s.ptr[r] + pc.k0 + k, c = s.col[e];
+            p.pc_src[q * W + k] = tile[c] == tile[r] ? slot[c] : -(xslot[c] + 1);
+            p.pc_val_pos[q * W + k] = s.val_pos[e];
         }
     }
     // 6. imports: the values each tile reads from other tiles, numbered per
-    //    tile in order of first use; entries refer to them by tile-local index
+    //    tile in order of first use; they follow the rows in LDS
     p.tile_import_begin.assign(p.ntiles + 1, 0);
     p.group_need.assign(p.ngroups, 0);
     p.imports.clear();
@@ -408,18 +465,16 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
             const int32_t ib = int32_t(p.imports.size());
             p.tile_import_begin[t] = ib;
             for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
-                int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
-                for (int64_t q = 0; q < m; q++)
-                    for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
-                        int64_t d = p.group_ent_begin[gi] + k * m + q;
-                        int32_t src = p.ent_src[d];
+                for (int64_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++)
+                    for (int32_t k = 0; k < p.pc_len[q]; k++) {
+                        int32_t& src = p.pc_src[q * W + k];
                         if (src >= 0) continue;
-                        int32_t xs = -(src + 1);
+                        const int32_t xs = -(src + 1);
                         if (local[xs] < 0) {
                             local[xs] = int32_t(p.imports.size()) - ib;
                             p.imports.push_back(xs);
                         }
-                        p.ent_src[d] = -(local[xs] + 1);
+                        src = int32_t(p.max_tile_rows) + local[xs];
                     }
                 p.group_need[gi] = int32_t(p.imports.size()) - ib;
             }
@@ -429,8 +484,12 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
         p.tile_import_begin[p.ntiles] = int32_t(p.imports.size());
         p.nimports = int64_t(p.imports.size());
     }
-    int64_t blk = ((std::min<int64_t>(maxgroup, 1024) + 63) / 64) * 64;
-    p.block = int(std::max<int64_t>(64, std::min<int64_t>(blk, 256)));
+    // padding entries read the zero slot after the imports with value 0:
+    // rr - 0 * (+0) == rr exactly for every rr, so padding changes nothing
+    if (p.max_tile_rows + p.max_tile_imports + 1 > 65535) throw std::runtime_error("sweep tile exceeds LDS index range");
+    const int32_t zero_slot = int32_t(p.max_tile_rows + p.max_tile_imports);
+    for (int64_t q = 0; q < p.npieces; q++)
+        for (int32_t k = p.pc_len[q]; k < W; k++) p.pc_src[q * W + k] = zero_slot;
     return p;
 }
 
@@ -440,102 +499,115 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
 std::string validate_sweep_plan(const SweepPlan& p, const SweepRows& s)
 {
     const int64_t n = p.n;
+    const int W = p.width;
     if (int64_t(p.rows.size()) != n) return "rows size";
-    std::vector<int32_t> tile_of(n, -1), slot_of(n, -1), grp_of(n, -1), x_of(n, -1);
+    if (int64_t(p.row_slot.size()) != n) return "row_slot size";
+    std::vector<int32_t> tile_of(n, -1), slot_of(n, -1), x_of(n, -1), row_at(n, -1);
     for (int64_t t = 0; t < p.ntiles; t++) {
-        if (p.tile_row_begin[t + 1] - p.tile_row_begin[t] > p.max_tile_rows) return "tile rows";
-        for (int32_t q = p.tile_row_begin[t]; q < p.tile_row_begin[t + 1]; q++) {
+        const int32_t rb = p.tile_row_begin[t], nrows = p.tile_row_begin[t + 1] - rb;
+        if (nrows > p.max_tile_rows) return "tile rows";
+        for (int32_t q = rb; q < rb + nrows; q++) {
             int32_t r = p.rows[q];
             if (r < 0 || r >= n || tile_of[r] != -1) return "row listed twice or out of range";
+            const int32_t sl = p.row_slot[q];
+            if (sl < 0 || sl >= nrows || row_at[rb + sl] != -1) return "slots are not a permutation";
             tile_of[r] = int32_t(t);
-            slot_of[r] = q - p.tile_row_begin[t];
+            slot_of[r] = sl;
+            row_at[rb + sl] = r;
         }
     }
     for (int64_t r = 0; r < n; r++)
         if (tile_of[r] < 0) return "row missing";
-    std::vector<int64_t> group_rank(n, -1);
-    int64_t seen_rows = 0;
+    if (p.group_piece_begin.front() != 0 || p.group_piece_begin.back() != p.npieces) return "piece range";
+    // walk each tile's groups: pieces of a row in entry order, dependencies final
+    std::vector<int64_t> done_k(n, 0);        // entries of the row consumed so far
+    std::vector<int64_t> final_group(n, -1);  // group of the row's last piece
+    std::vector<int64_t> piece_group(n, -1);  // group of the row's latest piece
     for (int64_t t = 0; t < p.ntiles; t++) {
+        const int32_t rb = p.tile_row_begin[t], nrows = p.tile_row_begin[t + 1] - rb;
+        int32_t xcount = 0;   // exports of the tile so far
         for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
-            for (int64_t q = p.group_row_begin[gi]; q < p.group_row_begin[gi + 1]; q++) {
-                int32_t sl = p.gr_slot[q];
-                if (sl < 0 || sl >= p.tile_row_begin[t + 1] - p.tile_row_begin[t]) return "group slot range";
-                int32_t r = p.rows[p.tile_row_begin[t] + sl];
-                if (grp_of[r] != -1) return "row in two groups";
-                grp_of[r] = gi;
-                x_of[r] = p.gr_export[q];
-                if (p.gr_len[q] != s.ptr[r + 1] - s.ptr[r]) return "row length";
-                if (p.gr_len[q] > p.group_width[gi]) return "group width";
-                if (p.gr_diag_pos[q] != s.diag_pos[r]) return "diag pos";
-                seen_rows++;
-            }
-        }
-    }
-    if (seen_rows != n) return "groups do not cover all rows";
-    for (int64_t t = 0; t < p.ntiles; t++)
-        for (int32_t q = p.tile_row_begin[t]; q < p.tile_row_begin[t + 1]; q++) {
-            int32_t r = p.rows[q];
-            int32_t xs = x_of[r];
-            if (xs >= 0 && (xs < p.tile_export_begin[t] || xs >= p.tile_export_begin[t + 1])) return "export slot owner";
-        }
-    // dependency checks: same tile -> earlier group; other tile -> earlier ticket and exported
-    for (int64_t gi = 0; gi < p.ngroups; gi++) {
-        int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
-        for (int64_t q = 0; q < m; q++) {
-            int32_t t = -1;
-            // find the row: slot -> row via its tile (the tile of group gi)
-            // (groups belong to exactly one tile)
-            int64_t lo = 0, hi = p.ntiles;
-            while (hi - lo > 1) {
-                int64_t mid = (lo + hi) / 2;
-                if (p.tile_group_begin[mid] <= gi) lo = mid; else hi = mid;
-            }
-            t = int32_t(lo);
-            int32_t r = p.rows[p.tile_row_begin[t] + p.gr_slot[b + q]];
-            for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
-                int64_t e = s.ptr[r] + k, c = s.col[e];
-                int64_t d = p.group_ent_begin[gi] + k * m + q;
-                if (p.ent_val_pos[d] != s.val_pos[e]) return "entry value position";
-                int32_t src = p.ent_src[d];
-                if (tile_of[c] == t) {
-                    if (src != slot_of[c]) return "intra-tile source";
-                    if (grp_of[c] >= gi) return "intra-tile dependency not in an earlier group";
-                } else {
-                    if (tile_of[c] >= t) return "dependency on a later tile";
-                    int32_t imp = -(src + 1);
-                    if (imp < 0 || p.tile_import_begin[t] + imp >= p.tile_import_begin[t + 1]) return "import range";
-                    if (x_of[c] < 0 || p.imports[p.tile_import_begin[t] + imp] != x_of[c]) return "missing export";
-                    if (imp >= p.group_need[gi]) return "import not counted before its first use";
+            const int64_t m = p.group_piece_begin[gi + 1] - p.group_piece_begin[gi];
+            if (m < 1 || m > 64) return "group size";
+            if (p.group_xbefore[gi] != xcount) return "exports before group";
+            for (int64_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++) {
+                const int32_t sl = p.pc_slot[q];
+                if (sl < 0 || sl >= nrows) return "piece slot range";
+                const int32_t r = row_at[rb + sl];
+                if (piece_group[r] == gi) return "two pieces of a row in one group";
+                if (final_group[r] >= 0) return "piece after the row's last piece";
+                piece_group[r] = gi;
+                const int64_t len = s.ptr[r + 1] - s.ptr[r];
+                if (p.pc_len[q] < 0 || p.pc_len[q] > W || (p.pc_len[q] == 0 && len != 0)) return "piece length";
+                for (int32_t k = 0; k < W; k++) {
+                    const int32_t src = p.pc_src[q * W + k];
+                    if (k >= p.pc_len[q]) {
+                        if (p.pc_val_pos[q * W + k] != -1) return "padding value";
+                        if (src != p.max_tile_rows + p.max_tile_imports) return "padding source";
+                        continue;
+                    }
+                    if (src < 0 || src >= p.max_tile_rows + p.max_tile_imports) return "LDS index range";
+                    const int64_t e = s.ptr[r] + done_k[r] + k;
+                    if (e >= s.ptr[r + 1]) return "piece past the row's end";
+                    const int64_t c = s.col[e];
+                    if (p.pc_val_pos[q * W + k] != s.val_pos[e]) return "entry value position";
+                    if (tile_of[c] == t) {
+                        if (src != slot_of[c]) return "intra-tile source";
+                        if (final_group[c] < 0 || final_group[c] >= gi) return "intra-tile dependency not final";
+                    } else {
+                        if (tile_of[c] >= t) return "dependency on a later tile";
+                        const int32_t imp = src - int32_t(p.max_tile_rows);
+                        if (imp < 0 || p.tile_import_begin[t] + imp >= p.tile_import_begin[t + 1]) return "import range";
+                        if (x_of[c] < 0 || p.imports[p.tile_import_begin[t] + imp] != x_of[c]) return "missing export";
+                        if (imp >= p.group_need[gi]) return "import not counted before its first use";
+                    }
+                }
+                done_k[r] += p.pc_len[q];
+                const bool last = done_k[r] == len;
+                if (p.pc_diag_pos[q] != (last ? s.diag_pos[r] : -1)) return "diag pos";
+                if (last) {
+                    final_group[r] = gi;
+                    x_of[r] = p.pc_export[q];
+                    if (x_of[r] >= 0) {
+                        if (x_of[r] != p.tile_export_begin[t] + xcount) return "export slots out of finish order";
+                        if (p.xlist[x_of[r]] != slot_of[r]) return "export list slot";
+                        xcount++;
+                    }
+                } else if (p.pc_export[q] != -1) {
+                    return "export before the last piece";
                 }
             }
         }
     }
+    for (int64_t r = 0; r < n; r++)
+        if (final_group[r] < 0) return "row not finished";
+    for (int64_t t = 0; t < p.ntiles; t++)
+        if (p.tile_export_begin[t + 1] - p.tile_export_begin[t] > p.max_tile_exports) return "tile exports";
     return "";
 }
 
 template <typename FV, typename VT>
 void emulate_sweep(const SweepPlan& p, const FV* f, const VT* v, VT* y)
 {
-    std::vector<VT> lds(p.max_tile_rows), xbuf(p.nexport);
+    const int W = p.width;
+    std::vector<VT> lds(p.max_tile_rows + p.max_tile_imports + 1, VT(0)), xbuf(p.nexport);
     for (int64_t t = 0; t < p.ntiles; t++) {
-        int32_t rb = p.tile_row_begin[t], re = p.tile_row_begin[t + 1];
-        for (int32_t q = rb; q < re; q++) lds[q - rb] = v[p.rows[q]];
-        for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
-            int64_t b = p.group_row_begin[gi], m = p.group_row_begin[gi + 1] - b;
-            for (int64_t q = 0; q < m; q++) {
-                VT rr = lds[p.gr_slot[b + q]];
-                for (int64_t k = 0; k < p.gr_len[b + q]; k++) {
-                    int64_t d = p.group_ent_begin[gi] + k * m + q;
-                    int32_t src = p.ent_src[d];
-                    VT yv = src >= 0 ? lds[src] : xbuf[p.imports[p.tile_import_begin[t] - src - 1]];
-                    rr -= VT(f[p.ent_val_pos[d]]) * yv;
+        const int32_t rb = p.tile_row_begin[t], re = p.tile_row_begin[t + 1];
+        for (int32_t q = rb; q < re; q++) lds[p.row_slot[q]] = v[p.rows[q]];
+        for (int32_t i = p.tile_import_begin[t]; i < p.tile_import_begin[t + 1]; i++)
+            lds[p.max_tile_rows + i - p.tile_import_begin[t]] = xbuf[p.imports[i]];
+        for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++)
+            for (int64_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++) {
+                VT rr = lds[p.pc_slot[q]];
+                for (int32_t k = 0; k < W; k++) {   // padding: 0 * (zero slot)
+                    const int64_t vp = p.pc_val_pos[q * W + k];
+                    rr -= VT(vp >= 0 ? f[vp] : FV(0)) * lds[p.pc_src[q * W + k]];
                 }
-                VT out = p.gr_diag_pos[b + q] >= 0 ? VT(rr * VT(f[p.gr_diag_pos[b + q]])) : rr;
-                lds[p.gr_slot[b + q]] = out;
-                if (p.gr_export[b + q] >= 0) xbuf[p.gr_export[b + q]] = out;
+                if (p.pc_diag_pos[q] >= 0) rr = VT(rr * VT(f[p.pc_diag_pos[q]]));
+                lds[p.pc_slot[q]] = rr;
+                if (p.pc_export[q] >= 0) xbuf[p.pc_export[q]] = rr;
             }
-        }
-        for (int32_t q = rb; q < re; q++) y[p.rows[q]] = lds[q - rb];
+        for (int32_t q = rb; q < re; q++) y[p.rows[q]] = lds[p.row_slot[q]];
     }
 }
 

@@ -35,42 +35,61 @@ struct TileShape {
     int64_t rows_per_block = 0;           // row-block tiles when !grid
 };
 
-// Compiled sweep: tiles in ticket (topological) order; inside a tile, level
-// groups in order; inside a group, rows processed concurrently.
+// Compiled sweep: tiles in ticket (topological) order; inside a tile, groups
+// of at most 64 pieces in order; the pieces of a group are independent and run
+// one per lane of a wavefront.
+//
+// A piece is a run of at most `width` consecutive entries of one row.  A row
+// with more entries has several pieces in successive groups, each continuing
+// the running value the previous one left in the row's LDS slot; the last
+// piece applies the diagonal and exports the result.  This keeps the per-row
+// operation order while every piece has the same fixed-size record.
 struct SweepPlan {
     int64_t n = 0;
     bool lower = true, unit = false;
-    int block = 64;                       // workgroup size
-    int64_t ntiles = 0, ngroups = 0, nent = 0, nexport = 0;
+    int width = 4;                        // entries per piece
+    int64_t ntiles = 0, ngroups = 0, npieces = 0, nexport = 0;
     int64_t max_tile_rows = 0, max_steps = 0, levels = 0;
     TileShape shape;
     std::vector<int32_t> tile_row_begin;  // [ntiles+1] into rows
-    std::vector<int32_t> rows;            // [n] global row of each (tile, slot)
+    std::vector<int32_t> rows;            // [n] rows of each tile, in natural order
+    std::vector<int32_t> row_slot;        // [n] LDS slot of each entry of rows
     std::vector<int32_t> tile_group_begin;// [ntiles+1] into groups
     std::vector<int32_t> tile_export_begin;// [ntiles+1] export slots owned by each tile
-    std::vector<int64_t> group_row_begin; // [ngroups+1] into the per-group-row arrays
-    std::vector<int64_t> group_ent_begin; // [ngroups+1] into the entry arrays
-    std::vector<int32_t> group_width;     // [ngroups] max entries of a row in the group
-    std::vector<int32_t> gr_slot;         // [n] LDS slot of the row in its tile
-    std::vector<int32_t> gr_len;          // [n] number of entries
-    std::vector<int32_t> gr_export;       // [n] export slot or -1
-    std::vector<int64_t> gr_diag_pos;     // [n] factor storage index of the diagonal or -1
-    std::vector<int32_t> ent_src;         // [nent] LDS slot (>= 0) or -(tile-local import + 1)
+    std::vector<int32_t> xlist;           // [nexport] LDS slot of the row behind each export slot
+    std::vector<int32_t> group_xbefore;   // [ngroups] exports of the tile finished before the group
+    int64_t max_tile_exports = 0;
+    std::vector<int32_t> group_piece_begin;// [ngroups+1] into the piece arrays
+    std::vector<int32_t> group_need;      // [ngroups] imports of the tile needed before the group runs
+    std::vector<int32_t> pc_slot;         // [npieces] LDS slot of the piece's row in its tile
+    std::vector<int32_t> pc_len;          // [npieces] entries of the piece (<= width)
+    std::vector<int32_t> pc_export;       // [npieces] export slot (last piece of an exported row) or -1
+    std::vector<int64_t> pc_diag_pos;     // [npieces] factor index of the diagonal; -1: none
+    std::vector<int32_t> pc_src;          // [npieces*width] LDS index of each entry's value:
+                                          //   slot (< max_tile_rows) or max_tile_rows + tile-local import
+    std::vector<int64_t> pc_val_pos;      // [npieces*width] factor index; -1: padding
     std::vector<int32_t> tile_import_begin;  // [ntiles+1] into imports
     std::vector<int32_t> imports;         // exchange slot of each import, per tile in first-use order
-    std::vector<int32_t> group_need;      // [ngroups] imports of the tile needed before the group runs
     int64_t max_tile_imports = 0, nimports = 0;
-    std::vector<int64_t> ent_val_pos;     // [nent] factor storage index (-1: padding)
 };
+
+// Entries per piece for rows of at most max_len entries.
+int sweep_piece_width(int64_t max_len);
 
 // Natural-order level of every row (0-based) of a lower (or upper) DAG.
 std::vector<int32_t> natural_levels(const SweepRows& s, int64_t* nlevels);
+
+// Default cost-model constants (microseconds): latency of a value crossing
+// from one tile to the next, and of one wavefront pass over a group.
+constexpr double kSweepHopUs = 3.0;
+constexpr double kSweepStepUs = 0.2;
 
 // Chooses a tile shape for the sweep (cost model: pipeline hops * hop_us +
 // in-tile levels * step_us) subject to max_tile_rows.
 TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t max_tile_rows, double hop_us, double step_us);
 
-SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t max_tile_rows);
+SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t max_tile_rows,
+                           int width = 0);
 
 // Checks every invariant the sweep kernel relies on (bounds, ordering of
 // dependencies before their use, tile order acyclic).  Returns "" when valid.

@@ -205,3 +205,34 @@ def test_sweep_schedule_general_pattern():
         out, st = g.analysis_pc_solve(g.PC_ILU0, A.starts, A.indices, fac, rhs, max_tile_rows=cap)
         assert st["grid_nx"] == 0
         np.testing.assert_array_equal(out, o.ilu0_solve(L, U, rhs))
+
+
+@pytest.mark.parametrize("width", [3, 4, 8])
+def test_sweep_schedule_split_rows(width, monkeypatch):
+    # rows longer than the piece width run as several pieces that carry the
+    # running value through LDS; the operation order, hence the result, is
+    # unchanged
+    monkeypatch.setenv("M4G_SWEEP_WIDTH", str(width))
+    S, I, V = g.convdiff27(9, 8, 7)
+    A = o.CSR(S, I, V)
+    _, L, U = o.ilu0_factor(A)
+    fac = _ilu_storage(A, L, U)
+    rhs = g.rng_uniform(5, 1, A.n)
+    for cap in (0, 100):
+        out, _ = g.analysis_pc_solve(g.PC_ILU0, S, I, fac, rhs, max_tile_rows=cap)
+        np.testing.assert_array_equal(out, o.ilu0_solve(L, U, rhs))
+        outa, _ = g.analysis_pc_solve(g.PC_ILU0, S, I, fac, rhs, adjoint=True, max_tile_rows=cap)
+        np.testing.assert_array_equal(outa, o.ilu0_adjoint_solve(L, U, rhs))
+
+
+def test_sweep_schedule_long_rows():
+    # rows of more than 16 entries (the widest piece) on a general pattern
+    from test_oracle import _rand_sparse
+    A, _ = _rand_sparse(200, seed=4, extra=24)
+    _, L, U = o.ilu0_factor(A)
+    assert max(np.diff(L.starts)) > 16
+    fac = _ilu_storage(A, L, U)
+    rhs = g.rng_uniform(2, 2, A.n)
+    for cap in (0, 64):
+        out, _ = g.analysis_pc_solve(g.PC_ILU0, A.starts, A.indices, fac, rhs, max_tile_rows=cap)
+        np.testing.assert_array_equal(out, o.ilu0_solve(L, U, rhs))
