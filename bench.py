@@ -35,7 +35,7 @@ class HipTimer:
     """hipEvent pairs recorded on the library's stream (where the kernels run)."""
 
     def __init__(self, stream):
-        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip = ctypes.CDLL("libamdhip64.so.7")   # the runtime already loaded (by soname)
         self.stream = ctypes.c_void_p(stream)
         self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
         self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]

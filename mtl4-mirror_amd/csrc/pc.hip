@@ -163,8 +163,9 @@ constexpr int kImportDepth = 4;       // imports each importer lane polls per wi
 constexpr int kFillUnroll = 26;       // row values each thread moves at once between HBM and LDS
 constexpr int kFillLists = 8;         // import and export list entries per thread in the first fill pass
 // Trace words per tile: start, compute end, compute time, polls waiting for
-// records, polls waiting for imports, fill end, first group start, drain end.
-constexpr int kTraceWords = 8;
+// records, polls waiting for imports, fill end, first group start, drain end,
+// first export stored, first import published.
+constexpr int kTraceWords = 10;
 
 // Byte offsets of the dynamic LDS regions of a sweep workgroup: header (the
 // workgroup's counters), values (rows, imports, one zero), group table, export
@@ -417,6 +418,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
             }
             wide = first_missing > head;
             if (wide) {
+                if (trace && head == 0 && lane == 0) trace[9] = int64_t(wall_clock64());
                 head = first_missing;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 if (lane == 0) __hip_atomic_store(&progress, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -521,6 +523,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
                 continue;
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (trace && e == 0 && lane == 0) trace[8] = int64_t(wall_clock64());
             for (int i = e + lane; i < avail; i += kWave)
                 __hip_atomic_store(xout + i, Sentinel<VT>::canon(lds[xl[i]]), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);

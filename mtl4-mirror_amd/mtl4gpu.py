@@ -21,6 +21,16 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmtl4gpu.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libmtl4gpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
 
+# One HIP runtime per process.  PyTorch for ROCm ships its own copy of the HIP
+# runtime (same soname, libamdhip64.so.7), and two copies in one process do not
+# share the GPU.  Importing torch first makes the dynamic linker resolve this
+# library's libamdhip64.so.7 to torch's copy, so tensors and this library's
+# kernels run on one runtime.  Without torch the system runtime is used.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _lib = ctypes.CDLL(LIB_PATH)
 
 # ---- constants (include/mtl4gpu.h) ------------------------------------------
