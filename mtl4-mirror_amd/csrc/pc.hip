@@ -152,7 +152,8 @@ struct SweepArgs {
     int* fault;                       // set when a bounded wait runs out
     int64_t other_nexport;            // size of the other direction's exchange buffer
     int64_t spin_limit;
-    int64_t* trace;                   // optional: [tile][kTraceWords] wall-clock stamps (100 MHz)
+    int64_t* trace;                   // optional: [tile][trace_words] wall-clock stamps (100 MHz)
+    int32_t trace_words;
 };
 
 // Workgroup roles: wavefront 0 imports, 1 computes, 2 and 3 load records,
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
     }
     __syncthreads();
     const int t = tile_sh;
-    int64_t* trace = a.trace ? a.trace + int64_t(t) * kTraceWords : nullptr;
+    int64_t* trace = a.trace ? a.trace + int64_t(t) * a.trace_words : nullptr;
     if (trace && tid == 0) trace[0] = int64_t(wall_clock64());
     // the other direction's exchange buffer is free: reset it for its next use
     for (int64_t x = int64_t(t) * kSweepThreads + tid; x < a.other_nexport; x += a.ntiles * int64_t(kSweepThreads))
@@ -391,6 +392,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         int head = 0;
         int64_t polls = 0;
         bool wide = true;
+#ifdef M4G_SWEEP_GROUP_TRACE
+        int igroup = 0;
+#endif
         while (head < ni) {
             int32_t xs[kImportDepth];
             VT val[kImportDepth];
@@ -419,6 +423,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
             wide = first_missing > head;
             if (wide) {
                 if (trace && head == 0 && lane == 0) trace[9] = int64_t(wall_clock64());
+#ifdef M4G_SWEEP_GROUP_TRACE
+                // diagnostic build only: when the imports each group needs were published
+                if (trace && lane == 0) {
+                    const int64_t now = int64_t(wall_clock64());
+                    while (igroup < ng && tab[igroup].y <= first_missing) {
+                        trace[kTraceWords + 2 * a.max_steps + igroup] = now;
+                        igroup++;
+                    }
+                }
+#endif
                 head = first_missing;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 if (lane == 0) __hip_atomic_store(&progress, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -480,6 +494,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
                     }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
+#ifdef M4G_SWEEP_GROUP_TRACE
+                // diagnostic build only: start time of every group
+                if (trace && lane == 0) trace[kTraceWords + b * B + j] = int64_t(wall_clock64());
+#endif
                 VT yv[W];
 #pragma unroll
                 for (int k = 0; k < W; k++) yv[k] = *lds_at<VT>(cur.src[k]);
@@ -515,6 +533,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         // (in finish order, so exchange slots xb, xb+1, ...) for later tiles.
         VT* xout = xbuf + xb;
         int e = 0;
+#ifdef M4G_SWEEP_GROUP_TRACE
+        int xgroup = 0;
+#endif
         while (e < nx) {
             const int fin = __hip_atomic_load(&finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const int avail = tab[fin].z;
@@ -528,6 +549,14 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
                 __hip_atomic_store(xout + i, Sentinel<VT>::canon(lds[xl[i]]), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             e = avail;
+#ifdef M4G_SWEEP_GROUP_TRACE
+            // diagnostic build only: when each group's exports were issued
+            if (trace && lane == 0) {
+                const int64_t now = int64_t(wall_clock64());
+                for (int gg = xgroup; gg < fin; gg++) trace[kTraceWords + a.max_steps + gg] = now;
+            }
+            xgroup = fin;
+#endif
         }
     } else {
         // ---- loaders: wavefront 2 fills ring half 0, wavefront 3 half 1 ----
@@ -829,11 +858,17 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
     a.other_nexport = other.plan_meta.nexport;
     a.spin_limit = spin_limit();
     a.trace = nullptr;
+    a.trace_words = kTraceWords;
     static const char* trace_path = std::getenv("M4G_SWEEP_TRACE");
     static int traced = 0;
     DevBuf<int64_t> trace_buf;
     if (trace_path && traced < 4) {
-        trace_buf.reset(size_t(p.ntiles) * kTraceWords);
+#ifdef M4G_SWEEP_GROUP_TRACE
+        a.trace_words = int32_t(kTraceWords + 3 * p.max_steps);
+#else
+        a.trace_words = kTraceWords;
+#endif
+        trace_buf.reset(size_t(p.ntiles) * a.trace_words);
         M4G_HIP(hipMemsetAsync(trace_buf.ptr, 0, trace_buf.count * sizeof(int64_t), s));
         a.trace = trace_buf.ptr;
     }
@@ -857,7 +892,7 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
         trace_buf.download(h.data(), h.size());
         std::string path = std::string(trace_path) + "." + std::to_string(traced++);
         if (FILE* f = std::fopen(path.c_str(), "wb")) {
-            int64_t hdr[3] = {p.ntiles, p.max_steps, kTraceWords};
+            int64_t hdr[3] = {p.ntiles, p.max_steps, a.trace_words};
             std::fwrite(hdr, sizeof(hdr), 1, f);
             std::fwrite(h.data(), sizeof(int64_t), h.size(), f);
             std::fclose(f);

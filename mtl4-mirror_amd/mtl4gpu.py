@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libmtl4gpu.so")
+# M4G_LIBRARY selects another build of the library (e.g. a diagnostic one)
+LIB_PATH = os.environ.get("M4G_LIBRARY") or os.path.join(_HERE, "lib", "libmtl4gpu.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libmtl4gpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
