@@ -69,6 +69,60 @@ def time_launches(timer, launch, reps, flush=None):
     return np.array([timer.elapsed_ms(a, b) for a, b in evs])
 
 
+def mixed_precision(g, N, steps=2):
+    """27-point convection-diffusion on an N^3 grid: plain FP64 ILU(0)-BiCGStab
+    to 1e-10 vs FP32 ILU(0)-BiCGStab to 1e-4 inside FP64 iterative refinement
+    to 1e-10.  Returns times, iteration counts, the refined solution's error
+    against the FP64 solution and the speedup."""
+    S, I, V = g.convdiff27(N)
+    n = len(S) - 1
+    A = g.Matrix(S, I, V)
+    xt = g.Vector(data=g.rng_uniform(3, 1, n))
+    b = g.Vector(n)
+    g.spmv(A, xt, b)
+    nb = g.two_norm(b)
+    P64 = g.PC(A, g.PC_ILU0)
+    P32 = g.PC(A, g.PC_ILU0, factor_dtype=np.float32)
+    x64, x32 = g.Vector(n), g.Vector(n)
+
+    def plain():
+        x64.fill(0.0)
+        it, _ = g.make_iteration(20000, 1e-10, 0.0, nb, history=False)
+        assert g.bicgstab(A, x64, b, P64, it) == 0
+        return it.iterations
+
+    def refined():
+        x32.fill(0.0)
+        it, _ = g.make_iteration(50, 1e-10, 0.0, nb, history=False)
+        st, info = g.refined_solve(g.METHOD_BICGSTAB, A, x32, b, P32, it, 1e-4)
+        assert st == 0
+        return info
+
+    def timed(f):
+        f()
+        g.synchronize()
+        ts, res = [], None
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            res = f()
+            g.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.mean(ts)), res
+
+    t64, it64 = timed(plain)
+    t32, info = timed(refined)
+    r = g.Vector(n)
+    g.residual(A, x32, b, r)
+    relres = g.two_norm(r) / nb
+    a64, a32 = x64.numpy(), x32.numpy()
+    err = float(np.max(np.abs(a32 - a64)) / np.max(np.abs(a64)))
+    return {"workload": f"convdiff27_{N}", "n": n, "nnz": int(S[-1]),
+            "fp64_ms": round(t64, 3), "fp64_iterations": it64,
+            "mixed_ms": round(t32, 3), "outer_iterations": info.outer_iterations,
+            "inner_iterations": info.inner_iterations, "true_relres": relres,
+            "err_vs_fp64_solution": err, "speedup": round(t64 / t32, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +132,8 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--cpu-iters", type=int, default=12, help="PCG iterations of the bounded CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
+    ap.add_argument("--mixed-grid", type=int, default=96)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the sweeps' exact natural-order recurrence does not shard "
@@ -137,8 +193,11 @@ def main():
     y = g.Vector(n)
     spmv_bytes = 8 * nnz + 4 * nnz + 4 * (n + 1) + 2 * 8 * n        # s_v*nnz + 4*nnz + 4(n+1) + 2*s_v*n
     warm = time_launches(timer, lambda: g.spmv(A, xt, y), 100)
-    flushbuf = g.Vector(64 * 1024 * 1024)                           # 512 MB > 256 MB Infinity Cache
-    flushed = time_launches(timer, lambda: g.spmv(A, xt, y), 30, flush=lambda: flushbuf.fill(1.0))
+    # evict the 256 MB Infinity Cache between launches by READING 512 MB (a
+    # norm), so no dirty lines of the flush are written back during the SpMV
+    flushbuf = g.Vector(64 * 1024 * 1024)
+    flushbuf.fill(1.0)
+    flushed = time_launches(timer, lambda: g.spmv(A, xt, y), 30, flush=lambda: g.two_norm(flushbuf))
     spmv_warm_gbs = spmv_bytes / (np.median(warm) * 1e-3) / 1e9
     spmv_flushed_gbs = spmv_bytes / (np.median(flushed) * 1e-3) / 1e9
 
@@ -178,6 +237,8 @@ def main():
                          f"{iters[-1]} iterations (solve only, factorisation excluded as on the GPU side)",
                "factor_ms": round(cpu_factor_ms, 1), "ms_per_iteration": round(per_iter_ms, 2)}
 
+    mixed = None if args.no_mixed else mixed_precision(g, args.mixed_grid)
+
     out = {
         "metric": METRIC,
         "value": round(ms_per_step, 3),
@@ -210,7 +271,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},
         "cpu_baseline": cpu,
-        "mixed_vs_fp64_speedup": None,
+        "mixed_vs_fp64_speedup": mixed["speedup"] if mixed else None,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 
