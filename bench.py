@@ -69,6 +69,83 @@ def time_launches(timer, launch, reps, flush=None):
     return np.array([timer.elapsed_ms(a, b) for a, b in evs])
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel of known
+    size); None when there is no such measurement."""
+    path = os.path.join(ROOT, "profiles", "k_sweep_k_sell_spmv_poisson3d_128_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    name = "k_sweep" if kernel.startswith("k_sweep") else "k_sell_spmv"
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == name:
+            return int(parts[2])
+    return None
+
+
+def batched(g, nsys=512, nrhs=8, sample=16):
+    """512 SPD 2D 5-point variable-coefficient systems (side ~ U{32..256},
+    kappa ~ lognormal(0,1) per cell), 8 right-hand sides ~ N(0,1) each, IC(0)-PCG
+    to 1e-8: one factorisation launch + one persistent solve launch (autotuned
+    grouping and block size) vs a loop of single solves (factorisation + 8 CG
+    solves per system) timed on `sample` systems spread over the sizes and
+    scaled to all systems."""
+    sides = g.rng_int(7, 1, nsys, 32, 256)
+    systems = [g.varcoef5(int(sd), int(sd), seed=1000 + k) for k, sd in enumerate(sides)]
+    sizes = np.array([len(S) - 1 for S, _, _ in systems])
+    n_total, nnz_total = int(sizes.sum()), int(sum(int(S[-1]) for S, _, _ in systems))
+    bs = [[g.rng_normal(2000 + k, j, int(sizes[k])) for j in range(nrhs)] for k in range(nsys)]
+    bh = np.concatenate([np.stack(v, axis=1).reshape(-1) for v in bs])
+    B = g.Batch(systems)
+    b = g.Vector(data=bh)
+    x = g.Vector(len(bh))
+    g.synchronize()
+    t0 = time.perf_counter()
+    B.factor_ic0()
+    factor_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    rpi, block, _ = B.autotune(nrhs, b)
+    tune_ms = (time.perf_counter() - t0) * 1e3
+    x.fill(0.0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    its, res, err = B.cg_ic0(nrhs, x, b, 10000, 1e-8, rhs_per_item=rpi, block=block)
+    solve_ms = (time.perf_counter() - t0) * 1e3
+    nb = np.array([[np.linalg.norm(bs[k][j]) for j in range(nrhs)] for k in range(nsys)])
+    relres = float(np.max(res / nb))
+    # loop of single solves on a sample spread over the sizes
+    order = np.argsort(sizes, kind="stable")
+    pick = [int(order[(2 * k + 1) * nsys // (2 * sample)]) for k in range(sample)]
+    loop_ms, loop_its = 0.0, []
+    for k in pick:
+        S, I, V = systems[k]
+        g.synchronize()
+        t0 = time.perf_counter()
+        A = g.Matrix(S, I, V)
+        P = g.PC(A, g.PC_IC0)
+        xs, bv = g.Vector(int(sizes[k])), g.Vector(int(sizes[k]))
+        for j in range(nrhs):
+            bv.upload(bs[k][j])
+            xs.fill(0.0)
+            it, _ = g.make_iteration(10000, 1e-8, 0.0, float(nb[k, j]), history=False)
+            assert g.cg(A, xs, bv, P, it) == 0
+            loop_its.append(it.iterations)
+        g.synchronize()
+        loop_ms += (time.perf_counter() - t0) * 1e3
+    loop_est_ms = loop_ms * nsys / sample
+    batched_ms = factor_ms + solve_ms
+    return {"workload": f"batched_varcoef5_{nsys}x{nrhs}", "nsys": nsys, "nrhs": nrhs, "n_total": n_total,
+            "nnz_total": nnz_total, "side_range": [int(sides.min()), int(sides.max())],
+            "factor_ms": round(factor_ms, 2), "solve_ms": round(solve_ms, 2), "autotune_ms": round(tune_ms, 1),
+            "block": block, "rhs_per_item_by_class": sorted(set(int(r) for r in rpi)),
+            "all_converged": bool((err == 0).all()), "max_relres": relres,
+            "iterations_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
+            "loop_sample_systems": sample, "loop_sample_ms": round(loop_ms, 1),
+            "loop_estimate_ms": round(loop_est_ms, 1),
+            "speedup_vs_loop": round(loop_est_ms / batched_ms, 2)}
+
+
 def mixed_precision(g, N, steps=2):
     """27-point convection-diffusion on an N^3 grid: plain FP64 ILU(0)-BiCGStab
     to 1e-10 vs FP32 ILU(0)-BiCGStab to 1e-4 inside FP64 iterative refinement
@@ -134,6 +211,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--mixed-grid", type=int, default=96)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-system workload")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling run: a calibration axpy of known size, 5 SpMVs and 5 IC(0) applications")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the sweeps' exact natural-order recurrence does not shard "
@@ -153,6 +233,25 @@ def main():
     g.spmv(A, xt, b)                          # b = A x_true on the GPU
     g.synchronize()
     setup_s = time.perf_counter() - t0
+
+    if args.kernels_only:
+        # axpy y += a x over 2^25 doubles: 512 MiB read, 256 MiB written -- the
+        # yardstick for the HBM byte counters
+        cal_x, cal_y = g.Vector(1 << 25), g.Vector(1 << 25)
+        cal_x.fill(1.0)
+        cal_y.fill(0.0)
+        g.synchronize()
+        for _ in range(3):
+            g.axpy(0.5, cal_x, cal_y)
+        P = g.PC(A, g.PC_IC0)
+        y = g.Vector(n)
+        for _ in range(5):
+            g.spmv(A, xt, y)
+        for _ in range(5):
+            P.apply_async(xt, y)
+        g.synchronize()
+        print(json.dumps({"kernels_only": True, "n": n, "nnz": nnz, "calibration_axpy_n": 1 << 25}), flush=True)
+        return
 
     # ---- IC(0) factorisation (once) ----
     t0 = time.perf_counter()
@@ -238,6 +337,7 @@ def main():
                "factor_ms": round(cpu_factor_ms, 1), "ms_per_iteration": round(per_iter_ms, 2)}
 
     mixed = None if args.no_mixed else mixed_precision(g, args.mixed_grid)
+    batch = None if args.no_batch else batched(g)
 
     out = {
         "metric": METRIC,
@@ -269,10 +369,11 @@ def main():
         "per_iteration_ms": round(iter_ms, 4),
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic(dom["kernel"])},
         "cpu_baseline": cpu,
         "mixed_vs_fp64_speedup": mixed["speedup"] if mixed else None,
         "mixed": mixed,
+        "batched": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -209,6 +209,42 @@ typedef struct {
 int m4g_refined_solve(int method, m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* outer,
                       double inner_rtol, int inner_max_iter, m4g_refine_info* info);
 
+/* ---- batched IC(0)-PCG ----------------------------------------------------
+ * Many independent SPD systems with nrhs right-hand sides each: every
+ * (system, rhs) pair runs itl::cg with itl::pc::ic_0 (cg.hpp:75-112,
+ * ic_0.hpp:135-186) and basic_iteration(b, max_iter, rtol, atol)
+ * (basic_iteration.hpp:24-92).  No single reference call does this; the
+ * reference's equivalent is a loop over cg() calls, one per system and rhs.
+ * Systems are given as concatenated CSR: sizes[s] rows each, starts = each
+ * system's n_s + 1 row pointers one after another, indices (local columns,
+ * sorted per row) and values concatenated. */
+typedef struct m4g_batch_s* m4g_batch;
+int m4g_batch_create(m4g_batch* B, int64_t nsys, const int64_t* sizes, const int64_t* starts,
+                     const int64_t* indices, const double* values);
+int m4g_batch_destroy(m4g_batch B);
+int m4g_batch_dims(m4g_batch B, int64_t* nsys, int64_t* n_total, int64_t* nnz_total);
+/* IC(0) factors of every system in one launch (the reference's per-row order) */
+int m4g_batch_factor_ic0(m4g_batch B);
+/* factor storage: each system's upper(A) rows, inverted diagonal first (ic_0.hpp:148-185) */
+int m4g_batch_download_factor(m4g_batch B, double* values);
+/* One persistent launch solving every (system, rhs).  x (in/out) and b hold
+ * n_total * nrhs doubles; entry (system s, row i, rhs j) is at
+ * (row0(s) + i) * nrhs + j with row0 the prefix sum of sizes.  rhs_per_item[s]
+ * (or NULL: 8) right-hand sides of system s share one workgroup; block is the
+ * workgroup size (256, 512, 1024; 0 = 512).  Per (s, j) at s * nrhs + j:
+ * iterations, final residual norm, error (0 converged, 1 too many iterations). */
+int m4g_batch_cg_ic0(m4g_batch B, int nrhs, m4g_vector x, m4g_vector b, int max_iter, double rtol,
+                     double atol, const int* rhs_per_item, int block, int* iterations, double* resid, int* error);
+/* Autotuner: times trial_iters iterations (0: 20) of the whole batch, first
+ * for every block size (256, 512, 1024) with one rhs grouping (8, 4, 2, 1
+ * dividing nrhs) for all systems, then, with the fastest block size, lets each
+ * third of the systems by size try the other groupings.  Returns the block
+ * size and each system's grouping.  trial_ms (optional, up to 24 entries)
+ * receives the trial times: 3 * groupings of the first stage, then 3 *
+ * groupings of the second. */
+int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int* rhs_per_item, int* block,
+                       double* trial_ms);
+
 #ifdef __cplusplus
 }
 #endif

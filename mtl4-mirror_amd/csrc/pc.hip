@@ -824,10 +824,13 @@ static void launch_sweep_w(SweepDev& d, const SweepArgs& a, const VT* v, VT* y, 
     const size_t lds = sweep_lds_bytes(p, sizeof(VT), ftype);
     const bool layout_ok = rec_chunks(ftype, W) == kRecChunks<FV, W>;
     M4G_REQUIRE(layout_ok, M4G_ERR_LOGIC, "record layout mismatch");
-    // static shared words (ticket, progress) come on top of the dynamic part
-    if (lds + 64 > 65536)
+    // beyond 64 KB of dynamic LDS must be requested, once per kernel and size
+    static size_t granted = 0;
+    if (lds > 65536 && lds > granted) {
         M4G_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(lds)));
+        granted = lds;
+    }
     hipLaunchKernelGGL(kern, dim3(p.ntiles), dim3(kSweepThreads), lds, s, a, d.recs.ptr, v, y, (VT*)d.xbuf.ptr, xo,
                        skip);
 }

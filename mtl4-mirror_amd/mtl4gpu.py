@@ -119,6 +119,14 @@ _SIGS = {
     "m4g_pattern_gradient": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "m4g_refined_solve": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), ctypes.c_double,
                                          ctypes.c_int, ctypes.POINTER(RefineInfo)]),
+    "m4g_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _vp, _vp, _vp, _vp]),
+    "m4g_batch_destroy": (ctypes.c_int, [_vp]),
+    "m4g_batch_dims": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i64p]),
+    "m4g_batch_factor_ic0": (ctypes.c_int, [_vp]),
+    "m4g_batch_download_factor": (ctypes.c_int, [_vp, _vp]),
+    "m4g_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "m4g_batch_cg_ic0": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        _vp, ctypes.c_int, _vp, _vp, _vp]),
     # host-only analysis (mtl4gpu_analysis.h)
     "m4g_analysis_grid": (ctypes.c_int, [_c_i64, _vp, _vp, _c_i64p, _c_i64p, _c_i64p]),
     "m4g_analysis_pc_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_i64, _vp, _vp, _vp, ctypes.c_int,
@@ -439,6 +447,60 @@ def refined_solve(method, A, x, b, P, it, inner_rtol, inner_max_iter=10000):
                                  float(inner_rtol), int(inner_max_iter), ctypes.byref(info))
     _check(st, "m4g_refined_solve", allow=(OK, MAX_ITER))
     return st, info
+
+
+class Batch:
+    """Many independent SPD systems (concatenated CSR) for the batched IC(0)-PCG."""
+
+    def __init__(self, systems):
+        """systems: list of (starts, indices, values) per system (local columns)."""
+        sizes = np.array([len(S) - 1 for S, _, _ in systems], np.int64)
+        starts = np.concatenate([np.asarray(S, np.int64) for S, _, _ in systems])
+        indices = np.concatenate([np.asarray(I, np.int64) for _, I, _ in systems])
+        values = np.concatenate([np.asarray(V, np.float64) for _, _, V in systems])
+        self.sizes = sizes
+        self.row0 = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.n_total = int(sizes.sum())
+        h = ctypes.c_void_p()
+        _check(_f["m4g_batch_create"](ctypes.byref(h), len(sizes), _ptr(sizes), _ptr(starts), _ptr(indices),
+                                      _ptr(values)), "m4g_batch_create")
+        self.h = h
+        self._keep = (sizes, starts, indices, values)
+        self.u_total = sum(int(((np.asarray(I) >= np.repeat(np.arange(len(S) - 1), np.diff(S))).sum()))
+                           for S, I, _ in systems)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _f.get("m4g_batch_destroy"):
+            _f["m4g_batch_destroy"](self.h)
+
+    def factor_ic0(self):
+        _check(_f["m4g_batch_factor_ic0"](self.h), "m4g_batch_factor_ic0")
+
+    def factor_values(self):
+        out = np.empty(self.u_total, np.float64)
+        _check(_f["m4g_batch_download_factor"](self.h, _ptr(out)), "m4g_batch_download_factor")
+        return out
+
+    def autotune(self, nrhs, b, trial_iters=20):
+        """Returns (rhs_per_item per system, block size, trial times in ms)."""
+        rpi = np.zeros(len(self.sizes), np.int32)
+        blk = ctypes.c_int()
+        trials = np.full(24, np.nan)
+        _check(_f["m4g_batch_autotune"](self.h, nrhs, b.h, int(trial_iters), _ptr(rpi), ctypes.byref(blk),
+                                        _ptr(trials)), "m4g_batch_autotune")
+        return rpi, blk.value, trials
+
+    def cg_ic0(self, nrhs, x, b, max_iter, rtol, atol=0.0, rhs_per_item=None, block=0):
+        """Solves every (system, rhs); returns (iterations, resid, error) arrays [nsys, nrhs]."""
+        ns = len(self.sizes)
+        it = np.zeros(ns * nrhs, np.int32)
+        res = np.zeros(ns * nrhs, np.float64)
+        err = np.zeros(ns * nrhs, np.int32)
+        rpi = None if rhs_per_item is None else np.ascontiguousarray(rhs_per_item, np.int32)
+        _check(_f["m4g_batch_cg_ic0"](self.h, nrhs, x.h, b.h, int(max_iter), float(rtol), float(atol),
+                                      _ptr(rpi) if rpi is not None else None, int(block), _ptr(it), _ptr(res),
+                                      _ptr(err)), "m4g_batch_cg_ic0")
+        return it.reshape(ns, nrhs), res.reshape(ns, nrhs), err.reshape(ns, nrhs)
 
 
 # ---------------------------------------------------------------------------
