@@ -299,6 +299,14 @@ def main():
     flushed = time_launches(timer, lambda: g.spmv(A, xt, y), 30, flush=lambda: g.two_norm(flushbuf))
     spmv_warm_gbs = spmv_bytes / (np.median(warm) * 1e-3) / 1e9
     spmv_flushed_gbs = spmv_bytes / (np.median(flushed) * 1e-3) / 1e9
+    # bytes the SELL-64 kernel actually streams: padded values, 16-bit column
+    # offsets (every column of this matrix is within +-16384 of its row), x and
+    # w once, two 8-byte words of slice metadata per 64 rows
+    nslc = (n + 63) // 64
+    rl = np.diff(np.asarray(S, np.int64))
+    rl = np.concatenate([rl, np.zeros(nslc * 64 - n, np.int64)])
+    padded = int(rl.reshape(nslc, 64).max(axis=1).sum()) * 64
+    spmv_moved = 8 * padded + 2 * padded + 16 * n + 16 * nslc
 
     e_factor = (nnz + n) // 2                                     # entries of L (= U^T), diagonal included
     sweep_bytes = 12 * e_factor + 4 * (n + 1) + 16 * n
@@ -362,7 +370,9 @@ def main():
         "setup_s": round(setup_s, 2),
         "spmv": {"bytes": spmv_bytes, "warm_ms_median": float(np.median(warm)),
                  "flushed_ms_median": float(np.median(flushed)), "warm_gbs": round(spmv_warm_gbs, 1),
-                 "flushed_gbs": round(spmv_flushed_gbs, 1), "flushed_frac": round(spmv_flushed_gbs / HBM_PEAK_GBS, 4)},
+                 "flushed_gbs": round(spmv_flushed_gbs, 1), "flushed_frac": round(spmv_flushed_gbs / HBM_PEAK_GBS, 4),
+                 "index_format": "int16 column - row per entry (SELL-64 slices)", "moved_bytes": spmv_moved,
+                 "moved_flushed_gbs": round(spmv_moved / (np.median(flushed) * 1e-3) / 1e9, 1)},
         "sweep": {"bytes": sweep_bytes, "ms": sweep_ms, "gbs": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1),
                   "tiles": pcstats["tiles_lower"], "steps": pcstats["steps_lower"],
                   "tile": [pcstats["tile_nx"], pcstats["tile_ny"], pcstats["tile_nz"]]},

@@ -222,7 +222,8 @@ __global__ __launch_bounds__(kNT) void k_cg_p(int64_t n, const T* __restrict__ z
 template <typename T>
 __global__ __launch_bounds__(kSpmvNT) void k_cg_spmv_dot(int64_t n, int64_t nslices, int64_t chunk,
                                                           const int64_t* __restrict__ slice_ptr,
-                                                          const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                          const int64_t* __restrict__ slice_idx,
+                                                          const uint8_t* __restrict__ idx, const T* __restrict__ vals,
                                                           const T* __restrict__ p, T* __restrict__ q, KState* st,
                                                           double* partials, unsigned* counter)
 {
@@ -232,9 +233,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_cg_spmv_dot(int64_t n, int64_t nsli
     const int64_t s = g * kSliceWaves + wave;
     Sums<1> acc{{0}};
     if (s < nslices) {
-        const int64_t base = slice_ptr[s];
-        const int width = int((slice_ptr[s + 1] - base) / kWave);
-        const T qv = sell_row_sum(cols, vals, p, base, width, lane);
+        const T qv = sell_row_sum(slice_ptr, slice_idx, idx, vals, p, s, lane);
         const int64_t row = s * kWave + lane;
         if (row < n) {
             q[row] = qv;
@@ -325,7 +324,8 @@ __global__ __launch_bounds__(kNT) void k_bicg_p(int64_t n, const T* __restrict__
 template <typename T>
 __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_gamma(int64_t n, int64_t nslices, int64_t chunk,
                                                               const int64_t* __restrict__ slice_ptr,
-                                                              const int32_t* __restrict__ cols,
+                                                              const int64_t* __restrict__ slice_idx,
+                                                              const uint8_t* __restrict__ idx,
                                                               const T* __restrict__ vals, const T* __restrict__ phat,
                                                               const T* __restrict__ rtilde, T* __restrict__ v,
                                                               KState* st, double* partials, unsigned* counter)
@@ -336,9 +336,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_gamma(int64_t n, int64_t 
     const int64_t s = g * kSliceWaves + wave;
     Sums<1> acc{{0}};
     if (s < nslices) {
-        const int64_t base = slice_ptr[s];
-        const int width = int((slice_ptr[s + 1] - base) / kWave);
-        const T vv = sell_row_sum(cols, vals, phat, base, width, lane);
+        const T vv = sell_row_sum(slice_ptr, slice_idx, idx, vals, phat, s, lane);
         const int64_t row = s * kWave + lane;
         if (row < n) {
             v[row] = vv;
@@ -388,7 +386,8 @@ __global__ __launch_bounds__(kNT) void k_bicg_exit(int64_t n, const T* __restric
 template <typename T>
 __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_omega(int64_t n, int64_t nslices, int64_t chunk,
                                                               const int64_t* __restrict__ slice_ptr,
-                                                              const int32_t* __restrict__ cols,
+                                                              const int64_t* __restrict__ slice_idx,
+                                                              const uint8_t* __restrict__ idx,
                                                               const T* __restrict__ vals, const T* __restrict__ shat,
                                                               const T* __restrict__ s_, T* __restrict__ t_, KState* st,
                                                               double* partials, unsigned* counter)
@@ -399,9 +398,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_omega(int64_t n, int64_t 
     const int64_t s = g * kSliceWaves + wave;
     Sums<2> acc{{0, 0}};
     if (s < nslices) {
-        const int64_t base = slice_ptr[s];
-        const int width = int((slice_ptr[s + 1] - base) / kWave);
-        const T tv = sell_row_sum(cols, vals, shat, base, width, lane);
+        const T tv = sell_row_sum(slice_ptr, slice_idx, idx, vals, shat, s, lane);
         const int64_t row = s * kWave + lane;
         if (row < n) {
             t_[row] = tv;
@@ -602,7 +599,7 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
         }
         hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
         hipLaunchKernelGGL(k_cg_spmv_dot<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
-                           A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)p.ptr, q.ptr, S, part, cnt);
+                           A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)p.ptr, q.ptr, S, part, cnt);
         FIN(1, FIN_CG_ALPHA, sgrid);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr, x, r.ptr,
                            S, int(identity), part, cnt);
@@ -653,7 +650,7 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
                            (const KState*)S);
         apply(p.ptr, phat.ptr);                                      // phat = solve(M, p) (:55)
         hipLaunchKernelGGL(k_bicg_spmv_gamma<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
-                           A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)phat.ptr, (const T*)rtilde.ptr,
+                           A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)phat.ptr, (const T*)rtilde.ptr,
                            v.ptr, S, part, cnt);
         FIN(1, FIN_BICG_GAMMA, sgrid);
         hipLaunchKernelGGL(k_bicg_s<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)v.ptr, s_.ptr, S,
@@ -662,7 +659,7 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
         hipLaunchKernelGGL(k_bicg_exit<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)phat.ptr, x, S, cnt + 1);
         apply(s_.ptr, shat.ptr);                                     // shat = solve(M, s) (:68)
         hipLaunchKernelGGL(k_bicg_spmv_omega<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
-                           A.sell.slice_ptr.ptr, A.sell.cols.ptr, vals, (const T*)shat.ptr, (const T*)s_.ptr, t_.ptr,
+                           A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)shat.ptr, (const T*)s_.ptr, t_.ptr,
                            S, part, cnt);
         FIN(2, FIN_BICG_OMEGA, sgrid);
         hipLaunchKernelGGL(k_bicg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)shat.ptr, (const T*)phat.ptr,

@@ -137,11 +137,14 @@ namespace m4g {
 
 // Sliced ELL: rows in slices of 64 (one wavefront); within a slice, entry k of
 // all 64 rows is contiguous, so lane l reads entry k of row 64*s + l with one
-// coalesced load.  Padding entries carry column -1.
+// coalesced load.  Column storage per slice: int16 column − row ("narrow") or
+// int32 columns ("wide"), see spmv_dev.hpp.
 struct Sell {
     int64_t n = 0, nslices = 0, padded = 0;
     DevBuf<int64_t> slice_ptr;   // [nslices+1] start of each slice (entries)
-    DevBuf<int32_t> cols;        // [padded]
+    DevBuf<int64_t> slice_idx;   // [nslices] byte offset of the slice's indices << 1 | wide
+    DevBuf<uint8_t> idx;         // index bytes of all slices
+    int64_t narrow_slices = 0;
     DevBuf<double> vals;         // [padded]
     DevBuf<float> vals32;        // [padded] (lazily built for FP32 paths)
     DevBuf<int64_t> csr_pos;     // [padded] CSR index of each entry (-1 for padding)

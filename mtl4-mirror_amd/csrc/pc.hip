@@ -16,6 +16,7 @@
 #include "krylov.hpp"
 
 #include <algorithm>
+#include <future>
 #include <cmath>
 #include <cstdlib>
 
@@ -958,17 +959,23 @@ int64_t sweep_tile_cap(int vtype)
     return cap;
 }
 
-SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype)
+// LDS bytes a sweep tile may use on the current device: a tile's rows,
+// imports, group table and record ring share the workgroup's LDS
+static size_t sweep_lds_budget()
 {
-    // a tile's rows, imports, group table and record ring share the
-    // workgroup's LDS; sized for double vectors, the widest the sweeps use
     int lds = 65536, dev = 0;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
             lds = std::min(v, 160 * 1024);
     }
-    const size_t budget = size_t(lds) - 64;   // static shared words
+    return size_t(lds) - 64;   // static shared words
+}
+
+// host only (no HIP call): safe on any thread
+static SweepPlan plan_sweep_budget(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype, size_t budget)
+{
+    // sized for double vectors, the widest the sweeps use
     SweepPlan p;
     for (;;) {
         TileShape shape = choose_tiles(rows, g, cap, env_double("M4G_SWEEP_HOP_US", kSweepHopUs),
@@ -983,6 +990,26 @@ SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int 
         M4G_REQUIRE(msg.empty(), M4G_ERR_LOGIC, "sweep plan invalid: " + msg);
     }
     return p;
+}
+
+SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype)
+{
+    return plan_sweep_budget(rows, g, cap, ftype, sweep_lds_budget());
+}
+
+void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
+                     SweepPlan& pl, SweepPlan& pu)
+{
+    // the two directions are independent: plan them on two host threads
+    const size_t budget = sweep_lds_budget();
+    auto fu = std::async(std::launch::async, [&] { return plan_sweep_budget(up, g, cap, ftype, budget); });
+    try {
+        pl = plan_sweep_budget(lo, g, cap, ftype, budget);
+    } catch (...) {
+        fu.wait();
+        throw;
+    }
+    pu = fu.get();
 }
 
 }  // namespace m4g
@@ -1060,8 +1087,13 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                                    A->indices.ptr, A->values.ptr, P->inv_diag.ptr);
             M4G_HIP(hipGetLastError());
         } else if (kind == M4G_PC_IC0 || kind == M4G_PC_ILU0) {
-            GridInfo g = infer_grid(n, A->h_starts.data(), A->h_indices.data());
-            P->grid = g;
+            // grid inference runs beside the pattern work on a second host thread
+            auto fg = std::async(std::launch::async,
+                                 [&] { return infer_grid(n, A->h_starts.data(), A->h_indices.data()); });
+            struct Join {
+                std::future<GridInfo>& f;
+                ~Join() { if (f.valid()) f.wait(); }
+            } join_g{fg};
             SweepRows lo, up;
             if (kind == M4G_PC_IC0) {
                 std::vector<int64_t> a_pos;
@@ -1088,9 +1120,13 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                                            hipMemcpyDeviceToDevice, s));
                 factorize_levels(*P, lo, *A);
             }
+            const GridInfo g = fg.get();
+            P->grid = g;
             int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
-            P->lower.upload(plan_sweep(lo, g, cap, ftype), P->fstore.ptr, ftype);
-            P->upper.upload(plan_sweep(up, g, cap, ftype), P->fstore.ptr, ftype);
+            SweepPlan pl, pu;
+            plan_sweep_pair(lo, up, g, cap, ftype, pl, pu);
+            P->lower.upload(std::move(pl), P->fstore.ptr, ftype);
+            P->upper.upload(std::move(pu), P->fstore.ptr, ftype);
             P->tmp_bytes = 0;
         }
         M4G_HIP(hipStreamSynchronize(s));
@@ -1190,8 +1226,10 @@ void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipS
             M4G_REQUIRE(ilu0_adjoint_rows(n, P.fS.data(), P.fI.data(), lo, up), M4G_ERR_MISSING_DIAGONAL,
                         "missing diagonal");
             int64_t cap = sweep_tile_cap(P.factor_type);
-            P.adj_lower.upload(plan_sweep(lo, P.grid, cap, P.factor_type), P.fstore.ptr, P.factor_type);
-            P.adj_upper.upload(plan_sweep(up, P.grid, cap, P.factor_type), P.fstore.ptr, P.factor_type);
+            SweepPlan pl, pu;
+            plan_sweep_pair(lo, up, P.grid, cap, P.factor_type, pl, pu);
+            P.adj_lower.upload(std::move(pl), P.fstore.ptr, P.factor_type);
+            P.adj_upper.upload(std::move(pu), P.fstore.ptr, P.factor_type);
             P.adj_ready = true;
         }
         run_sweep_pair(P.adj_lower, P.adj_upper, x, tmp, y, vtype, skip, s);

@@ -37,6 +37,9 @@ void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp,
 int check_sweep_fault(SweepDev& d);
 int64_t sweep_tile_cap(int vtype);
 SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype);
+// both directions of a preconditioner, planned on two host threads
+void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
+                     SweepPlan& pl, SweepPlan& pu);
 
 }  // namespace m4g
 

@@ -36,18 +36,31 @@ namespace m4g {
 // grid, or -1 when some entry couples cells further apart than one step.
 static int64_t grid_score(int64_t n, const int64_t* S, const int64_t* I, int64_t nx, int64_t ny, int64_t stride)
 {
-    const int64_t nxy = nx * ny;
-    std::set<int64_t> seen;
+    const int64_t nxy = nx * ny, nz = n / nxy;
+    uint32_t seen = 0;   // bit (di+1) + 3(dj+1) + 9(dk+1)
+    // the 27 neighbour offsets; c - r = off with the neighbour inside the
+    // grid identifies (di, dj, dk) uniquely (mixed-radix digits)
+    int64_t off[27];
+    for (int code = 0; code < 27; code++)
+        off[code] = (code % 3 - 1) + nx * ((code / 3) % 3 - 1) + nxy * (code / 9 - 1);
     for (int64_t r = 0; r < n; r += stride) {
-        int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
+        const int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
+        int guess = 0;
         for (int64_t p = S[r]; p < S[r + 1]; p++) {
-            int64_t c = I[p];
-            int64_t di = c % nx - ri, dj = (c / nx) % ny - rj, dk = c / nxy - rk;
-            if (std::llabs(di) > 1 || std::llabs(dj) > 1 || std::llabs(dk) > 1) return -1;
-            seen.insert((di + 1) + 3 * (dj + 1) + 9 * (dk + 1));
+            const int64_t d = I[p] - r;
+            int found = -1;
+            for (int t = 0; t < 27 && found < 0; t++) {
+                const int code = (guess + t) % 27;
+                if (off[code] != d) continue;
+                const int64_t ci = ri + code % 3 - 1, cj = rj + (code / 3) % 3 - 1, ck = rk + code / 9 - 1;
+                if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz) found = code;
+            }
+            if (found < 0) return -1;
+            seen |= uint32_t(1) << found;
+            guess = found + 1;   // columns ascend: the next entry's offset is larger
         }
     }
-    return int64_t(seen.size());
+    return int64_t(__builtin_popcount(seen));
 }
 
 GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
@@ -55,12 +68,20 @@ GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
     GridInfo best;
     if (n < 4) return best;
     std::set<int64_t> offs;
-    for (int64_t r = 0; r < n; r++)
-        for (int64_t p = S[r]; p < S[r + 1]; p++)
-            if (I[p] < r) {
-                offs.insert(r - I[p]);
-                if (offs.size() > 64) return best;    // not a stencil pattern
-            }
+    {
+        // distinct lower offsets r - c; a short list scanned linearly
+        std::vector<int64_t> seen;
+        for (int64_t r = 0; r < n; r++)
+            for (int64_t p = S[r]; p < S[r + 1]; p++)
+                if (I[p] < r) {
+                    const int64_t d = r - I[p];
+                    if (std::find(seen.begin(), seen.end(), d) == seen.end()) {
+                        seen.push_back(d);
+                        if (seen.size() > 64) return best;    // not a stencil pattern
+                    }
+                }
+        offs.insert(seen.begin(), seen.end());
+    }
     if (offs.empty()) return best;
     std::set<int64_t> nxs;
     for (int64_t d : offs)
@@ -129,8 +150,7 @@ static Coef skew_coefficients(const SweepRows& s, const GridInfo& g)
     if (!g.nx) return k;
     const int64_t nx = g.nx, ny = g.ny, nxy = g.nx * g.ny;
     int64_t a = 0, b = 0;
-    std::vector<std::array<int, 3>> deltas;
-    std::set<std::array<int, 3>> seen;
+    std::vector<std::array<int, 3>> deltas;   // distinct, in order of first appearance
     for (int64_t r = 0; r < s.n; r++) {
         int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
         for (int64_t p = s.ptr[r]; p < s.ptr[r + 1]; p++) {
@@ -138,7 +158,7 @@ static Coef skew_coefficients(const SweepRows& s, const GridInfo& g)
             int di = int(c % nx - ri), dj = int((c / nx) % ny - rj), dk = int(c / nxy - rk);
             if (!s.lower) { di = -di; dj = -dj; dk = -dk; }
             std::array<int, 3> d = {di, dj, dk};
-            if (seen.insert(d).second) deltas.push_back(d);
+            if (std::find(deltas.begin(), deltas.end(), d) == deltas.end()) deltas.push_back(d);
         }
     }
     for (auto& d : deltas) {
@@ -287,31 +307,37 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     p.shape = shape;
     natural_levels(s, &p.levels);
 
-    // 1. tile key of every row, in topological order
-    std::vector<int64_t> key(n);
+    // 1. tile of every row: linear box index, then tickets = the occupied
+    //    boxes in topological order (ascending box index for the lower
+    //    sweep, descending for the upper)
+    std::vector<int64_t> lin(n);
+    int64_t nbox = 1;
     if (shape.grid) {
+        const int64_t N1 = (g.nz - 1) / shape.e1 + 1;
+        const int64_t N2 = ((g.ny - 1) + shape.a * (g.nz - 1)) / shape.e2 + 1;
+        const int64_t N3 = ((g.nx - 1) + shape.b * (g.ny - 1) + shape.c * (g.nz - 1)) / shape.e3 + 1;
+        nbox = N1 * N2 * N3;
+        const int64_t nxy = g.nx * g.ny;
         for (int64_t r = 0; r < n; r++) {
-            int64_t i, j, k;
-            grid_coords(g, r, i, j, k);
+            const int64_t k = r / nxy, j = (r - k * nxy) / g.nx, i = r - k * nxy - j * g.nx;
             int64_t t1 = k, t2 = j + shape.a * k, t3 = i + shape.b * j + shape.c * k;
-            int64_t T1 = t1 / shape.e1, T2 = t2 / shape.e2, T3 = t3 / shape.e3;
-            int64_t N2 = ((g.ny - 1) + shape.a * (g.nz - 1)) / shape.e2 + 1;
-            int64_t N3 = ((g.nx - 1) + shape.b * (g.ny - 1) + shape.c * (g.nz - 1)) / shape.e3 + 1;
-            int64_t lin = (T1 * N2 + T2) * N3 + T3;
-            key[r] = s.lower ? lin : -lin;
+            lin[r] = ((t1 / shape.e1) * N2 + t2 / shape.e2) * N3 + t3 / shape.e3;
         }
     } else {
         int64_t rb = std::max<int64_t>(1, shape.rows_per_block);
-        for (int64_t r = 0; r < n; r++) key[r] = s.lower ? r / rb : -(r / rb);
+        nbox = n ? (n - 1) / rb + 1 : 1;
+        for (int64_t r = 0; r < n; r++) lin[r] = r / rb;
     }
-    // compact keys to ticket order
-    std::vector<int64_t> keys(key);
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    p.ntiles = int64_t(keys.size());
+    std::vector<int32_t> rank(size_t(nbox), -1);
+    for (int64_t r = 0; r < n; r++) rank[size_t(lin[r])] = 0;
+    p.ntiles = 0;
+    for (int64_t b = 0; b < nbox; b++) {
+        int32_t& rk = rank[size_t(s.lower ? b : nbox - 1 - b)];
+        if (rk == 0) rk = int32_t(p.ntiles++);
+    }
     std::vector<int32_t> tile(n);
-    for (int64_t r = 0; r < n; r++)
-        tile[r] = int32_t(std::lower_bound(keys.begin(), keys.end(), key[r]) - keys.begin());
+    for (int64_t r = 0; r < n; r++) tile[r] = rank[size_t(lin[r])];
+    std::vector<int64_t>().swap(lin);
 
     // 2. rows of each tile in natural order -> LDS slots
     p.tile_row_begin.assign(p.ntiles + 1, 0);
@@ -361,20 +387,40 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     if (p.npieces >= (int64_t(1) << 31) / std::max(W, 1)) throw std::runtime_error("sweep too large");
     // 4. groups: pieces ordered by (tile, local level, processing order),
     //    each run of one (tile, level) cut into groups of at most 64
-    std::vector<int32_t> order(p.npieces);
-    std::iota(order.begin(), order.end(), 0);   // pieces were made in processing order
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        const Piece &a = pieces[x], &b = pieces[y];
-        if (tile[a.row] != tile[b.row]) return tile[a.row] < tile[b.row];
-        return a.level < b.level;
-    });
+    // (pieces were made in processing order, so a stable counting sort on
+    // the key (tile, level) gives the (tile, level, processing) order; the
+    // records are moved once, so later passes read them sequentially)
+    {
+        int32_t maxlev = 0;
+        for (const Piece& pc : pieces) maxlev = std::max(maxlev, pc.level);
+        const int64_t nl = int64_t(maxlev) + 1;
+        std::vector<Piece> sorted(size_t(p.npieces));
+        if (p.ntiles * nl <= 4 * p.npieces + 4096) {
+            std::vector<int64_t> cnt(size_t(p.ntiles * nl) + 1, 0);
+            for (const Piece& pc : pieces) cnt[size_t(int64_t(tile[pc.row]) * nl + pc.level) + 1]++;
+            for (size_t k = 1; k < cnt.size(); k++) cnt[k] += cnt[k - 1];
+            for (const Piece& pc : pieces) sorted[size_t(cnt[size_t(int64_t(tile[pc.row]) * nl + pc.level)]++)] = pc;
+        } else {
+            // many (tile, level) pairs: by level, then stably by tile
+            std::vector<int64_t> cnt(size_t(std::max<int64_t>(nl, p.ntiles)) + 1, 0);
+            std::vector<Piece> by_level(size_t(p.npieces));
+            for (const Piece& pc : pieces) cnt[size_t(pc.level) + 1]++;
+            for (int64_t l = 0; l < nl; l++) cnt[size_t(l) + 1] += cnt[size_t(l)];
+            for (const Piece& pc : pieces) by_level[size_t(cnt[size_t(pc.level)]++)] = pc;
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (const Piece& pc : by_level) cnt[size_t(tile[pc.row]) + 1]++;
+            for (int64_t t = 0; t < p.ntiles; t++) cnt[size_t(t) + 1] += cnt[size_t(t)];
+            for (const Piece& pc : by_level) sorted[size_t(cnt[size_t(tile[pc.row])]++)] = pc;
+        }
+        pieces.swap(sorted);
+    }
+    std::vector<int32_t> ptile(size_t(p.npieces));
+    for (int64_t q = 0; q < p.npieces; q++) ptile[size_t(q)] = tile[pieces[size_t(q)].row];
     p.group_piece_begin.clear();
     for (int64_t q = 0; q < p.npieces;) {
-        const Piece& h = pieces[order[q]];
+        const Piece& h = pieces[q];
         int64_t q1 = q;
-        while (q1 < p.npieces && q1 - q < 64 && tile[pieces[order[q1]].row] == tile[h.row] &&
-               pieces[order[q1]].level == h.level)
-            q1++;
+        while (q1 < p.npieces && q1 - q < 64 && ptile[q1] == ptile[q] && pieces[q1].level == h.level) q1++;
         p.group_piece_begin.push_back(int32_t(q));
         q = q1;
     }
@@ -382,7 +428,7 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     p.group_piece_begin.push_back(int32_t(p.npieces));
     p.tile_group_begin.assign(p.ntiles + 1, int32_t(p.ngroups));
     for (int64_t gi = p.ngroups - 1; gi >= 0; gi--)
-        p.tile_group_begin[tile[pieces[order[p.group_piece_begin[gi]]].row]] = int32_t(gi);
+        p.tile_group_begin[ptile[p.group_piece_begin[gi]]] = int32_t(gi);
     for (int64_t t = p.ntiles - 1; t >= 0; t--)   // tiles without rows: empty group range
         if (p.tile_row_begin[t + 1] == p.tile_row_begin[t]) p.tile_group_begin[t] = p.tile_group_begin[t + 1];
     for (int64_t t = 0; t < p.ntiles; t++)
@@ -394,7 +440,7 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
         std::vector<int32_t> next(p.ntiles, 0);
         std::vector<char> placed(n, 0);
         for (int64_t q = 0; q < p.npieces; q++) {
-            const int32_t r = pieces[order[q]].row;
+            const int32_t r = pieces[q].row;
             if (!placed[r]) {
                 placed[r] = 1;
                 slot[r] = next[tile[r]]++;
@@ -416,7 +462,7 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
         for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
             p.group_xbefore[gi] = int32_t(p.xlist.size()) - xb;
             for (int32_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++) {
-                const Piece& pc = pieces[order[q]];
+                const Piece& pc = pieces[q];
                 const int32_t r = pc.row;
                 if (exported[r] && pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r]) {
                     xslot[r] = int32_t(p.xlist.size());
@@ -437,7 +483,14 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     p.pc_src.assign(size_t(p.npieces) * W, 0);
     p.pc_val_pos.assign(size_t(p.npieces) * W, -1);
     for (int64_t q = 0; q < p.npieces; q++) {
-        const Piece& pc = pieces[order[q]];
+        if (q + 16 < p.npieces) {   // rows of a group are far apart: fetch ahead
+            const int32_t rf = pieces[q + 16].row;
+            __builtin_prefetch(&s.ptr[rf]);
+            __builtin_prefetch(&s.col[s.ptr[rf] + pieces[q + 16].k0]);
+            __builtin_prefetch(&s.val_pos[s.ptr[rf] + pieces[q + 16].k0]);
+            __builtin_prefetch(&slot[rf]);
+        }
+        const Piece& pc = pieces[q];
         const int32_t r = pc.row;
         const bool last = pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r];
         p.pc_slot[q] = slot[r];
@@ -625,6 +678,8 @@ bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<in
     US.assign(n + 1, 0);
     UI.clear();
     a_pos.clear();
+    UI.reserve(size_t((S[n] + n) / 2 + 1));
+    a_pos.reserve(UI.capacity());
     for (int64_t r = 0; r < n; r++) {
         bool first = true;
         for (int64_t p = S[r]; p < S[r + 1]; p++) {
@@ -649,6 +704,8 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
     upper.ptr.assign(n + 1, 0);
     upper.col.clear();
     upper.val_pos.clear();
+    upper.col.reserve(UI.size());
+    upper.val_pos.reserve(UI.size());
     upper.diag_pos.assign(n, -1);
     for (int64_t r = 0; r < n; r++) {
         if (US[r] == US[r + 1] || UI[US[r]] != r) return false;

@@ -19,9 +19,10 @@
 namespace m4g {
 
 __global__ __launch_bounds__(kSpmvNT) void k_sell_fill(int64_t n, const int64_t* __restrict__ slice_ptr,
+                                                       const int64_t* __restrict__ slice_idx,
                                                        const int32_t* __restrict__ starts,
                                                        const int32_t* __restrict__ indices,
-                                                       const double* __restrict__ values, int32_t* __restrict__ cols,
+                                                       const double* __restrict__ values, uint8_t* __restrict__ idx,
                                                        double* __restrict__ vals, int64_t* __restrict__ pos)
 {
     int64_t r = blockIdx.x * int64_t(kSpmvNT) + threadIdx.x;
@@ -31,6 +32,9 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_fill(int64_t n, const int64_t*
     if (s >= nslices) return;
     int64_t base = slice_ptr[s];
     int width = int((slice_ptr[s + 1] - base) / kWave);
+    const int64_t si = slice_idx[s];
+    int32_t* wide = reinterpret_cast<int32_t*>(idx + (si >> 1));
+    int16_t* narrow = reinterpret_cast<int16_t*>(idx + (si >> 1));
     int64_t b = 0, len = 0;
     if (r < n) {
         b = starts[r];
@@ -38,12 +42,15 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_fill(int64_t n, const int64_t*
     }
     for (int k = 0; k < width; k++) {
         int64_t d = base + int64_t(k) * kWave + lane;
+        int64_t e = int64_t(k) * kWave + lane;
         if (k < len) {
-            cols[d] = indices[b + k];
+            if (si & 1) wide[e] = indices[b + k];
+            else narrow[e] = int16_t(indices[b + k] - r);
             vals[d] = values[b + k];
             pos[d] = b + k;
         } else {
-            cols[d] = -1;
+            if (si & 1) wide[e] = -1;
+            else narrow[e] = kSellPad16;
             vals[d] = 0.0;
             pos[d] = -1;
         }
@@ -71,28 +78,43 @@ void sell_build(m4g_matrix_s& A)
     Sell& S = A.sell;
     S.n = A.nrows;
     S.nslices = (A.nrows + kWave - 1) / kWave;
-    std::vector<int64_t> sp(S.nslices + 1, 0);
+    std::vector<int64_t> sp(S.nslices + 1, 0), si(std::max<int64_t>(S.nslices, 1), 0);
     int maxw = 0;
+    int64_t ibytes = 0;
+    S.narrow_slices = 0;
     for (int64_t s = 0; s < S.nslices; s++) {
         int64_t r0 = s * kWave, r1 = std::min<int64_t>(A.nrows, r0 + kWave);
         int64_t w = 0;
-        for (int64_t r = r0; r < r1; r++) w = std::max<int64_t>(w, A.h_starts[r + 1] - A.h_starts[r]);
+        bool narrow = true;
+        for (int64_t r = r0; r < r1; r++) {
+            w = std::max<int64_t>(w, A.h_starts[r + 1] - A.h_starts[r]);
+            for (int64_t e = A.h_starts[r]; e < A.h_starts[r + 1]; e++) {
+                const int64_t d = A.h_indices[e] - r;
+                narrow = narrow && d >= -32767 && d <= 32767;
+            }
+        }
         maxw = std::max<int>(maxw, int(w));
         sp[s + 1] = sp[s] + w * kWave;
+        si[s] = (ibytes << 1) | (narrow ? 0 : 1);
+        ibytes += w * kWave * (narrow ? 2 : 4);
+        S.narrow_slices += narrow;
     }
     S.padded = sp[S.nslices];
     S.max_width = maxw;
     M4G_REQUIRE(S.padded < (int64_t(1) << 40), M4G_ERR_INVALID, "matrix too large");
     S.slice_ptr.reset(S.nslices + 1);
     S.slice_ptr.upload(sp.data(), S.nslices + 1);
-    S.cols.reset(S.padded);
+    S.slice_idx.reset(si.size());
+    S.slice_idx.upload(si.data(), si.size());
+    S.idx.reset(std::max<int64_t>(ibytes, 4));
     S.vals.reset(S.padded);
     S.csr_pos.reset(S.padded);
     S.vals32.release();
     if (S.nslices) {
         int64_t threads = S.nslices * kWave;
         hipLaunchKernelGGL(k_sell_fill, dim3((threads + kSpmvNT - 1) / kSpmvNT), dim3(kSpmvNT), 0, current_stream(),
-                           A.nrows, S.slice_ptr.ptr, A.starts.ptr, A.indices.ptr, A.values.ptr, S.cols.ptr,
+                           A.nrows, S.slice_ptr.ptr, S.slice_idx.ptr, A.starts.ptr, A.indices.ptr, A.values.ptr,
+                           S.idx.ptr,
                            S.vals.ptr, S.csr_pos.ptr);
         M4G_HIP(hipGetLastError());
     }
@@ -130,7 +152,8 @@ void sell_build_f32(m4g_matrix_s& A)
 template <int MODE, bool DOT>
 __global__ __launch_bounds__(kSpmvNT) void k_sell_spmv(int64_t n, int64_t nslices, int64_t chunk,
                                                        const int64_t* __restrict__ slice_ptr,
-                                                       const int32_t* __restrict__ cols,
+                                                       const int64_t* __restrict__ slice_idx,
+                                                       const uint8_t* __restrict__ idx,
                                                        const double* __restrict__ vals, const double* __restrict__ x,
                                                        double* __restrict__ w, const double* __restrict__ b,
                                                        double* partials, unsigned* counter, double* out)
@@ -141,9 +164,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_spmv(int64_t n, int64_t nslice
     const int64_t s = g * kSliceWaves + wave;
     double dotp = 0;
     if (s < nslices) {
-        const int64_t base = slice_ptr[s];
-        const int width = int((slice_ptr[s + 1] - base) / kWave);
-        const double acc = sell_row_sum(cols, vals, x, base, width, lane);
+        const double acc = sell_row_sum(slice_ptr, slice_idx, idx, vals, x, s, lane);
         const int64_t row = s * kWave + lane;
         if (row < n) {
             if (MODE == 0) w[row] = acc;
@@ -162,7 +183,8 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_spmv(int64_t n, int64_t nslice
 template <typename T>
 __global__ __launch_bounds__(kSpmvNT) void k_sell_residual(int64_t n, int64_t nslices, int64_t chunk,
                                                            const int64_t* __restrict__ slice_ptr,
-                                                           const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                           const int64_t* __restrict__ slice_idx,
+                                                           const uint8_t* __restrict__ idx, const T* __restrict__ vals,
                                                            const T* __restrict__ x, const T* __restrict__ b,
                                                            T* __restrict__ r)
 {
@@ -170,9 +192,7 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_residual(int64_t n, int64_t ns
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t s = g * kSliceWaves + wave;
     if (s >= nslices) return;
-    const int64_t base = slice_ptr[s];
-    const int width = int((slice_ptr[s + 1] - base) / kWave);
-    const T acc = sell_row_sum(cols, vals, x, base, width, lane);
+    const T acc = sell_row_sum(slice_ptr, slice_idx, idx, vals, x, s, lane);
     const int64_t row = s * kWave + lane;
     if (row < n) r[row] = b[row] - acc;
 }
@@ -191,7 +211,7 @@ void launch_spmv(const m4g_matrix_s& A, const double* x, double* w, int mode, co
     spmv_geometry(A, chunk, grid);
     const Sell& S = A.sell;
 #define M4G_SPMV_ARGS                                                                                   \
-    dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk, S.slice_ptr.ptr, S.cols.ptr, S.vals.ptr, x, w, b, \
+    dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk, S.slice_ptr.ptr, S.slice_idx.ptr, S.idx.ptr, S.vals.ptr, x, w, b, \
         (double*)nullptr, (unsigned*)nullptr, (double*)nullptr
     if (b) hipLaunchKernelGGL((k_sell_spmv<3, false>), M4G_SPMV_ARGS);
     else if (mode == M4G_ASSIGN) hipLaunchKernelGGL((k_sell_spmv<0, false>), M4G_SPMV_ARGS);
@@ -211,7 +231,7 @@ void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* 
     spmv_geometry(A, chunk, grid);
     const Sell& S = A.sell;
     hipLaunchKernelGGL((k_sell_spmv<0, true>), dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
-                       S.slice_ptr.ptr, S.cols.ptr, S.vals.ptr, p, q, (const double*)nullptr,
+                       S.slice_ptr.ptr, S.slice_idx.ptr, S.idx.ptr, S.vals.ptr, p, q, (const double*)nullptr,
                        A.red_partials.ptr, A.red_counter.ptr, out);
     launch_sum_partials(A.red_partials.ptr, int(grid), out, st);
     M4G_HIP(hipGetLastError());
@@ -245,7 +265,7 @@ void launch_residual_t(const m4g_matrix_s& A, const T* x, const T* b, T* r, hipS
     const T* vals = sizeof(T) == 8 ? (const T*)S.vals.ptr : (const T*)S.vals32.ptr;
     M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
     hipLaunchKernelGGL(k_sell_residual<T>, dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
-                       S.slice_ptr.ptr, S.cols.ptr, vals, x, b, r);
+                       S.slice_ptr.ptr, S.slice_idx.ptr, S.idx.ptr, vals, x, b, r);
     M4G_HIP(hipGetLastError());
 }
 

@@ -65,6 +65,79 @@ def test_spmv_bitexact(g, name):
     assert abs(dg - d) <= 1e-12 * abs(d)
 
 
+def mixed_width_matrix(n=70000, seed=5):
+    """Rows with near entries only (16-bit column deltas), far entries in every
+    third slice of 64 rows (32-bit columns), deltas of exactly ±32767 (still
+    16-bit) and +32768 (32-bit), empty rows and ragged lengths."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for r in range(n):
+        if r % 97 == 5:
+            rows.append([])
+            continue
+        c = {r}
+        if r > 0:
+            c.add(r - 1)
+        if r + 1 < n:
+            c.add(r + 1)
+        if (r // 64) % 3 == 0:
+            c.add((r + 40000) % n)
+        if r % 5 == 0 and r - 32767 >= 0:
+            c.add(r - 32767)
+        if r % 7 == 0 and r + 32768 < n:
+            c.add(r + 32768)
+        if r % 11 == 0 and r + 32767 < n:
+            c.add(r + 32767)
+        rows.append(sorted(c))
+    S = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int64)
+    I = np.array([j for c in rows for j in c], np.int64)
+    V = rng.standard_normal(len(I))
+    return S, I, V
+
+
+def test_spmv_mixed_index_widths(g):
+    say("spmv mixed 16/32-bit slices")
+    S, I, V = mixed_width_matrix()
+    A = o.CSR(S, I, V)
+    Ag = g.Matrix(S, I, V)
+    x = g.rng_uniform(4, 1, A.n)
+    w0 = g.rng_uniform(4, 2, A.n)
+    xv, wv, rv = g.Vector(data=x), g.Vector(data=w0), g.Vector(A.n)
+    g.spmv(Ag, xv, wv, g.ASSIGN)
+    np.testing.assert_array_equal(wv.numpy(), o.spmv(A, x))
+    wv.upload(w0)
+    g.spmv(Ag, xv, wv, g.MINUS)
+    np.testing.assert_array_equal(wv.numpy(), o.spmv(A, x, w0, 2))
+    g.residual(Ag, xv, g.Vector(data=w0), rv)
+    np.testing.assert_array_equal(rv.numpy(), o.residual(A, x, w0))
+    q, d = o.fused_spmv_dot(A, x)
+    qv = g.Vector(A.n)
+    dg = g.spmv_dot(Ag, xv, qv)
+    np.testing.assert_array_equal(qv.numpy(), q)
+    assert abs(dg - d) <= 1e-12 * abs(d)
+
+
+def test_spmv_more_rows_than_columns(g):
+    say("spmv 300 x 40")
+    rng = np.random.default_rng(9)
+    m, n = 300, 40
+    rows = [sorted(rng.choice(n, size=int(rng.integers(0, 6)), replace=False)) for _ in range(m)]
+    S = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int64)
+    I = np.array([j for c in rows for j in c], np.int64)
+    V = rng.standard_normal(len(I))
+    Ag = g.Matrix(S, I, V, ncols=n)
+    x = rng.standard_normal(n)
+    w = g.Vector(m)
+    g.spmv(Ag, g.Vector(data=x), w)
+    ref = np.zeros(m)
+    for r in range(m):   # sequential row sums in stored order, as the oracle
+        acc = 0.0
+        for e in range(S[r], S[r + 1]):
+            acc += V[e] * x[I[e]]
+        ref[r] = acc
+    np.testing.assert_array_equal(w.numpy(), ref)
+
+
 def test_matrix_vector_product_reference_stencil(g):
     # T-mtl/matrix_vector_product_test.cpp:56-89
     S, I, V = g.laplacian2d(5, 5)
