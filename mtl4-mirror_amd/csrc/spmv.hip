@@ -73,6 +73,8 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_to_f32(int64_t padded, const d
         f[d] = float(v[d]);
 }
 
+static int pipe_grid();
+
 void sell_build(m4g_matrix_s& A)
 {
     Sell& S = A.sell;
@@ -106,8 +108,10 @@ void sell_build(m4g_matrix_s& A)
     S.slice_ptr.upload(sp.data(), S.nslices + 1);
     S.slice_idx.reset(si.size());
     S.slice_idx.upload(si.data(), si.size());
-    S.idx.reset(std::max<int64_t>(ibytes, 4));
-    S.vals.reset(S.padded);
+    S.idx.reset(ibytes + 4 * kSellSlack);
+    // kSellSlack entries after the last slice keep the pipelined kernel's
+    // look-ahead loads of an empty final slice in bounds
+    S.vals.reset(S.padded + kSellSlack);
     S.csr_pos.reset(S.padded);
     S.vals32.release();
     if (S.nslices) {
@@ -121,7 +125,7 @@ void sell_build(m4g_matrix_s& A)
     // reduction workspace for the fused SpMV + dot
     int64_t groups = (S.nslices + kSliceWaves - 1) / kSliceWaves;
     int64_t chunk = (groups + kXcds - 1) / kXcds;
-    A.red_partials.reset(std::max<int64_t>(1, chunk * kXcds));
+    A.red_partials.reset(std::max<int64_t>({int64_t(1), chunk * kXcds, int64_t(pipe_grid())}));
     A.red_counter.reset(1);
     M4G_HIP(hipMemsetAsync(A.red_counter.ptr, 0, sizeof(unsigned), current_stream()));
 }
@@ -141,10 +145,138 @@ void sell_build_f32(m4g_matrix_s& A)
 {
     Sell& S = A.sell;
     if (!S.padded) return;
-    if (!S.vals32.ptr) S.vals32.reset(S.padded);
+    if (!S.vals32.ptr) S.vals32.reset(S.padded + kSellSlack);
     int64_t g = std::min<int64_t>((S.padded + kSpmvNT - 1) / kSpmvNT, 16384);
     hipLaunchKernelGGL(k_sell_to_f32, dim3(g), dim3(kSpmvNT), 0, current_stream(), S.padded, S.vals.ptr,
                        S.vals32.ptr);
+    M4G_HIP(hipGetLastError());
+}
+
+// Pipelined SpMV for matrices whose slices all use 16-bit offsets and hold
+// at most MAXW entries per row: a persistent grid (kPipeBlocksPerCU workgroups
+// per CU) in which wavefront g takes slices g, g + G, g + 2G, ... (G = all
+// wavefronts, so the whole chip streams one front through the matrix) and
+// issues the value/offset loads of its next slice before it waits for the x
+// gathers of the current one: HBM loads stay in flight through the gather
+// phase.  Row sums as sell_row_sum (stored order, padding skipped).
+// MODE 0: w = A v; 1: w += A v; 2: w -= A v; 3: w = b - A v.  DOT: per
+// workgroup partial of v.(A v) over its slices.
+template <typename T, int MODE, bool DOT, int MAXW>
+__global__ __launch_bounds__(kSpmvNT) void k_sell_pipe(int64_t n, int64_t nslices, const int64_t* __restrict__ slice_ptr,
+                                                       const int64_t* __restrict__ slice_idx,
+                                                       const uint8_t* __restrict__ idx, const T* __restrict__ vals,
+                                                       const T* __restrict__ x, T* __restrict__ w,
+                                                       const T* __restrict__ b, double* partials)
+{
+    __shared__ double scratch[kSliceWaves];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+    const int64_t stride = int64_t(gridDim.x) * kSliceWaves;
+    int64_t s = int64_t(blockIdx.x) * kSliceWaves + wave;
+    double dotp = 0;
+    if (s < nslices) {
+        // entries k >= the slice's width re-load its last column (an empty
+        // slice: the entry after it, or the slack, read as padding) and are
+        // not summed
+        // every slice narrow: its offsets start at entry slice_ptr[s] of idx
+        const int16_t* i16 = reinterpret_cast<const int16_t*>(idx);
+        int64_t base = slice_ptr[s];
+        int wd = int((slice_ptr[s + 1] - base) / kWave);
+        const int16_t* d16 = i16 + base + lane;
+        int16_t d[MAXW];
+        T v[MAXW];
+#pragma unroll
+        for (int k = 0; k < MAXW; k++) {
+            const int kk = k < wd ? k : (wd > 0 ? wd - 1 : 0);
+            const int16_t t = d16[kk * kWave];
+            d[k] = wd > 0 ? t : kSellPad16;
+            v[k] = vals[base + kk * kWave + lane];
+        }
+        for (;;) {
+            const int64_t sn = s + stride;
+            const bool more = sn < nslices;
+            const int64_t row = s * kWave + lane;
+            const int64_t rr = row < n ? row : n - 1;
+            T xv[MAXW];
+#pragma unroll
+            for (int k = 0; k < MAXW; k++) xv[k] = x[d[k] == kSellPad16 ? 0 : row + d[k]];
+            T prior = 0;
+            if (MODE == 1 || MODE == 2) prior = w[rr];
+            if (MODE == 3) prior = b[rr];
+            // look-ahead: the next slice's offsets and values (this slice again at the end)
+            const int64_t sl = more ? sn : s;
+            const int64_t bn = slice_ptr[sl];
+            const int wdn = int((slice_ptr[sl + 1] - bn) / kWave);
+            const int16_t* dn16 = i16 + bn + lane;
+            int16_t dn[MAXW];
+            T vn[MAXW];
+#pragma unroll
+            for (int k = 0; k < MAXW; k++) {
+                const int kk = k < wdn ? k : (wdn > 0 ? wdn - 1 : 0);
+                const int16_t t = dn16[kk * kWave];
+                dn[k] = wdn > 0 ? t : kSellPad16;
+                vn[k] = vals[bn + kk * kWave + lane];
+            }
+            T acc = 0;
+#pragma unroll
+            for (int k = 0; k < MAXW; k++)
+                if (k < wd && d[k] != kSellPad16) acc += v[k] * xv[k];
+            if (row < n) {
+                if (MODE == 0) w[row] = acc;
+                else if (MODE == 1) w[row] = prior + acc;
+                else if (MODE == 2) w[row] = prior - acc;
+                else w[row] = prior - acc;
+                if (DOT) dotp += double(x[row] * acc);
+            }
+            if (!more) break;
+            s = sn;
+            wd = wdn;
+#pragma unroll
+            for (int k = 0; k < MAXW; k++) {
+                d[k] = dn[k];
+                v[k] = vn[k];
+            }
+        }
+    }
+    if (DOT) {
+        double blk = block_sum<kSpmvNT>(dotp, scratch);
+        if (threadIdx.x == 0) partials[blockIdx.x] = blk;
+    }
+}
+
+// the pipelined kernel applies: every slice narrow, rows of at most 16 entries
+static int pipe_width(const m4g_matrix_s& A)
+{
+    const Sell& S = A.sell;
+    if (A.ncols == 0 || S.nslices == 0 || S.narrow_slices != S.nslices || std::getenv("M4G_SPMV_NO_PIPE")) return 0;
+    if (S.max_width <= 3) return 3;
+    if (S.max_width <= 8) return S.max_width;
+    if (S.max_width <= 16) return 16;
+    return 0;
+}
+
+static int pipe_grid()
+{
+    return std::max(1, device_state().num_cus) * kPipeBlocksPerCU;
+}
+
+template <typename T, int MODE, bool DOT>
+static void launch_pipe(const m4g_matrix_s& A, int mw, const T* vals, const T* x, T* w, const T* b, double* partials,
+                        hipStream_t st)
+{
+    const Sell& S = A.sell;
+#define M4G_PIPE(MW)                                                                                                \
+    hipLaunchKernelGGL((k_sell_pipe<T, MODE, DOT, MW>), dim3(pipe_grid()), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, \
+                       S.slice_ptr.ptr, S.slice_idx.ptr, S.idx.ptr, vals, x, w, b, partials)
+    switch (mw) {
+    case 3: M4G_PIPE(3); break;
+    case 4: M4G_PIPE(4); break;
+    case 5: M4G_PIPE(5); break;
+    case 6: M4G_PIPE(6); break;
+    case 7: M4G_PIPE(7); break;
+    case 8: M4G_PIPE(8); break;
+    default: M4G_PIPE(16); break;
+    }
+#undef M4G_PIPE
     M4G_HIP(hipGetLastError());
 }
 
@@ -207,6 +339,14 @@ static void spmv_geometry(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
 void launch_spmv(const m4g_matrix_s& A, const double* x, double* w, int mode, const double* b, hipStream_t st)
 {
     if (A.nrows == 0) return;
+    if (const int mw = pipe_width(A)) {
+        const double* v = A.sell.vals.ptr;
+        if (b) launch_pipe<double, 3, false>(A, mw, v, x, w, b, nullptr, st);
+        else if (mode == M4G_ASSIGN) launch_pipe<double, 0, false>(A, mw, v, x, w, b, nullptr, st);
+        else if (mode == M4G_PLUS) launch_pipe<double, 1, false>(A, mw, v, x, w, b, nullptr, st);
+        else launch_pipe<double, 2, false>(A, mw, v, x, w, b, nullptr, st);
+        return;
+    }
     int64_t chunk, grid;
     spmv_geometry(A, chunk, grid);
     const Sell& S = A.sell;
@@ -225,6 +365,13 @@ void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* 
 {
     if (A.nrows == 0) {
         M4G_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
+        return;
+    }
+    if (const int mw = pipe_width(A)) {
+        M4G_REQUIRE(A.red_partials.count >= size_t(pipe_grid()), M4G_ERR_INVALID, "reduction workspace too small");
+        launch_pipe<double, 0, true>(A, mw, A.sell.vals.ptr, p, q, (const double*)nullptr, A.red_partials.ptr, st);
+        launch_sum_partials(A.red_partials.ptr, pipe_grid(), out, st);
+        M4G_HIP(hipGetLastError());
         return;
     }
     int64_t chunk, grid;
@@ -264,6 +411,10 @@ void launch_residual_t(const m4g_matrix_s& A, const T* x, const T* b, T* r, hipS
     const Sell& S = A.sell;
     const T* vals = sizeof(T) == 8 ? (const T*)S.vals.ptr : (const T*)S.vals32.ptr;
     M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
+    if (const int mw = pipe_width(A)) {
+        launch_pipe<T, 3, false>(A, mw, vals, x, r, b, nullptr, st);
+        return;
+    }
     hipLaunchKernelGGL(k_sell_residual<T>, dim3(grid), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, chunk,
                        S.slice_ptr.ptr, S.slice_idx.ptr, S.idx.ptr, vals, x, b, r);
     M4G_HIP(hipGetLastError());

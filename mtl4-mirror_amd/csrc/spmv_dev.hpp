@@ -16,6 +16,8 @@ constexpr int kSpmvNT = kSliceWaves * kWave;   // 256 threads
 // banded/stencil matrices of the solvers and halves the index bytes.  Bit 0
 // set ("wide"): int32 columns, −1 for padding.
 constexpr int16_t kSellPad16 = -32768;
+constexpr int64_t kSellSlack = 64;      // entries allocated after the last slice
+constexpr int kPipeBlocksPerCU = 3;     // pipelined SpMV: persistent workgroups per CU
 
 // Row sum of lane `lane` of slice s: sequential in stored order from zero, as
 // mat_vec_mult.hpp:662-687; padding entries are skipped, not added as zeros.

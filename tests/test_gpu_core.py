@@ -117,6 +117,57 @@ def test_spmv_mixed_index_widths(g):
     assert abs(dg - d) <= 1e-12 * abs(d)
 
 
+def banded_ragged_matrix(n, maxlen, empty, seed):
+    """Columns within +-100 of the row (16-bit offsets), 0..maxlen entries per
+    row, the rows in `empty` without entries."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for r in range(n):
+        if r in empty:
+            rows.append([])
+            continue
+        lo, hi = max(0, r - 100), min(n, r + 101)
+        k = int(rng.integers(1, maxlen + 1))
+        rows.append(sorted(rng.choice(np.arange(lo, hi), size=min(k, hi - lo), replace=False)))
+    S = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int64)
+    I = np.array([j for c in rows for j in c], np.int64)
+    V = rng.standard_normal(len(I))
+    return S, I, V
+
+
+@pytest.mark.parametrize("case", ["w4", "w8", "w16", "empty"])
+def test_spmv_pipelined_shapes(g, case):
+    # the pipelined SpMV (all slices 16-bit, rows of <= 16 entries): whole
+    # empty slices, an empty last slice, ragged rows, every mode and the fused dot
+    say(f"spmv pipelined {case}")
+    n = 5000
+    if case == "empty":
+        S, I, V = np.zeros(301, np.int64), np.zeros(0, np.int64), np.zeros(0)
+        n = 300
+    else:
+        maxlen = {"w4": 4, "w8": 7, "w16": 13}[case]
+        empty = set(range(64, 128)) | set(range(4992, 5000)) | {7, 300, 301}
+        S, I, V = banded_ragged_matrix(n, maxlen, empty, seed=len(case))
+    A = o.CSR(S, I, V)
+    Ag = g.Matrix(S, I, V)
+    x = g.rng_uniform(6, 1, n)
+    w0 = g.rng_uniform(6, 2, n)
+    xv, wv, rv = g.Vector(data=x), g.Vector(data=w0), g.Vector(n)
+    g.spmv(Ag, xv, wv, g.ASSIGN)
+    np.testing.assert_array_equal(wv.numpy(), o.spmv(A, x))
+    for mode, m in ((g.PLUS, 1), (g.MINUS, 2)):
+        wv.upload(w0)
+        g.spmv(Ag, xv, wv, mode)
+        np.testing.assert_array_equal(wv.numpy(), o.spmv(A, x, w0, m))
+    g.residual(Ag, xv, g.Vector(data=w0), rv)
+    np.testing.assert_array_equal(rv.numpy(), o.residual(A, x, w0))
+    q, d = o.fused_spmv_dot(A, x)
+    qv = g.Vector(n)
+    dg = g.spmv_dot(Ag, xv, qv)
+    np.testing.assert_array_equal(qv.numpy(), q)
+    assert abs(dg - d) <= 1e-12 * max(abs(d), 1e-300)
+
+
 def test_spmv_more_rows_than_columns(g):
     say("spmv 300 x 40")
     rng = np.random.default_rng(9)
