@@ -114,6 +114,12 @@ def batched(g, nsys=512, nrhs=8, sample=16):
     solve_ms = (time.perf_counter() - t0) * 1e3
     nb = np.array([[np.linalg.norm(bs[k][j]) for j in range(nrhs)] for k in range(nsys)])
     relres = float(np.max(res / nb))
+    # HBM bytes the solve moves, per (system row, rhs, iteration): 17 vector
+    # passes of 8 bytes (sweeps r->t->z with r for r.z, p update, q = A p with
+    # p for p.q, x/r update) plus the matrix rows (5 x 12 B) and both packed
+    # sweeps (2 x 36 B) shared by the rhs of a work item (8 here)
+    row_its = float(np.sum(its * sizes[:, None]))
+    solve_bytes = row_its * (17 * 8 + (5 * 12 + 2 * 36) / 8.0)
     # loop of single solves on a sample spread over the sizes
     order = np.argsort(sizes, kind="stable")
     pick = [int(order[(2 * k + 1) * nsys // (2 * sample)]) for k in range(sample)]
@@ -143,6 +149,8 @@ def batched(g, nsys=512, nrhs=8, sample=16):
             "iterations_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
             "loop_sample_systems": sample, "loop_sample_ms": round(loop_ms, 1),
             "loop_estimate_ms": round(loop_est_ms, 1),
+            "solve_hbm_bytes_est": int(solve_bytes),
+            "solve_gbs_est": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 1),
             "speedup_vs_loop": round(loop_est_ms / batched_ms, 2)}
 
 

@@ -235,7 +235,8 @@ int m4g_batch_download_factor(m4g_batch B, double* values);
  * iterations, final residual norm, error (0 converged, 1 too many iterations). */
 int m4g_batch_cg_ic0(m4g_batch B, int nrhs, m4g_vector x, m4g_vector b, int max_iter, double rtol,
                      double atol, const int* rhs_per_item, int block, int* iterations, double* resid, int* error);
-/* Autotuner: times trial_iters iterations (0: 20) of the whole batch, first
+/* Autotuner: times capped runs of the whole batch (system s runs
+ * trial_iters * sqrt(n_s / n_max) iterations, trial_iters 0: 40), first
  * for every block size (256, 512, 1024) with one rhs grouping (8, 4, 2, 1
  * dividing nrhs) for all systems, then, with the fastest block size, lets each
  * third of the systems by size try the other groupings.  Returns the block

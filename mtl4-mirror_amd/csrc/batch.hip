@@ -33,6 +33,8 @@ struct BatchSys {
     int64_t lvl_l, nlev_l;    // levels of the lower sweep: first pointer, count
     int64_t lvl_u, nlev_u;    // levels of the upper sweep
     int64_t rows_l, rows_u;   // first entry of each sweep's rows in level order
+    int64_t reach_l, reach_u; // largest (end of a level) - (position of a dependency of that level)
+    int64_t level_width;      // rows of the widest level of either sweep
 };
 
 // Each sweep's rows in level order, packed for the solve: the row, its
@@ -58,7 +60,9 @@ struct m4g_batch_s {
     m4g::DevBuf<int64_t> u_apos;
     m4g::DevBuf<double> a_v, u_v;
     m4g::DevBuf<double> work;           // r, z, p, q and the sweep intermediate of every (system, rhs)
-    int wl = 0, wu = 0;                 // widths of the packed lower / upper sweeps
+    int W = 0;                          // width of the packed sweeps; rows of A hold 2W+1 entries
+    m4g::DevBuf<int32_t> a_eci;         // A as fixed-width rows (-1 = none)
+    m4g::DevBuf<double> a_ev;
     m4g::DevBuf<int32_t> lq_row, lq_ci, lq_up, lq_dp, uq_row, uq_ci, uq_up, uq_dp;   // positions into U
     m4g::DevBuf<int64_t> q_ubase;       // [n_total] first U entry of each position's system
     m4g::DevBuf<double> lq_val, lq_diag, uq_val, uq_diag;
@@ -129,8 +133,9 @@ __global__ void k_batch_pack(int64_t count, int w, const int32_t* __restrict__ u
 struct BatchCgArgs {
     BatchSweep lo, up;
     const BatchSys* sys;
-    const int32_t *a_st, *a_ci, *u_st, *u_ci, *l_st, *l_ci, *l_up, *lv_ptr, *lv_rows;
-    const double *a_v, *u_v;
+    const int32_t* a_ci;       // A as fixed-width rows: [n_total * (2W+1)], -1 = none
+    const double* a_v;
+    const int32_t* lv_ptr;
     const int4* items;         // in processing order: system, first rhs, rhs count
     int32_t nitems, nrhs;
     unsigned* queue;
@@ -138,7 +143,10 @@ struct BatchCgArgs {
     double* x;
     double *r, *z, *p, *q, *t;   // work vectors, same layout as x
     int max_iter;
+    const int* sys_max_iter;   // per-system iteration limit (autotuner trials) or null
     double rtol, atol;
+    int ring_doubles;          // dynamic LDS of the sweep ring (a power of two)
+    double* dump;              // [grid * NT] stores of unused sweep pairs (never read)
     int* out_iters;            // [nsys * nrhs]
     double* out_resid;
     int* out_error;
@@ -157,37 +165,173 @@ __device__ inline void rhs_sums(double v, double* red, int R)
     }
 }
 
-// One triangular sweep, level by level, over a system's packed rows:
-// out_i = (in_i - sum_k val_k * out_{ci_k}) * diag_i in stored order.  The
-// metadata loads of a row are independent of each other, so a level costs two
-// dependent memory round trips (row data, then the neighbours' values) and a
-// workgroup barrier.
-template <typename At>
-__device__ inline void sweep_levels(const BatchSweep& w, const BatchSys& S, const int32_t* lp, int64_t nlev, int r0,
-                                    int rstep, bool run, const double* in, double* out, At at)
+typedef __attribute__((address_space(3))) double lds_double;   // LDS-qualified, so accesses are ds_* ops
+
+// Barrier for values exchanged through LDS only: waits for this wave's LDS
+// writes, not for its outstanding global loads and stores.
+__device__ inline void lds_barrier()
 {
-    const int32_t* row = w.row + S.row0;
-    const int32_t* ci = w.ci + S.row0 * w.w;
-    const double* val = w.val + S.row0 * w.w;
-    const double* diag = w.diag + S.row0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The inputs of one (position, rhs) pair of a sweep that do not depend on
+// earlier levels: the row, its dependencies (positions in the sweep's level
+// order, -1 = none), factor values and inverted diagonal.
+template <int W>
+struct SweepMeta {
+    int32_t q, i;
+    int32_t c[W];
+    double v[W];
+    double d;
+};
+
+// One triangular sweep of one system, level by level:
+// out_i = (in_i - sum_k val_k * out_{c_k}) * diag_i in stored order.  Results
+// also go to an LDS ring indexed by level-order position (ring_pos slots per
+// rhs, plus one scratch slot).  DOT: also accumulates dv_i * out_i into `dot`.
+//
+// Pipelined form, for systems whose levels fit in one pass of the block and
+// whose dependencies all lie within ring_pos positions of the end of their
+// level: a level costs a barrier and an LDS round trip.  Each thread's pair of
+// level l + D is loaded while level l runs and its right-hand side (which
+// needs the row) two levels ahead; all loads are unconditional (clamped to
+// position 0, results of unused pairs go to the scratch slot) and the level
+// loop is unrolled D times so slot indices are constants: the compiler's wait
+// counts then never hold a level for a load issued after the level began.
+template <int W, bool DOT, typename At>
+__device__ inline void sweep_ring_pipe(const BatchSweep& sw, const BatchSys& S, const int32_t* lp, int64_t nlev,
+                                       int r0, int jj, int R, bool run, const double* in, double* out, At at,
+                                       lds_double* ring, int ring_pos, const double* dv, double& dot, double* dump)
+{
+    constexpr int D = W <= 4 ? 4 : 3;
+    const int32_t* row = sw.row + S.row0;
+    const int32_t* ci = sw.ci + S.row0 * W;
+    const double* val = sw.val + S.row0 * W;
+    const double* diag = sw.diag + S.row0;
+    const int mask = ring_pos - 1;
+    auto meta = [&](int32_t q) {
+        SweepMeta<W> m;
+        m.q = q;
+        m.i = row[q];
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            m.c[k] = ci[int64_t(q) * W + k];
+            m.v[k] = val[int64_t(q) * W + k];
+        }
+        m.d = diag[q];
+        return m;
+    };
+    // this thread's pair of level l, or position 0 (not used); the level
+    // bounds are uniform (clamped, so read unconditionally)
+    auto first = [&](int64_t l, bool& ok) -> int32_t {
+        const int32_t b0 = lp[l < nlev ? l : nlev], b1 = lp[l < nlev ? l + 1 : nlev];
+        ok = run & (b0 + r0 < b1);
+        return ok ? b0 + r0 : 0;
+    };
+    const int dump_slot = ring_pos, zero_slot = ring_pos + 1;
+    SweepMeta<W> m[D];
+    bool ok[D];
+    double x[D], y[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        m[k] = meta(first(k, ok[k]));
+        x[k] = 0.0;
+        y[k] = 0.0;
+    }
+    x[0] = in[at(m[0].i)];
+    x[1] = in[at(m[1].i)];
+    if (DOT) {
+        y[0] = dv[at(m[0].i)];
+        y[1] = dv[at(m[1].i)];
+    }
+    for (int64_t l0 = 0; l0 < nlev; l0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; u++) {
+            const int64_t l = l0 + u;
+            if (l < nlev) {
+                const int u2 = (u + 2) % D;   // a constant once unrolled
+                x[u2] = in[at(m[u2].i)];     // right-hand side of level l + 2
+                if (DOT) y[u2] = dv[at(m[u2].i)];
+                // padding entries (c = -1, value 0) read the zero slot:
+                // acc - 0 * 0 == acc exactly, so every entry is applied
+                // without a branch; an unused pair writes the dump slots
+                double acc = x[u];
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    const int32_t c = m[u].c[k];
+                    const double yv = ring[(c >= 0 ? (c & mask) : zero_slot) * R + jj];
+                    acc -= m[u].v[k] * yv;
+                }
+                const double res = acc * m[u].d;
+                ring[(ok[u] ? (m[u].q & mask) : dump_slot) * R + jj] = res;
+                *(ok[u] ? &out[at(m[u].i)] : &dump[threadIdx.x]) = res;
+                if (DOT) dot = ok[u] ? dot + y[u] * res : dot;
+                m[u] = meta(first(l + D, ok[u]));   // level l + D
+                lds_barrier();
+            }
+        }
+    }
+}
+
+// General form: any level width, dependencies beyond the ring read from
+// `out` behind full barriers; loads are issued in the level that uses them.
+template <int W, bool DOT, typename At>
+__device__ inline void sweep_ring_general(const BatchSweep& sw, const BatchSys& S, const int32_t* lp, int64_t nlev,
+                                          int r0, int rstep, int jj, int R, bool run, const double* in, double* out,
+                                          At at, lds_double* ring, int ring_pos, const double* dv, double& dot)
+{
+    const int32_t* row = sw.row + S.row0;
+    const int32_t* ci = sw.ci + S.row0 * W;
+    const double* val = sw.val + S.row0 * W;
+    const double* diag = sw.diag + S.row0;
+    const int mask = ring_pos - 1;
     for (int64_t l = 0; l < nlev; l++) {
+        const int32_t lend = lp[l + 1];
         if (run)
-            for (int32_t q = lp[l] + r0; q < lp[l + 1]; q += rstep) {
+            for (int32_t q = lp[l] + r0; q < lend; q += rstep) {
                 const int32_t i = row[q];
                 double acc = in[at(i)];
-                for (int k = 0; k < w.w; k++) {
-                    const int32_t c = ci[int64_t(q) * w.w + k];
-                    if (c >= 0) acc -= val[int64_t(q) * w.w + k] * out[at(c)];
+                for (int k = 0; k < W; k++) {
+                    const int32_t c = ci[int64_t(q) * W + k];
+                    if (c >= 0) {
+                        const double yv = c >= lend - ring_pos ? ring[(c & mask) * R + jj] : out[at(row[c])];
+                        acc -= val[int64_t(q) * W + k] * yv;
+                    }
                 }
-                out[at(i)] = acc * diag[q];
+                const double res = acc * diag[q];
+                ring[(q & mask) * R + jj] = res;
+                out[at(i)] = res;
+                if (DOT) dot += dv[at(i)] * res;
             }
         __syncthreads();
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a)
+template <int NT, int W, bool DOT, typename At>
+__device__ inline void sweep_ring(const BatchSweep& sw, const BatchSys& S, const int32_t* lp, int64_t nlev,
+                                  int64_t reach, int r0, int rstep, int jj, int R, bool run, const double* in,
+                                  double* out, At at, lds_double* ring, int ring_pos, const double* dv, double& dot,
+                                  double* dump)
 {
+    if (reach <= ring_pos && S.level_width * R <= NT)
+        sweep_ring_pipe<W, DOT>(sw, S, lp, nlev, r0, jj, R, run, in, out, at, ring, ring_pos, dv, dot, dump);
+    else
+        sweep_ring_general<W, DOT>(sw, S, lp, nlev, r0, rstep, jj, R, run, in, out, at, ring, ring_pos, dv, dot);
+}
+
+// Row loops of the vector kernels: each thread takes rows r0, r0 + rstep, ...
+// in groups of kRowsInFlight so that the loads of a group are issued together.
+constexpr int kRowsInFlight = 2;
+
+template <int NT, int W>
+// lv_ptr is read-only for the whole launch (a separate __restrict__ argument
+// so its uniform reads become scalar loads that never wait on vector memory)
+__global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a, const int32_t* __restrict__ lv_ptr)
+{
+    constexpr int E = 2 * W + 1;   // entries per row of A
+    extern __shared__ double ring[];
     __shared__ double red[NT];
     __shared__ double norm_r0[kBatchMaxR], rho[kBatchMaxR], rho_1[kBatchMaxR], alpha[kBatchMaxR], beta[kBatchMaxR];
     __shared__ double resid[kBatchMaxR];
@@ -197,7 +341,8 @@ __global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a)
     for (;;) {
         if (tid == 0) item_sh = int(atomicAdd(a.queue, 1u));
         __syncthreads();
-        const int item = item_sh;
+        // uniform from here on: item, system and level pointers live in scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(item_sh);
         __syncthreads();
         if (item >= a.nitems) break;
         const int4 it = a.items[item];
@@ -209,21 +354,67 @@ __global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a)
         const int n = int(S.n);
         const int rhs = it.y + jj;          // global right-hand side of this thread
         const int64_t base = S.row0 * a.nrhs;
+        const int ring_pos = a.ring_doubles / R;
+        lds_double* ring3 = (lds_double*)ring;
+        // two slots after the ring: dump (writes of unused pairs), zero (padding reads)
+        if (tid < 2 * kBatchMaxR) ring3[a.ring_doubles + tid] = 0.0;
+        double* dump = a.dump + int64_t(blockIdx.x) * NT;
         auto at = [&](int i) { return base + int64_t(i) * a.nrhs + rhs; };
-        const int32_t* ast = a.a_st + S.a_st;
-        const int32_t* aci = a.a_ci + S.a_nz;
-        const double* av = a.a_v + S.a_nz;
-        const int32_t* lpl = a.lv_ptr + S.lvl_l;                // level pointers of the two sweeps
-        const int32_t* lpu = a.lv_ptr + S.lvl_u;
+        const int32_t* aci = a.a_ci + S.row0 * E;
+        const double* av = a.a_v + S.row0 * E;
+        const int32_t* lpl = lv_ptr + S.lvl_l;                  // level pointers of the two sweeps
+        const int32_t* lpu = lv_ptr + S.lvl_u;
+        // (A v)_i for the rows of a group, entries in stored order (mat_vec_mult.hpp:662-687)
+        // (rows loaded together: as many as keep the registers within bounds)
+        auto spmv_rows = [&](const double* v, const int* ii, double* out) {
+            constexpr int RS = E <= 5 ? 2 : 1;
+#pragma unroll
+            for (int u0 = 0; u0 < kRowsInFlight; u0 += RS) {
+                int32_t c[RS][E];
+                double w[RS][E], xv[RS][E];
+#pragma unroll
+                for (int u = 0; u < RS; u++)
+#pragma unroll
+                    for (int k = 0; k < E; k++) {
+                        const int i = ii[u0 + u] < n ? ii[u0 + u] : n - 1;
+                        c[u][k] = aci[int64_t(i) * E + k];
+                        w[u][k] = av[int64_t(i) * E + k];
+                    }
+#pragma unroll
+                for (int u = 0; u < RS; u++)
+#pragma unroll
+                    for (int k = 0; k < E; k++) xv[u][k] = v[at(c[u][k] >= 0 ? c[u][k] : 0)];
+#pragma unroll
+                for (int u = 0; u < RS; u++) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int k = 0; k < E; k++)
+                        if (c[u][k] >= 0) sum += w[u][k] * xv[u][k];
+                    out[u0 + u] = sum;
+                }
+            }
+        };
 
-        // r = b - A x (compute_summand.hpp:42-60) and ||b|| (basic_iteration ctor)
-        double bb = 0.0;
-        for (int i = r0; i < n; i += rstep) {
-            double sum = 0.0;
-            for (int32_t e = ast[i]; e < ast[i + 1]; e++) sum += av[e] * a.x[base + int64_t(aci[e]) * a.nrhs + rhs];
-            const double bi = a.b[at(i)];
-            a.r[at(i)] = bi - sum;
-            bb += bi * bi;
+        // r = b - A x (compute_summand.hpp:42-60) and ||b|| (basic_iteration ctor);
+        // rr: this thread's part of |r|^2 for the next convergence test
+        double bb = 0.0, rr = 0.0;
+        for (int i0 = r0; i0 < n; i0 += kRowsInFlight * rstep) {
+            int ii[kRowsInFlight];
+            double ax[kRowsInFlight], bv[kRowsInFlight];
+#pragma unroll
+            for (int u = 0; u < kRowsInFlight; u++) {
+                ii[u] = i0 + u * rstep;
+                bv[u] = a.b[at(ii[u] < n ? ii[u] : n - 1)];
+            }
+            spmv_rows(a.x, ii, ax);
+#pragma unroll
+            for (int u = 0; u < kRowsInFlight; u++)
+                if (ii[u] < n) {
+                    const double ri = bv[u] - ax[u];
+                    a.r[at(ii[u])] = ri;
+                    bb += bv[u] * bv[u];
+                    rr += ri * ri;
+                }
         }
         rhs_sums<NT>(bb, red, R);
         if (tid < R) {
@@ -234,12 +425,8 @@ __global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a)
         }
         __syncthreads();
         for (;;) {
-            // ---- while (!iter.finished(r)) { ++iter; ...  (two_norm(r)) ----
-            double rr = 0.0;
-            for (int i = r0; i < n; i += rstep) {
-                const double ri = a.r[at(i)];
-                rr += ri * ri;
-            }
+            // ---- while (!iter.finished(r)) { ++iter; ...  (two_norm(r), summed
+            //      when r was written) ----
             rhs_sums<NT>(rr, red, R);
             if (tid == 0) {
                 int active = 0;
@@ -248,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a)
                     const double res = sqrt(red[j]);
                     resid[j] = res;
                     const bool conv = norm_r0[j] == 0 ? res <= a.atol : (res <= a.rtol * norm_r0[j] || res <= a.atol);
-                    if (conv || iters[j] >= a.max_iter) {
+                    if (conv || iters[j] >= (a.sys_max_iter ? a.sys_max_iter[s] : a.max_iter)) {
                         state[j] = 1;
                         const int idx = s * a.nrhs + it.y + j;
                         a.out_iters[idx] = iters[j];
@@ -267,44 +454,81 @@ __global__ __launch_bounds__(NT) void k_batch_cg(BatchCgArgs a)
             // ---- z = solve(P, r): lower sweep on L = trans(U) (lower_trisolve.hpp:191-209):
             //      t_i = (r_i - sum_k L_ik t_k) * L_ii, k ascending; then the upper
             //      sweep on U (upper_trisolve.hpp:159-176): z_i = (t_i - sum_j U_ij z_j) * U_ii
-            sweep_levels(a.lo, S, lpl, S.nlev_l, r0, rstep, run, a.r, a.t, at);
-            sweep_levels(a.up, S, lpu, S.nlev_u, r0, rstep, run, a.t, a.z, at);
-            // ---- rho = dot(r, z) ----
+            //      rho = dot(r, z) is summed as z is produced
             double rz = 0.0;
-            if (run)
-                for (int i = r0; i < n; i += rstep) rz += a.r[at(i)] * a.z[at(i)];
+            sweep_ring<NT, W, false>(a.lo, S, lpl, S.nlev_l, S.reach_l, r0, rstep, jj, R, run, a.r, a.t, at, ring3,
+                                     ring_pos, nullptr, rz, dump);
+            __syncthreads();   // t complete in global memory; the ring is reused
+            sweep_ring<NT, W, true>(a.up, S, lpu, S.nlev_u, S.reach_u, r0, rstep, jj, R, run, a.t, a.z, at, ring3,
+                                    ring_pos, a.r, rz, dump);
+            // ---- p = z (first) or p = z + beta p ----
             rhs_sums<NT>(rz, red, R);
             if (tid < R) {
                 rho[tid] = red[tid];
                 beta[tid] = iters[tid] <= 1 ? 0.0 : red[tid] / rho_1[tid];
             }
             __syncthreads();
-            // ---- p = z (first) or p = z + beta p ----
             if (run) {
                 const bool first = iters[jj] <= 1;
                 const double bj = beta[jj];
-                for (int i = r0; i < n; i += rstep)
-                    a.p[at(i)] = first ? a.z[at(i)] : a.z[at(i)] + bj * a.p[at(i)];
+                for (int i0 = r0; i0 < n; i0 += kRowsInFlight * rstep) {
+                    double zv[kRowsInFlight], pv[kRowsInFlight];
+#pragma unroll
+                    for (int u = 0; u < kRowsInFlight; u++) {
+                        const int i = i0 + u * rstep < n ? i0 + u * rstep : n - 1;
+                        zv[u] = a.z[at(i)];
+                        pv[u] = a.p[at(i)];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kRowsInFlight; u++)
+                        if (i0 + u * rstep < n) a.p[at(i0 + u * rstep)] = first ? zv[u] : zv[u] + bj * pv[u];
+                }
             }
             __syncthreads();
             // ---- q = A p, alpha = rho / dot(p, q) ----
             double pq = 0.0;
             if (run)
-                for (int i = r0; i < n; i += rstep) {
-                    double sum = 0.0;
-                    for (int32_t e = ast[i]; e < ast[i + 1]; e++) sum += av[e] * a.p[at(aci[e])];
-                    a.q[at(i)] = sum;
-                    pq += a.p[at(i)] * sum;
+                for (int i0 = r0; i0 < n; i0 += kRowsInFlight * rstep) {
+                    int ii[kRowsInFlight];
+                    double qv[kRowsInFlight], pv[kRowsInFlight];
+#pragma unroll
+                    for (int u = 0; u < kRowsInFlight; u++) {
+                        ii[u] = i0 + u * rstep;
+                        pv[u] = a.p[at(ii[u] < n ? ii[u] : n - 1)];
+                    }
+                    spmv_rows(a.p, ii, qv);
+#pragma unroll
+                    for (int u = 0; u < kRowsInFlight; u++)
+                        if (ii[u] < n) {
+                            a.q[at(ii[u])] = qv[u];
+                            pq += pv[u] * qv[u];
+                        }
                 }
             rhs_sums<NT>(pq, red, R);
             if (tid < R) alpha[tid] = rho[tid] / red[tid];
             __syncthreads();
-            // ---- x += alpha p; r -= alpha q; rho_1 = rho ----
+            // ---- x += alpha p; r -= alpha q (and |r|^2); rho_1 = rho ----
+            rr = 0.0;
             if (run) {
                 const double al = alpha[jj];
-                for (int i = r0; i < n; i += rstep) {
-                    a.x[at(i)] += al * a.p[at(i)];
-                    a.r[at(i)] -= al * a.q[at(i)];
+                for (int i0 = r0; i0 < n; i0 += kRowsInFlight * rstep) {
+                    double xv[kRowsInFlight], pv[kRowsInFlight], rv[kRowsInFlight], qv[kRowsInFlight];
+#pragma unroll
+                    for (int u = 0; u < kRowsInFlight; u++) {
+                        const int i = i0 + u * rstep < n ? i0 + u * rstep : n - 1;
+                        xv[u] = a.x[at(i)];
+                        pv[u] = a.p[at(i)];
+                        rv[u] = a.r[at(i)];
+                        qv[u] = a.q[at(i)];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kRowsInFlight; u++)
+                        if (i0 + u * rstep < n) {
+                            const double ri = rv[u] - al * qv[u];
+                            a.x[at(i0 + u * rstep)] = xv[u] + al * pv[u];
+                            a.r[at(i0 + u * rstep)] = ri;
+                            rr += ri * ri;
+                        }
                 }
             }
             if (tid < R) rho_1[tid] = rho[tid];
@@ -346,13 +570,32 @@ static void levels_of(int64_t n, const std::vector<int32_t>& st, const std::vect
     }
 }
 
+// LDS ring of the sweeps per workgroup size: with the kernel's static LDS,
+// 2048 threads' worth of workgroups fit in a CU's 160 KB
+static int ring_doubles_for(int NT)
+{
+    return NT == 1024 ? 8192 : NT == 512 ? 4096 : 2048;
+}
+
+template <int NT>
+static void launch_batch_cg_w(int W, const BatchCgArgs& a, int grid, hipStream_t s)
+{
+    const size_t lds = size_t(a.ring_doubles + 2 * kBatchMaxR) * sizeof(double);   // ring + dump and zero slots
+    switch (W) {
+    case 2: hipLaunchKernelGGL((k_batch_cg<NT, 2>), dim3(grid), dim3(NT), lds, s, a, a.lv_ptr); break;
+    case 4: hipLaunchKernelGGL((k_batch_cg<NT, 4>), dim3(grid), dim3(NT), lds, s, a, a.lv_ptr); break;
+    case 8: hipLaunchKernelGGL((k_batch_cg<NT, 8>), dim3(grid), dim3(NT), lds, s, a, a.lv_ptr); break;
+    default: hipLaunchKernelGGL((k_batch_cg<NT, 16>), dim3(grid), dim3(NT), lds, s, a, a.lv_ptr); break;
+    }
+}
+
 // Launches the persistent CG with block size NT on the given items.
 static void launch_batch_cg(m4g_batch_s& B, int NT, const BatchCgArgs& a, int grid, hipStream_t s)
 {
     switch (NT) {
-    case 256: hipLaunchKernelGGL(k_batch_cg<256>, dim3(grid), dim3(256), 0, s, a); break;
-    case 512: hipLaunchKernelGGL(k_batch_cg<512>, dim3(grid), dim3(512), 0, s, a); break;
-    case 1024: hipLaunchKernelGGL(k_batch_cg<1024>, dim3(grid), dim3(1024), 0, s, a); break;
+    case 256: launch_batch_cg_w<256>(B.W, a, grid, s); break;
+    case 512: launch_batch_cg_w<512>(B.W, a, grid, s); break;
+    case 1024: launch_batch_cg_w<1024>(B.W, a, grid, s); break;
     default: throw Error(M4G_ERR_INVALID, "batch block size must be 256, 512 or 1024");
     }
     M4G_HIP(hipGetLastError());
@@ -363,15 +606,16 @@ static void launch_batch_cg(m4g_batch_s& B, int NT, const BatchCgArgs& a, int gr
 // rhs_per_item[s]), largest systems first.
 static void batch_run(m4g_batch_s& B, const std::vector<int>& systems, const int* rhs_per_item, int nrhs, int block,
                       double* x, const double* b, int max_iter, double rtol, double atol, int* out_it,
-                      double* out_res, int* out_err, hipStream_t s)
+                      double* out_res, int* out_err, hipStream_t s, const int* sys_max_iter = nullptr)
 {
     std::vector<int> order(systems);
     std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return B.h_sys[p].n > B.h_sys[q].n; });
     std::vector<int4> items;
     for (int sidx : order) {
         const int want = std::max(1, std::min(rhs_per_item ? rhs_per_item[sidx] : kBatchMaxR, kBatchMaxR));
+        // at most as many rhs as keep a sweep level within one pass of the block
         int R = 1;
-        while (R * 2 <= want && nrhs % (R * 2) == 0) R *= 2;
+        while (R * 2 <= want && nrhs % (R * 2) == 0 && R * 2 * B.h_sys[sidx].level_width <= block) R *= 2;
         for (int j = 0; j < nrhs; j += R) items.push_back(make_int4(sidx, j, R, 0));
     }
     if (items.empty()) return;
@@ -382,20 +626,13 @@ static void batch_run(m4g_batch_s& B, const std::vector<int>& systems, const int
     DevBuf<unsigned> queue(1);
     M4G_HIP(hipMemsetAsync(queue.ptr, 0, sizeof(unsigned), s));
     BatchCgArgs a;
-    a.lo = BatchSweep{B.lq_row.ptr, B.lq_ci.ptr, B.lq_val.ptr, B.lq_diag.ptr, B.wl};
-    a.up = BatchSweep{B.uq_row.ptr, B.uq_ci.ptr, B.uq_val.ptr, B.uq_diag.ptr, B.wu};
+    a.lo = BatchSweep{B.lq_row.ptr, B.lq_ci.ptr, B.lq_val.ptr, B.lq_diag.ptr, B.W};
+    a.up = BatchSweep{B.uq_row.ptr, B.uq_ci.ptr, B.uq_val.ptr, B.uq_diag.ptr, B.W};
     a.sys = B.sys.ptr;
-    a.a_st = B.a_st.ptr;
-    a.a_ci = B.a_ci.ptr;
-    a.u_st = B.u_st.ptr;
-    a.u_ci = B.u_ci.ptr;
-    a.l_st = B.l_st.ptr;
-    a.l_ci = B.l_ci.ptr;
-    a.l_up = B.l_up.ptr;
+    a.a_ci = B.a_eci.ptr;
+    a.a_v = B.a_ev.ptr;
     a.lv_ptr = B.lv_ptr.ptr;
-    a.lv_rows = B.lv_rows.ptr;
-    a.a_v = B.a_v.ptr;
-    a.u_v = B.u_v.ptr;
+    a.ring_doubles = ring_doubles_for(block);
     a.items = d_items.ptr;
     a.nitems = int32_t(items.size());
     a.nrhs = nrhs;
@@ -408,6 +645,7 @@ static void batch_run(m4g_batch_s& B, const std::vector<int>& systems, const int
     a.q = B.work.ptr + 3 * nv;
     a.t = B.work.ptr + 4 * nv;
     a.max_iter = max_iter;
+    a.sys_max_iter = sys_max_iter;
     a.rtol = rtol;
     a.atol = atol;
     a.out_iters = out_it;
@@ -416,6 +654,8 @@ static void batch_run(m4g_batch_s& B, const std::vector<int>& systems, const int
     // persistent grid: as many workgroups as fit at once (at most one per item)
     const int per_cu = std::max(1, 2048 / block);
     const int grid = int(std::min<int64_t>(int64_t(items.size()), int64_t(device_state().num_cus) * per_cu));
+    DevBuf<double> dump(size_t(std::max(1, grid)) * block);
+    a.dump = dump.ptr;
     launch_batch_cg(B, block, a, std::max(1, grid), s);
     M4G_HIP(hipStreamSynchronize(s));   // items and queue are released on return
 }
@@ -528,17 +768,35 @@ int m4g_batch_create(m4g_batch* out, int64_t nsys, const int64_t* sizes, const i
         }
         M4G_REQUIRE(a_nz < (int64_t(1) << 31) && u_nz < (int64_t(1) << 31), M4G_ERR_INVALID,
                     "m4g_batch_create: batch too large");
-        // packed sweeps: rows in level order with fixed-width dependency lists
-        int wl = 1, wu = 1;
+        // packed sweeps: rows in level order with fixed-width dependency
+        // lists (positions in the sweep's level order); A as fixed-width rows
+        int wmax = 1;
         for (const BatchSys& d : sys)
             for (int64_t i = 0; i < d.n; i++) {
-                wl = std::max(wl, l_st[d.l_st + i + 1] - l_st[d.l_st + i]);
-                wu = std::max(wu, u_st[d.u_st + i + 1] - u_st[d.u_st + i] - 1);
+                wmax = std::max(wmax, l_st[d.l_st + i + 1] - l_st[d.l_st + i]);
+                wmax = std::max(wmax, u_st[d.u_st + i + 1] - u_st[d.u_st + i] - 1);
+                wmax = std::max(wmax, (a_st[d.a_st + i + 1] - a_st[d.a_st + i]) / 2);
             }
-        std::vector<int32_t> lq_row(row0), lq_ci(row0 * wl, -1), lq_up(row0 * wl, -1), lq_dp(row0);
-        std::vector<int32_t> uq_row(row0), uq_ci(row0 * wu, -1), uq_up(row0 * wu, -1), uq_dp(row0);
+        int W = 2;
+        while (W < wmax) W *= 2;
+        M4G_REQUIRE(W <= 16, M4G_ERR_INVALID, "m4g_batch_create: rows of more than 33 entries are not supported");
+        const int E = 2 * W + 1;
+        std::vector<int32_t> lq_row(row0), lq_ci(row0 * W, -1), lq_up(row0 * W, -1), lq_dp(row0);
+        std::vector<int32_t> uq_row(row0), uq_ci(row0 * W, -1), uq_up(row0 * W, -1), uq_dp(row0);
+        std::vector<int32_t> a_eci(row0 * E, -1);
+        std::vector<double> a_ev(row0 * E, 0.0);
         std::vector<int64_t> q_ubase(row0);
-        for (const BatchSys& d : sys)
+        for (BatchSys& d : sys) {
+            std::vector<int32_t> posl(d.n), posu(d.n);
+            for (int64_t q = 0; q < d.n; q++) {
+                posl[lv_rows[d.rows_l + q]] = int32_t(q);
+                posu[lv_rows[d.rows_u + q]] = int32_t(q);
+            }
+            for (int64_t i = 0; i < d.n; i++)
+                for (int32_t e = a_st[d.a_st + i], k = 0; e < a_st[d.a_st + i + 1]; e++, k++) {
+                    a_eci[(d.row0 + i) * E + k] = a_ci[d.a_nz + e];
+                    a_ev[(d.row0 + i) * E + k] = a_v[d.a_nz + e];
+                }
             for (int64_t q = 0; q < d.n; q++) {
                 const int64_t g = d.row0 + q;
                 q_ubase[g] = d.u_nz;
@@ -547,18 +805,40 @@ int m4g_batch_create(m4g_batch* out, int64_t nsys, const int64_t* sizes, const i
                 lq_row[g] = i;
                 lq_dp[g] = u_st[d.u_st + i];
                 for (int32_t e = l_st[d.l_st + i], k = 0; e < l_st[d.l_st + i + 1]; e++, k++) {
-                    lq_ci[g * wl + k] = l_ci[d.l_nz + e];
-                    lq_up[g * wl + k] = l_up[d.l_nz + e];
+                    lq_ci[g * W + k] = posl[l_ci[d.l_nz + e]];
+                    lq_up[g * W + k] = l_up[d.l_nz + e];
                 }
                 // upper sweep: U row j after its diagonal, ascending columns
                 const int32_t j = lv_rows[d.rows_u + q];
                 uq_row[g] = j;
                 uq_dp[g] = u_st[d.u_st + j];
                 for (int32_t e = u_st[d.u_st + j] + 1, k = 0; e < u_st[d.u_st + j + 1]; e++, k++) {
-                    uq_ci[g * wu + k] = u_ci[d.u_nz + e];
-                    uq_up[g * wu + k] = e;
+                    uq_ci[g * W + k] = posu[u_ci[d.u_nz + e]];
+                    uq_up[g * W + k] = e;
                 }
             }
+            // reach of each sweep: how far before the end of its level a
+            // dependency may lie (decides whether the LDS ring holds it)
+            auto reach = [&](int64_t lvl, int64_t nlev, const std::vector<int32_t>& qci) {
+                int64_t rmax = 0;
+                for (int64_t l = 0; l < nlev; l++) {
+                    const int32_t lb = lv_ptr[lvl + l], le = lv_ptr[lvl + l + 1];
+                    for (int32_t q = lb; q < le; q++)
+                        for (int k = 0; k < W; k++) {
+                            const int32_t c = qci[(d.row0 + q) * W + k];
+                            if (c >= 0) rmax = std::max<int64_t>(rmax, le - c);
+                        }
+                }
+                return rmax;
+            };
+            d.reach_l = reach(d.lvl_l, d.nlev_l, lq_ci);
+            d.reach_u = reach(d.lvl_u, d.nlev_u, uq_ci);
+            d.level_width = 1;
+            for (int64_t l = 0; l < d.nlev_l; l++)
+                d.level_width = std::max<int64_t>(d.level_width, lv_ptr[d.lvl_l + l + 1] - lv_ptr[d.lvl_l + l]);
+            for (int64_t l = 0; l < d.nlev_u; l++)
+                d.level_width = std::max<int64_t>(d.level_width, lv_ptr[d.lvl_u + l + 1] - lv_ptr[d.lvl_u + l]);
+        }
         B->n_total = row0;
         B->nnz_total = a_nz;
         B->u_total = u_nz;
@@ -583,8 +863,10 @@ int m4g_batch_create(m4g_batch* out, int64_t nsys, const int64_t* sizes, const i
         B->a_v.reset(std::max<size_t>(a_v.size(), 1));
         B->a_v.upload(a_v.data(), a_v.size());
         B->u_v.reset(std::max<int64_t>(u_nz, 1));
-        B->wl = wl;
-        B->wu = wu;
+        B->W = W;
+        up32(B->a_eci, a_eci);
+        B->a_ev.reset(std::max<size_t>(a_ev.size(), 1));
+        B->a_ev.upload(a_ev.data(), a_ev.size());
         up32(B->lq_row, lq_row);
         up32(B->lq_ci, lq_ci);
         up32(B->lq_up, lq_up);
@@ -595,9 +877,9 @@ int m4g_batch_create(m4g_batch* out, int64_t nsys, const int64_t* sizes, const i
         up32(B->uq_dp, uq_dp);
         B->q_ubase.reset(std::max<int64_t>(row0, 1));
         B->q_ubase.upload(q_ubase.data(), q_ubase.size());
-        B->lq_val.reset(std::max<int64_t>(row0 * wl, 1));
+        B->lq_val.reset(std::max<int64_t>(row0 * W, 1));
         B->lq_diag.reset(std::max<int64_t>(row0, 1));
-        B->uq_val.reset(std::max<int64_t>(row0 * wu, 1));
+        B->uq_val.reset(std::max<int64_t>(row0 * W, 1));
         B->uq_diag.reset(std::max<int64_t>(row0, 1));
         M4G_HIP(hipStreamSynchronize(current_stream()));
         *out = B.release();
@@ -637,9 +919,9 @@ int m4g_batch_factor_ic0(m4g_batch B)
                            B->l_up.ptr, B->u_st.ptr, B->lv_ptr.ptr, B->lv_rows.ptr, B->u_v.ptr);
         // factor values into the packed sweeps
         const int pg = int(std::min<int64_t>((B->n_total + 255) / 256, 16384));
-        hipLaunchKernelGGL(k_batch_pack, dim3(pg), dim3(256), 0, s, B->n_total, B->wl, B->lq_up.ptr, B->lq_dp.ptr,
+        hipLaunchKernelGGL(k_batch_pack, dim3(pg), dim3(256), 0, s, B->n_total, B->W, B->lq_up.ptr, B->lq_dp.ptr,
                            B->q_ubase.ptr, B->u_v.ptr, B->lq_val.ptr, B->lq_diag.ptr);
-        hipLaunchKernelGGL(k_batch_pack, dim3(pg), dim3(256), 0, s, B->n_total, B->wu, B->uq_up.ptr, B->uq_dp.ptr,
+        hipLaunchKernelGGL(k_batch_pack, dim3(pg), dim3(256), 0, s, B->n_total, B->W, B->uq_up.ptr, B->uq_dp.ptr,
                            B->q_ubase.ptr, B->u_v.ptr, B->uq_val.ptr, B->uq_diag.ptr);
         M4G_HIP(hipGetLastError());
         M4G_HIP(hipStreamSynchronize(s));
@@ -689,7 +971,7 @@ int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int
         M4G_REQUIRE(B && b && rhs_per_item && block, M4G_ERR_INVALID, "m4g_batch_autotune: null argument");
         M4G_REQUIRE(B->factored, M4G_ERR_INVALID, "m4g_batch_autotune: call m4g_batch_factor_ic0 first");
         M4G_REQUIRE(b->type == M4G_F64 && b->n == B->n_total * nrhs, M4G_ERR_SIZE, "m4g_batch_autotune: b size");
-        if (trial_iters <= 0) trial_iters = 20;
+        if (trial_iters <= 0) trial_iters = 40;
         hipStream_t s = current_stream();
         // size classes: thirds of the systems by size
         std::vector<int> order(B->nsys);
@@ -707,6 +989,17 @@ int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int
         DevBuf<double> d_res(size_t(B->nsys) * nrhs);
         std::vector<int> all(B->nsys);
         std::iota(all.begin(), all.end(), 0);
+        // trials run each system for trial_iters * sqrt(n / n_max) iterations:
+        // IC(0)-CG on a 2D problem needs iterations in proportion to its side,
+        // so a trial keeps the real run's shape (long largest systems, short
+        // small ones) at a fraction of its cost
+        int64_t nmax = 1;
+        for (const BatchSys& d : B->h_sys) nmax = std::max(nmax, d.n);
+        std::vector<int> caps(B->nsys);
+        for (int64_t k = 0; k < B->nsys; k++)
+            caps[k] = std::max(2, int(std::ceil(trial_iters * std::sqrt(double(B->h_sys[k].n) / double(nmax)))));
+        DevBuf<int> d_caps(caps.size());
+        d_caps.upload(caps.data(), caps.size());
         hipEvent_t e0, e1;
         M4G_HIP(hipEventCreate(&e0));
         M4G_HIP(hipEventCreate(&e1));
@@ -715,7 +1008,7 @@ int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int
             M4G_HIP(hipMemsetAsync(xs.ptr, 0, xs.count * sizeof(double), s));
             M4G_HIP(hipEventRecord(e0, s));
             batch_run(*B, all, rpi.data(), nrhs, blk, xs.ptr, static_cast<const double*>(b->ptr), trial_iters, 0.0,
-                      0.0, d_it.ptr, d_res.ptr, d_err.ptr, s);
+                      0.0, d_it.ptr, d_res.ptr, d_err.ptr, s, d_caps.ptr);
             M4G_HIP(hipEventRecord(e1, s));
             M4G_HIP(hipEventSynchronize(e1));
             float ms = 0;

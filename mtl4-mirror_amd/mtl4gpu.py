@@ -481,7 +481,7 @@ class Batch:
         _check(_f["m4g_batch_download_factor"](self.h, _ptr(out)), "m4g_batch_download_factor")
         return out
 
-    def autotune(self, nrhs, b, trial_iters=20):
+    def autotune(self, nrhs, b, trial_iters=0):
         """Returns (rhs_per_item per system, block size, trial times in ms)."""
         rpi = np.zeros(len(self.sizes), np.int32)
         blk = ctypes.c_int()
