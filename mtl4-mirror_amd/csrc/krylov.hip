@@ -243,6 +243,26 @@ __global__ __launch_bounds__(kSpmvNT) void k_cg_spmv_dot(int64_t n, int64_t nsli
     store_partials<kSpmvNT, 1>(acc, partials);
 }
 
+// The same with the pipelined slice loop (all-narrow matrices): per-thread
+// partials summed over the wavefront's slices.
+template <typename T, int MAXW>
+__global__ __launch_bounds__(kSpmvNT) void k_cg_spmv_pipe(int64_t n, int64_t nslices, int64_t padded,
+                                                           const int64_t* __restrict__ slice_ptr,
+                                                           const uint8_t* __restrict__ idx, const T* __restrict__ vals,
+                                                           const T* __restrict__ p, T* __restrict__ q, KState* st,
+                                                           double* partials)
+{
+    if (st->done) return;
+    Sums<1> acc{{0}};
+    sell_pipe_run<T, MAXW>(
+        n, nslices, padded, slice_ptr, idx, vals, p, [&](int64_t rr) { return p[rr]; },
+        [&](int64_t row, T qv, T pr) {
+            q[row] = qv;
+            acc.v[0] += double(pr * qv);
+        });
+    store_partials<kSpmvNT, 1>(acc, partials);
+}
+
 // x += alpha*p; rho_1 = rho; (lazy(r) -= alpha*q) || (lazy(rr) = lazy_unary_dot(r)); then the
 // loop test finished(sqrt|rr|) (cg.hpp:107-109, 95-96)
 template <typename T>
@@ -346,6 +366,26 @@ __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_gamma(int64_t n, int64_t 
     store_partials<kSpmvNT, 1>(acc, partials);
 }
 
+template <typename T, int MAXW>
+__global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_gamma_pipe(int64_t n, int64_t nslices, int64_t padded,
+                                                                   const int64_t* __restrict__ slice_ptr,
+                                                                   const uint8_t* __restrict__ idx,
+                                                                   const T* __restrict__ vals,
+                                                                   const T* __restrict__ phat,
+                                                                   const T* __restrict__ rtilde, T* __restrict__ v,
+                                                                   KState* st, double* partials)
+{
+    if (st->done) return;
+    Sums<1> acc{{0}};
+    sell_pipe_run<T, MAXW>(
+        n, nslices, padded, slice_ptr, idx, vals, phat, [&](int64_t rr) { return rtilde[rr]; },
+        [&](int64_t row, T vv, T rt) {
+            v[row] = vv;
+            acc.v[0] += double(rt * vv);
+        });
+    store_partials<kSpmvNT, 1>(acc, partials);
+}
+
 // s = r - alpha*v; finished(two_norm(s)) -> early exit flag (:62-67)
 template <typename T>
 __global__ __launch_bounds__(kNT) void k_bicg_s(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
@@ -409,6 +449,27 @@ __global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_omega(int64_t n, int64_t 
     store_partials<kSpmvNT, 2>(acc, partials);
 }
 
+template <typename T, int MAXW>
+__global__ __launch_bounds__(kSpmvNT) void k_bicg_spmv_omega_pipe(int64_t n, int64_t nslices, int64_t padded,
+                                                                   const int64_t* __restrict__ slice_ptr,
+                                                                   const uint8_t* __restrict__ idx,
+                                                                   const T* __restrict__ vals,
+                                                                   const T* __restrict__ shat,
+                                                                   const T* __restrict__ s_, T* __restrict__ t_,
+                                                                   KState* st, double* partials)
+{
+    if (st->done) return;
+    Sums<2> acc{{0, 0}};
+    sell_pipe_run<T, MAXW>(
+        n, nslices, padded, slice_ptr, idx, vals, shat, [&](int64_t rr) { return s_[rr]; },
+        [&](int64_t row, T tv, T sv) {
+            t_[row] = tv;
+            acc.v[0] += double(tv * sv);
+            acc.v[1] += double(tv * tv);
+        });
+    store_partials<kSpmvNT, 2>(acc, partials);
+}
+
 // x += omega*shat + alpha*phat; r = s - omega*t; rho_2 = rho_1; loop top (:72-75, 43-53)
 template <typename T>
 __global__ __launch_bounds__(kNT) void k_bicg_update(int64_t n, const T* __restrict__ shat, const T* __restrict__ phat,
@@ -456,6 +517,20 @@ KrylovWs::~KrylovWs()
 {
     if (host_state) (void)hipHostFree(host_state);
 }
+
+// launches KERNEL<T, MAXW> for the pipelined SpMV's width mw
+#define M4G_PIPE_LAUNCH(KERNEL, T, mw, ...)                                                                      \
+    do {                                                                                                       \
+        switch (mw) {                                                                                          \
+        case 3: hipLaunchKernelGGL((KERNEL<T, 3>), __VA_ARGS__); break;                                       \
+        case 4: hipLaunchKernelGGL((KERNEL<T, 4>), __VA_ARGS__); break;                                       \
+        case 5: hipLaunchKernelGGL((KERNEL<T, 5>), __VA_ARGS__); break;                                       \
+        case 6: hipLaunchKernelGGL((KERNEL<T, 6>), __VA_ARGS__); break;                                       \
+        case 7: hipLaunchKernelGGL((KERNEL<T, 7>), __VA_ARGS__); break;                                       \
+        case 8: hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__); break;                                       \
+        default: hipLaunchKernelGGL((KERNEL<T, 16>), __VA_ARGS__); break;                                     \
+        }                                                                                                      \
+    } while (0)
 
 static void spmv_geom(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
 {
@@ -563,7 +638,9 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
     M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
     int64_t chunk, sgrid;
     spmv_geom(A, chunk, sgrid);
-    ws.prepare(sgrid, it->history ? it->history_capacity : 0);
+    const int pw = sell_pipe_width(A);
+    const int pgrid = sell_pipe_grid();
+    ws.prepare(std::max<int64_t>(sgrid, pgrid), it->history ? it->history_capacity : 0);
     KState h;
     init_state(h, it, ws.history.ptr);
     M4G_HIP(hipMemcpyAsync(ws.state.ptr, &h, sizeof(KState), hipMemcpyHostToDevice, st));
@@ -598,9 +675,16 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
             zz = z.ptr;
         }
         hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
-        hipLaunchKernelGGL(k_cg_spmv_dot<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
-                           A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)p.ptr, q.ptr, S, part, cnt);
-        FIN(1, FIN_CG_ALPHA, sgrid);
+        if (pw) {
+            M4G_PIPE_LAUNCH(k_cg_spmv_pipe, T, pw, dim3(pgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices,
+                            A.sell.padded, A.sell.slice_ptr.ptr, A.sell.idx.ptr, vals, (const T*)p.ptr, q.ptr, S, part);
+            FIN(1, FIN_CG_ALPHA, pgrid);
+        } else {
+            hipLaunchKernelGGL(k_cg_spmv_dot<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
+                               A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)p.ptr,
+                               q.ptr, S, part, cnt);
+            FIN(1, FIN_CG_ALPHA, sgrid);
+        }
         hipLaunchKernelGGL(k_cg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr, x, r.ptr,
                            S, int(identity), part, cnt);
         FIN(1, FIN_CG_UPDATE, rg);
@@ -623,7 +707,9 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
     M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
     int64_t chunk, sgrid;
     spmv_geom(A, chunk, sgrid);
-    ws.prepare(sgrid, it->history ? it->history_capacity : 0);
+    const int pw = sell_pipe_width(A);
+    const int pgrid = sell_pipe_grid();
+    ws.prepare(std::max<int64_t>(sgrid, pgrid), it->history ? it->history_capacity : 0);
     KState h;
     init_state(h, it, ws.history.ptr);
     M4G_HIP(hipMemcpyAsync(ws.state.ptr, &h, sizeof(KState), hipMemcpyHostToDevice, st));
@@ -649,19 +735,33 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
         hipLaunchKernelGGL(k_bicg_p<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)v.ptr, p.ptr,
                            (const KState*)S);
         apply(p.ptr, phat.ptr);                                      // phat = solve(M, p) (:55)
-        hipLaunchKernelGGL(k_bicg_spmv_gamma<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
-                           A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)phat.ptr, (const T*)rtilde.ptr,
-                           v.ptr, S, part, cnt);
-        FIN(1, FIN_BICG_GAMMA, sgrid);
+        if (pw) {
+            M4G_PIPE_LAUNCH(k_bicg_spmv_gamma_pipe, T, pw, dim3(pgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices,
+                            A.sell.padded, A.sell.slice_ptr.ptr, A.sell.idx.ptr, vals, (const T*)phat.ptr,
+                            (const T*)rtilde.ptr, v.ptr, S, part);
+            FIN(1, FIN_BICG_GAMMA, pgrid);
+        } else {
+            hipLaunchKernelGGL(k_bicg_spmv_gamma<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
+                               A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)phat.ptr,
+                               (const T*)rtilde.ptr, v.ptr, S, part, cnt);
+            FIN(1, FIN_BICG_GAMMA, sgrid);
+        }
         hipLaunchKernelGGL(k_bicg_s<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)v.ptr, s_.ptr, S,
                            part, cnt);
         FIN(1, FIN_BICG_S, rg);
         hipLaunchKernelGGL(k_bicg_exit<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)phat.ptr, x, S, cnt + 1);
         apply(s_.ptr, shat.ptr);                                     // shat = solve(M, s) (:68)
-        hipLaunchKernelGGL(k_bicg_spmv_omega<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
-                           A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)shat.ptr, (const T*)s_.ptr, t_.ptr,
-                           S, part, cnt);
-        FIN(2, FIN_BICG_OMEGA, sgrid);
+        if (pw) {
+            M4G_PIPE_LAUNCH(k_bicg_spmv_omega_pipe, T, pw, dim3(pgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices,
+                            A.sell.padded, A.sell.slice_ptr.ptr, A.sell.idx.ptr, vals, (const T*)shat.ptr,
+                            (const T*)s_.ptr, t_.ptr, S, part);
+            FIN(2, FIN_BICG_OMEGA, pgrid);
+        } else {
+            hipLaunchKernelGGL(k_bicg_spmv_omega<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
+                               A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)shat.ptr,
+                               (const T*)s_.ptr, t_.ptr, S, part, cnt);
+            FIN(2, FIN_BICG_OMEGA, sgrid);
+        }
         hipLaunchKernelGGL(k_bicg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)shat.ptr, (const T*)phat.ptr,
                            (const T*)s_.ptr, (const T*)t_.ptr, (const T*)rtilde.ptr, x, r.ptr, S, part, cnt);
         FIN(2, FIN_BICG_UPDATE, rg);

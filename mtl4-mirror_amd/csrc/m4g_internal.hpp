@@ -193,6 +193,9 @@ void launch_xpay(int64_t n, const double* x, double beta, double* y, hipStream_t
 void launch_axpy_sumsq(int64_t n, double alpha, const double* q, double* r, double* out, hipStream_t s);
 void launch_fill(int64_t n, double value, void* x, int type, hipStream_t s);
 void launch_sum_partials(const double* partials, int nblocks, double* out, hipStream_t s);
+// pipelined SpMV: its MAXW for A (0: not applicable) and its grid
+int sell_pipe_width(const m4g_matrix_s& A);
+int sell_pipe_grid();
 void launch_pattern_gradient(const m4g_matrix_s& A, const double* lambda, const double* x, double* grad,
                              hipStream_t s);
 void launch_gather_values(int64_t count, const int64_t* pos, const double* src, double* dst, hipStream_t s);

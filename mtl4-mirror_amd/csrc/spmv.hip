@@ -73,7 +73,6 @@ __global__ __launch_bounds__(kSpmvNT) void k_sell_to_f32(int64_t padded, const d
         f[d] = float(v[d]);
 }
 
-static int pipe_grid();
 
 void sell_build(m4g_matrix_s& A)
 {
@@ -113,6 +112,12 @@ void sell_build(m4g_matrix_s& A)
     // look-ahead loads of an empty final slice in bounds
     S.vals.reset(S.padded + kSellSlack);
     S.csr_pos.reset(S.padded);
+    // slack after the last slice (read by the pipelined kernel for an empty
+    // final slice): value 0 and padding offsets
+    M4G_HIP(hipMemsetAsync(S.vals.ptr + S.padded, 0, kSellSlack * sizeof(double), current_stream()));
+    if (S.narrow_slices == S.nslices)
+        M4G_HIP(hipMemsetD16Async(reinterpret_cast<hipDeviceptr_t>(S.idx.ptr + 2 * S.padded),
+                                  static_cast<unsigned short>(kSellPad16), kSellSlack, current_stream()));
     S.vals32.release();
     if (S.nslices) {
         int64_t threads = S.nslices * kWave;
@@ -125,7 +130,7 @@ void sell_build(m4g_matrix_s& A)
     // reduction workspace for the fused SpMV + dot
     int64_t groups = (S.nslices + kSliceWaves - 1) / kSliceWaves;
     int64_t chunk = (groups + kXcds - 1) / kXcds;
-    A.red_partials.reset(std::max<int64_t>({int64_t(1), chunk * kXcds, int64_t(pipe_grid())}));
+    A.red_partials.reset(std::max<int64_t>({int64_t(1), chunk * kXcds, int64_t(sell_pipe_grid())}));
     A.red_counter.reset(1);
     M4G_HIP(hipMemsetAsync(A.red_counter.ptr, 0, sizeof(unsigned), current_stream()));
 }
@@ -145,7 +150,10 @@ void sell_build_f32(m4g_matrix_s& A)
 {
     Sell& S = A.sell;
     if (!S.padded) return;
-    if (!S.vals32.ptr) S.vals32.reset(S.padded + kSellSlack);
+    if (!S.vals32.ptr) {
+        S.vals32.reset(S.padded + kSellSlack);
+        M4G_HIP(hipMemsetAsync(S.vals32.ptr + S.padded, 0, kSellSlack * sizeof(float), current_stream()));
+    }
     int64_t g = std::min<int64_t>((S.padded + kSpmvNT - 1) / kSpmvNT, 16384);
     hipLaunchKernelGGL(k_sell_to_f32, dim3(g), dim3(kSpmvNT), 0, current_stream(), S.padded, S.vals.ptr,
                        S.vals32.ptr);
@@ -162,89 +170,40 @@ void sell_build_f32(m4g_matrix_s& A)
 // MODE 0: w = A v; 1: w += A v; 2: w -= A v; 3: w = b - A v.  DOT: per
 // workgroup partial of v.(A v) over its slices.
 template <typename T, int MODE, bool DOT, int MAXW>
-__global__ __launch_bounds__(kSpmvNT) void k_sell_pipe(int64_t n, int64_t nslices, const int64_t* __restrict__ slice_ptr,
-                                                       const int64_t* __restrict__ slice_idx,
+__global__ __launch_bounds__(kSpmvNT) void k_sell_pipe(int64_t n, int64_t nslices, int64_t padded,
+                                                       const int64_t* __restrict__ slice_ptr,
                                                        const uint8_t* __restrict__ idx, const T* __restrict__ vals,
                                                        const T* __restrict__ x, T* __restrict__ w,
                                                        const T* __restrict__ b, double* partials)
 {
     __shared__ double scratch[kSliceWaves];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
-    const int64_t stride = int64_t(gridDim.x) * kSliceWaves;
-    int64_t s = int64_t(blockIdx.x) * kSliceWaves + wave;
     double dotp = 0;
-    if (s < nslices) {
-        // entries k >= the slice's width re-load its last column (an empty
-        // slice: the entry after it, or the slack, read as padding) and are
-        // not summed
-        // every slice narrow: its offsets start at entry slice_ptr[s] of idx
-        const int16_t* i16 = reinterpret_cast<const int16_t*>(idx);
-        int64_t base = slice_ptr[s];
-        int wd = int((slice_ptr[s + 1] - base) / kWave);
-        const int16_t* d16 = i16 + base + lane;
-        int16_t d[MAXW];
-        T v[MAXW];
-#pragma unroll
-        for (int k = 0; k < MAXW; k++) {
-            const int kk = k < wd ? k : (wd > 0 ? wd - 1 : 0);
-            const int16_t t = d16[kk * kWave];
-            d[k] = wd > 0 ? t : kSellPad16;
-            v[k] = vals[base + kk * kWave + lane];
-        }
-        for (;;) {
-            const int64_t sn = s + stride;
-            const bool more = sn < nslices;
-            const int64_t row = s * kWave + lane;
-            const int64_t rr = row < n ? row : n - 1;
-            T xv[MAXW];
-#pragma unroll
-            for (int k = 0; k < MAXW; k++) xv[k] = x[d[k] == kSellPad16 ? 0 : row + d[k]];
-            T prior = 0;
-            if (MODE == 1 || MODE == 2) prior = w[rr];
-            if (MODE == 3) prior = b[rr];
-            // look-ahead: the next slice's offsets and values (this slice again at the end)
-            const int64_t sl = more ? sn : s;
-            const int64_t bn = slice_ptr[sl];
-            const int wdn = int((slice_ptr[sl + 1] - bn) / kWave);
-            const int16_t* dn16 = i16 + bn + lane;
-            int16_t dn[MAXW];
-            T vn[MAXW];
-#pragma unroll
-            for (int k = 0; k < MAXW; k++) {
-                const int kk = k < wdn ? k : (wdn > 0 ? wdn - 1 : 0);
-                const int16_t t = dn16[kk * kWave];
-                dn[k] = wdn > 0 ? t : kSellPad16;
-                vn[k] = vals[bn + kk * kWave + lane];
-            }
-            T acc = 0;
-#pragma unroll
-            for (int k = 0; k < MAXW; k++)
-                if (k < wd && d[k] != kSellPad16) acc += v[k] * xv[k];
-            if (row < n) {
-                if (MODE == 0) w[row] = acc;
-                else if (MODE == 1) w[row] = prior + acc;
-                else if (MODE == 2) w[row] = prior - acc;
-                else w[row] = prior - acc;
-                if (DOT) dotp += double(x[row] * acc);
-            }
-            if (!more) break;
-            s = sn;
-            wd = wdn;
-#pragma unroll
-            for (int k = 0; k < MAXW; k++) {
-                d[k] = dn[k];
-                v[k] = vn[k];
-            }
-        }
-    }
+    struct Pre {
+        T prior, xr;
+    };
+    sell_pipe_run<T, MAXW>(
+        n, nslices, padded, slice_ptr, idx, vals, x,
+        [&](int64_t rr) {
+            Pre p{T(0), T(0)};
+            if (MODE == 1 || MODE == 2) p.prior = w[rr];
+            if (MODE == 3) p.prior = b[rr];
+            if (DOT) p.xr = x[rr];
+            return p;
+        },
+        [&](int64_t row, T acc, const Pre& p) {
+            if (MODE == 0) w[row] = acc;
+            else if (MODE == 1) w[row] = p.prior + acc;
+            else w[row] = p.prior - acc;
+            if (DOT) dotp += double(p.xr * acc);
+        });
     if (DOT) {
         double blk = block_sum<kSpmvNT>(dotp, scratch);
         if (threadIdx.x == 0) partials[blockIdx.x] = blk;
     }
 }
 
-// the pipelined kernel applies: every slice narrow, rows of at most 16 entries
-static int pipe_width(const m4g_matrix_s& A)
+// the pipelined kernels apply: every slice narrow, rows of at most 16 entries
+int sell_pipe_width(const m4g_matrix_s& A)
 {
     const Sell& S = A.sell;
     if (A.ncols == 0 || S.nslices == 0 || S.narrow_slices != S.nslices || std::getenv("M4G_SPMV_NO_PIPE")) return 0;
@@ -254,7 +213,7 @@ static int pipe_width(const m4g_matrix_s& A)
     return 0;
 }
 
-static int pipe_grid()
+int sell_pipe_grid()
 {
     return std::max(1, device_state().num_cus) * kPipeBlocksPerCU;
 }
@@ -265,8 +224,8 @@ static void launch_pipe(const m4g_matrix_s& A, int mw, const T* vals, const T* x
 {
     const Sell& S = A.sell;
 #define M4G_PIPE(MW)                                                                                                \
-    hipLaunchKernelGGL((k_sell_pipe<T, MODE, DOT, MW>), dim3(pipe_grid()), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, \
-                       S.slice_ptr.ptr, S.slice_idx.ptr, S.idx.ptr, vals, x, w, b, partials)
+    hipLaunchKernelGGL((k_sell_pipe<T, MODE, DOT, MW>), dim3(sell_pipe_grid()), dim3(kSpmvNT), 0, st, A.nrows, S.nslices, \
+                       S.padded, S.slice_ptr.ptr, S.idx.ptr, vals, x, w, b, partials)
     switch (mw) {
     case 3: M4G_PIPE(3); break;
     case 4: M4G_PIPE(4); break;
@@ -339,7 +298,7 @@ static void spmv_geometry(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
 void launch_spmv(const m4g_matrix_s& A, const double* x, double* w, int mode, const double* b, hipStream_t st)
 {
     if (A.nrows == 0) return;
-    if (const int mw = pipe_width(A)) {
+    if (const int mw = sell_pipe_width(A)) {
         const double* v = A.sell.vals.ptr;
         if (b) launch_pipe<double, 3, false>(A, mw, v, x, w, b, nullptr, st);
         else if (mode == M4G_ASSIGN) launch_pipe<double, 0, false>(A, mw, v, x, w, b, nullptr, st);
@@ -367,10 +326,10 @@ void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* 
         M4G_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
         return;
     }
-    if (const int mw = pipe_width(A)) {
-        M4G_REQUIRE(A.red_partials.count >= size_t(pipe_grid()), M4G_ERR_INVALID, "reduction workspace too small");
+    if (const int mw = sell_pipe_width(A)) {
+        M4G_REQUIRE(A.red_partials.count >= size_t(sell_pipe_grid()), M4G_ERR_INVALID, "reduction workspace too small");
         launch_pipe<double, 0, true>(A, mw, A.sell.vals.ptr, p, q, (const double*)nullptr, A.red_partials.ptr, st);
-        launch_sum_partials(A.red_partials.ptr, pipe_grid(), out, st);
+        launch_sum_partials(A.red_partials.ptr, sell_pipe_grid(), out, st);
         M4G_HIP(hipGetLastError());
         return;
     }
@@ -411,7 +370,7 @@ void launch_residual_t(const m4g_matrix_s& A, const T* x, const T* b, T* r, hipS
     const Sell& S = A.sell;
     const T* vals = sizeof(T) == 8 ? (const T*)S.vals.ptr : (const T*)S.vals32.ptr;
     M4G_REQUIRE(vals, M4G_ERR_INVALID, "float matrix values not prepared");
-    if (const int mw = pipe_width(A)) {
+    if (const int mw = sell_pipe_width(A)) {
         launch_pipe<T, 3, false>(A, mw, vals, x, r, b, nullptr, st);
         return;
     }
