@@ -73,10 +73,10 @@ def measured_traffic(kernel):
     """HBM bytes per launch of the kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel of known
     size); None when there is no such measurement."""
-    path = os.path.join(ROOT, "profiles", "k_sweep_k_sell_spmv_poisson3d_128_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "k_sweep_k_sell_pipe_poisson3d_128_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    name = "k_sweep" if kernel.startswith("k_sweep") else "k_sell_spmv"
+    name = "k_sweep" if kernel.startswith("k_sweep") else "k_sell_pipe"
     for line in open(path):
         parts = line.split()
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == name:
@@ -330,7 +330,7 @@ def main():
     if sweep_share >= spmv_share:
         dom = dict(kernel="k_sweep (IC(0) triangular sweep)", bytes=sweep_bytes, ms=sweep_ms)
     else:
-        dom = dict(kernel="k_cg_spmv_dot / k_sell_spmv", bytes=spmv_bytes, ms=spmv_ms)
+        dom = dict(kernel="k_sell_pipe (SpMV)", bytes=spmv_bytes, ms=spmv_ms)
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
 
     # ---- CPU baseline: the oracle (MTL4's algorithms restated), one thread ----
