@@ -43,6 +43,8 @@ enum {
     M4G_ERR_SIZE = 6,               /* mtl::incompatible_size */
     M4G_ERR_INVALID = 7,            /* null handle, bad enum, bad argument */
     M4G_ERR_INTERRUPTED = 8,        /* interruptible_iteration flag set (interruptible_iteration.hpp:37-40) */
+    M4G_ERR_FILE = 9,               /* mtl::file_not_found (matrix_market.hpp:50-60, exception.hpp:232-236) */
+    M4G_ERR_FORMAT = 10,            /* mtl::runtime_error from the Matrix Market reader (matrix_market.hpp:172-215) */
     M4G_ERR_HIP = -1,               /* HIP runtime error */
     M4G_ERR_NO_DEVICE = -2,         /* no GPU visible */
     M4G_ERR_TIMEOUT = -3            /* a bounded device-side wait ran out (schedule fault) */

@@ -34,6 +34,7 @@
 
 #include "../mtl4gpu.h"
 #include "../mtl4gpu_gen.h"
+#include "../mtl4gpu_io.h"
 
 namespace mtl {
 
@@ -57,6 +58,12 @@ struct matrix_not_square : logic_error {
 };
 struct missing_diagonal : logic_error {
     explicit missing_diagonal(const char* s = "Diagonal entry missing or not where it belongs to.") : logic_error(s) {}
+};
+struct io_error : runtime_error {
+    explicit io_error(const char* s = "I/O error") : runtime_error(s) {}
+};
+struct file_not_found : io_error {   // exception.hpp:232-236
+    explicit file_not_found(const char* s = "File not found") : io_error(s) {}
 };
 
 #define MTL_THROW_IF(Test, Exception) \
@@ -91,6 +98,8 @@ inline void throw_status(int st, const char* where)
     case M4G_ERR_ORTHOGONALITY: throw itl::unexpected_orthogonality();
     case M4G_ERR_LOGIC: throw logic_error(m4g_last_error());
     case M4G_ERR_SIZE: throw incompatible_size();
+    case M4G_ERR_FILE: throw file_not_found(m4g_last_error());
+    case M4G_ERR_FORMAT: throw runtime_error(m4g_last_error());
     default: throw runtime_error(msg.c_str());
     }
 }
@@ -102,6 +111,21 @@ template <> struct value_code<float> { static const int code = M4G_F32; };
 
 namespace mat { template <typename Value> class compressed2D; }
 namespace vec { template <typename Value> class dense_vector; }
+
+namespace io {
+// A Matrix Market file name, read on construction or assignment of a matrix:
+// `compressed2D<double> A(io::matrix_market("a.mtx"))`, `A = io::matrix_market(f)`
+// (matrix_file.hpp, matrix_market.hpp)
+class matrix_market {
+  public:
+    explicit matrix_market(const std::string& file_name) : file_name_(file_name) {}
+    const std::string& file_name() const { return file_name_; }
+    template <typename Matrix> void read_into(Matrix& A) const;
+
+  private:
+    std::string file_name_;
+};
+}  // namespace io
 
 // ---------------------------------------------------------------------------
 // lazy expressions of the path
@@ -426,6 +450,12 @@ class compressed2D {
     {
         if (o.h_) upload();
     }
+    compressed2D(const io::matrix_market& f) { f.read_into(*this); }
+    compressed2D& operator=(const io::matrix_market& f)
+    {
+        f.read_into(*this);
+        return *this;
+    }
     compressed2D& operator=(const compressed2D& o)
     {
         if (this != &o) {
@@ -656,6 +686,60 @@ template <typename T> struct Collection { typedef typename T::value_type value_t
 
 // resource(x): size of the temporaries (mtl/operation/resource.hpp:38-44)
 template <typename V> inline std::size_t resource(const vec::dense_vector<V>& v) { return v.size(); }
+
+namespace io {
+// matrix_market_istream(path) >> A (matrix_market.hpp:44-217) through m4g_mm_read
+class matrix_market_istream {
+  public:
+    explicit matrix_market_istream(const char* p) : path_(p) {}
+    explicit matrix_market_istream(const std::string& p) : path_(p) {}
+    template <typename V>
+    matrix_market_istream& operator>>(mat::compressed2D<V>& A)
+    {
+        int64_t nr = 0, nc = 0, nz = 0;
+        detail::throw_status(m4g_mm_read(path_.c_str(), &nr, &nc, &nz, nullptr, nullptr, nullptr),
+                             "matrix_market_istream");
+        std::vector<int64_t> S(size_t(nr) + 1), I(size_t(std::max<int64_t>(nz, 1)));
+        std::vector<double> D(size_t(std::max<int64_t>(nz, 1)));
+        detail::throw_status(m4g_mm_read(path_.c_str(), &nr, &nc, &nz, S.data(), I.data(), D.data()),
+                             "matrix_market_istream");
+        I.resize(size_t(nz));
+        A.change_dim(std::size_t(nr), std::size_t(nc));
+        A.assign_csr(std::move(S), std::move(I), std::vector<V>(D.begin(), D.begin() + nz));
+        return *this;
+    }
+    void close() {}
+
+  private:
+    std::string path_;
+};
+
+// matrix_market_ostream(path) << A (matrix_market.hpp:219-335) through m4g_mm_write
+class matrix_market_ostream {
+  public:
+    explicit matrix_market_ostream(const char* p) : path_(p) {}
+    explicit matrix_market_ostream(const std::string& p) : path_(p) {}
+    template <typename V>
+    matrix_market_ostream& operator<<(const mat::compressed2D<V>& A)
+    {
+        std::vector<double> D(A.data().begin(), A.data().end());
+        detail::throw_status(m4g_mm_write(path_.c_str(), int64_t(A.num_rows()), int64_t(A.num_cols()),
+                                          A.ref_major().data(), A.ref_minor().data(), D.data()),
+                             "matrix_market_ostream");
+        return *this;
+    }
+    void close() {}
+
+  private:
+    std::string path_;
+};
+
+template <typename Matrix>
+void matrix_market::read_into(Matrix& A) const
+{
+    matrix_market_istream(file_name_) >> A;
+}
+}  // namespace io
 
 }  // namespace mtl
 

@@ -37,6 +37,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 # ---- constants (include/mtl4gpu.h) ------------------------------------------
 OK, MAX_ITER = 0, 1
 ERR_NOT_SQUARE, ERR_MISSING_DIAGONAL, ERR_ORTHOGONALITY, ERR_LOGIC, ERR_SIZE, ERR_INVALID, ERR_INTERRUPTED = range(2, 9)
+ERR_FILE, ERR_FORMAT = 9, 10
 ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3
 F64, F32 = 0, 1
 IDX_I32, IDX_I64, IDX_U32 = 0, 1, 2
@@ -125,6 +126,8 @@ _SIGS = {
     "m4g_batch_factor_ic0": (ctypes.c_int, [_vp]),
     "m4g_batch_download_factor": (ctypes.c_int, [_vp, _vp]),
     "m4g_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "m4g_mm_read": (ctypes.c_int, [ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "m4g_mm_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "m4g_batch_cg_ic0": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         _vp, ctypes.c_int, _vp, _vp, _vp]),
     # host-only analysis (mtl4gpu_analysis.h)
@@ -504,8 +507,34 @@ class Batch:
 
 
 # ---------------------------------------------------------------------------
-# host-only helpers: analysis and generators
+# host-only helpers: Matrix Market files, analysis and generators
 # ---------------------------------------------------------------------------
+def mm_read(path):
+    """Reads a Matrix Market file as matrix_market_istream >> compressed2D<double>
+    (matrix_market.hpp:166-217).  Returns (starts, indices, values, nrows, ncols)."""
+    p = os.fsencode(path)
+    nr, nc, nz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(0)
+    _check(_f["m4g_mm_read"](p, ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nz), None, None, None),
+           "m4g_mm_read")
+    S = np.zeros(nr.value + 1, np.int64)
+    I = np.zeros(max(nz.value, 1), np.int64)
+    V = np.zeros(max(nz.value, 1), np.float64)
+    _check(_f["m4g_mm_read"](p, ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nz), _ptr(S), _ptr(I), _ptr(V)),
+           "m4g_mm_read")
+    return S, I[:nz.value], V[:nz.value], nr.value, nc.value
+
+
+def mm_write(path, starts, indices, values, ncols=None):
+    """Writes CSR arrays as matrix_market_ostream << compressed2D<double>
+    (matrix_market.hpp:235-256)."""
+    S = np.ascontiguousarray(starts, np.int64)
+    I = np.ascontiguousarray(indices, np.int64)
+    V = np.ascontiguousarray(values, np.float64)
+    n = len(S) - 1
+    _check(_f["m4g_mm_write"](os.fsencode(path), n, n if ncols is None else int(ncols), _ptr(S), _ptr(I), _ptr(V)),
+           "m4g_mm_write")
+
+
 def analysis_grid(starts, indices):
     S = np.ascontiguousarray(starts, np.int64)
     I = np.ascontiguousarray(indices, np.int64)
