@@ -334,7 +334,7 @@ def main():
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
 
     # ---- CPU baseline: the oracle (MTL4's algorithms restated), one thread ----
-    cpu = None
+    cpu = cpu_omp = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
@@ -353,6 +353,19 @@ def main():
                          f"iterations on the same 128^3 system; value = per-iteration time x the GPU's "
                          f"{iters[-1]} iterations (solve only, factorisation excluded as on the GPU side)",
                "factor_ms": round(cpu_factor_ms, 1), "ms_per_iteration": round(per_iter_ms, 2)}
+        # MTL_WITH_OPENMP semantics: parallel SpMV, dots and vector loops,
+        # sequential factorisation and sweeps, on every host core available
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
+        t0 = time.perf_counter()
+        _, omp_its = o.omp_ic0_pcg(Ao, U, L, bh, args.rtol, args.cpu_iters, threads)
+        omp_iter_ms = (time.perf_counter() - t0) * 1e3 / max(1, omp_its)
+        cpu_omp = {"value": round(omp_iter_ms * iters[-1], 1), "unit": "ms", "cores": threads, "kind": "port",
+                   "sample": f"oracle IC(0) factor, then {omp_its} PCG iterations of the same 128^3 system with "
+                             f"{threads} OpenMP threads (row loops of SpMV, dots and updates parallel, sweeps "
+                             f"sequential, as MTL_WITH_OPENMP); value = per-iteration time x the GPU's "
+                             f"{iters[-1]} iterations",
+                   "ms_per_iteration": round(omp_iter_ms, 2)}
 
     mixed = None if args.no_mixed else mixed_precision(g, args.mixed_grid)
     batch = None if args.no_batch else batched(g)
@@ -391,6 +404,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom["kernel"])},
         "cpu_baseline": cpu,
+        "cpu_baseline_openmp": cpu_omp,
         "mixed_vs_fp64_speedup": mixed["speedup"] if mixed else None,
         "mixed": mixed,
         "batched": batch,

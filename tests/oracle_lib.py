@@ -72,6 +72,24 @@ def _load():
 lib = _load()
 
 
+def omp_ic0_pcg(A, U, L, b, rtol, max_iter, threads):
+    """IC(0)-PCG with MTL_WITH_OPENMP semantics (oracle/cpu_openmp.cpp), a CPU
+    timing baseline: returns (x, iterations)."""
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle_omp.so")
+    if not os.path.exists(path):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    olib = ctypes.CDLL(path)
+    f = olib.ompc_ic0_pcg
+    f.restype = ctypes.c_int
+    f.argtypes = [_i64] + [_i64p, _i64p, _f64p] * 3 + [_f64p, _f64p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.POINTER(ctypes.c_int)]
+    x = np.zeros(A.n)
+    its = ctypes.c_int(0)
+    f(A.n, A.starts, A.indices, A.data, U.starts, U.indices, U.data, L.starts, L.indices, L.data,
+      np.ascontiguousarray(b, np.float64), x, rtol, int(max_iter), int(threads), ctypes.byref(its))
+    return x, its.value
+
+
 class CSR:
     """Host CSR triple with int64 indices (MTL4's default size_type)."""
 

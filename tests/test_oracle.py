@@ -329,3 +329,18 @@ def test_oracle_plant_a_mistake_is_detected():
     y = o.ic0_solve(Lb, bad, np.arange(9.0), factor_dtype=np.float32)
     yc = np.array([1.03194, 1.60198, 1.45357, 2.5258, 3.55386, 3.16, 2.91787, 4.09338, 3.81335])
     assert np.linalg.norm(y - yc) > 1e-3
+
+
+def test_openmp_baseline_matches_the_oracle_solve():
+    # the OpenMP timing baseline (oracle/cpu_openmp.cpp) runs the same IC(0)-CG:
+    # iteration count within one of the oracle's (its dots are summed in
+    # per-thread partials), residual at the target
+    S, I, V = o.laplacian_setup(24, 24).starts, o.laplacian_setup(24, 24).indices, o.laplacian_setup(24, 24).data
+    A = o.CSR(S, I, V)
+    st, U, L = o.ic0_factor(A)
+    b = np.linspace(-1.0, 1.0, A.n)
+    ref = o.solve("cg", A, b, pc=o.PC_IC0, max_iter=500, rtol=1e-10)
+    x, its = o.omp_ic0_pcg(A, U, L, b, 1e-10, 500, 4)
+    assert abs(its - ref["iterations"]) <= 1
+    assert o.two_norm(o.residual(A, x, b)) <= 1e-10 * o.two_norm(b) * 1.0001
+    np.testing.assert_allclose(x, ref["x"], rtol=0, atol=1e-8 * np.max(np.abs(ref["x"])))
