@@ -45,6 +45,7 @@ enum {
     M4G_ERR_INTERRUPTED = 8,        /* interruptible_iteration flag set (interruptible_iteration.hpp:37-40) */
     M4G_ERR_FILE = 9,               /* mtl::file_not_found (matrix_market.hpp:50-60, exception.hpp:232-236) */
     M4G_ERR_FORMAT = 10,            /* mtl::runtime_error from the Matrix Market reader (matrix_market.hpp:172-215) */
+    M4G_BREAKDOWN = 11,             /* iter.fail(2, "... breakdown") of cgs / bicg: the reference returns 2 (cgs.hpp:40, bicg.hpp:44) */
     M4G_ERR_HIP = -1,               /* HIP runtime error */
     M4G_ERR_NO_DEVICE = -2,         /* no GPU visible */
     M4G_ERR_TIMEOUT = -3            /* a bounded device-side wait ran out (schedule fault) */
@@ -176,6 +177,20 @@ typedef struct {
 int m4g_cg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* iter);
 /* itl::bicgstab(A, x, b, M, iter) (bicgstab.hpp:27-78). */
 int m4g_bicgstab(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* iter);
+
+/* itl::cgs(A, x, b, M, iter) (krylov/cgs.hpp:24-62).  Returns the iteration's
+ * error: 0, 1 (too many iterations) or M4G_BREAKDOWN when rho_1 was 0 (the
+ * reference records error 2 and continues; it->error holds 2).  Scalars are
+ * computed on the host from device reductions (one read-back per dot). */
+int m4g_cgs(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc M, m4g_iteration* it);
+/* itl::bicg(A, x, b, M, iter) (krylov/bicg.hpp:27-68): solve and
+ * adjoint_solve of M, A * p and adjoint(A) * p_tilde (an explicit transpose).
+ * M4G_BREAKDOWN (it->error = 2) when rho_1 = dot(z_tilde, z) is 0. */
+int m4g_bicg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc M, m4g_iteration* it);
+/* itl::tfqmr(A, x, b, L, R, iter) (krylov/tfqmr.hpp:32-99) with left
+ * preconditioner L and right preconditioner R (NULL: identity); breakdowns
+ * (sigma = 0, rho = 0) return error 1 as the reference does. */
+int m4g_tfqmr(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it);
 
 /* ---- differentiation of x = A(theta)^{-1} b --------------------------------- */
 /* No reference counterpart.  The adjoint of the solve is A^T lambda = dL/dx,

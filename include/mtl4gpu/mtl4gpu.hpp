@@ -1055,7 +1055,7 @@ inline int run(Fn fn, const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense
     iter.to_c(c, hist);
     int st = fn(A.handle(), x.dev(), b.dev(), P, &c);
     x.device_modified();
-    if (st == M4G_OK || st == M4G_MAX_ITER) {
+    if (st == M4G_OK || st == M4G_MAX_ITER || st == M4G_BREAKDOWN) {   // breakdown: iter.fail(2, ...)
         iter.from_c(c, hist);
         return iter;
     }
@@ -1090,6 +1090,32 @@ inline int bicgstab(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_v
                     const Preconditioner& M, Iteration& iter)
 {
     return detail::run(m4g_bicgstab, A, x, b, M.handle(), iter);
+}
+
+// itl::cgs(A, x, b, M, iter) (cgs.hpp:24-62)
+template <typename Matrix, typename V, typename Preconditioner, typename Iteration>
+inline int cgs(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b, const Preconditioner& M,
+               Iteration& iter)
+{
+    return detail::run(m4g_cgs, A, x, b, M.handle(), iter);
+}
+// itl::bicg(A, x, b, M, iter) (bicg.hpp:27-68)
+template <typename Matrix, typename V, typename Preconditioner, typename Iteration>
+inline int bicg(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b, const Preconditioner& M,
+                Iteration& iter)
+{
+    return detail::run(m4g_bicg, A, x, b, M.handle(), iter);
+}
+// itl::tfqmr(A, x, b, L, R, iter) (tfqmr.hpp:32-99)
+template <typename Matrix, typename V, typename LeftPreconditioner, typename RightPreconditioner, typename Iteration>
+inline int tfqmr(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                 const LeftPreconditioner& L, const RightPreconditioner& R, Iteration& iter)
+{
+    const m4g_pc r = R.handle();
+    auto fn = [r](m4g_matrix a, m4g_vector xx, m4g_vector bb, m4g_pc l, m4g_iteration* c) {
+        return m4g_tfqmr(a, xx, bb, l, r, c);
+    };
+    return detail::run(fn, A, x, b, L.handle(), iter);
 }
 
 // base_solver (base_solver.hpp:24-84) and the solver classes (cg.hpp:125-179, bicgstab.hpp:84-112)
@@ -1182,6 +1208,69 @@ class bicgstab_solver
 
   private:
     Preconditioner L;
+};
+
+// cgs_solver (cgs.hpp:66-97), bicg_solver (bicg.hpp:72-103): right preconditioner ignored
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class cgs_solver : public base_solver<cgs_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<cgs_solver, LinearOperator> base;
+
+  public:
+    explicit cgs_solver(const LinearOperator& A) : base(A), L(A)
+    {
+        if (!pc::static_is_identity<RightPreconditioner>::value) std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return cgs(this->A, x, b, L, iter);
+    }
+
+  private:
+    Preconditioner L;
+};
+
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class bicg_solver
+    : public base_solver<bicg_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<bicg_solver, LinearOperator> base;
+
+  public:
+    explicit bicg_solver(const LinearOperator& A) : base(A), L(A)
+    {
+        if (!pc::static_is_identity<RightPreconditioner>::value) std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return bicg(this->A, x, b, L, iter);
+    }
+
+  private:
+    Preconditioner L;
+};
+
+// tfqmr_solver (tfqmr.hpp:103-131): left and right preconditioners
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class tfqmr_solver
+    : public base_solver<tfqmr_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<tfqmr_solver, LinearOperator> base;
+
+  public:
+    explicit tfqmr_solver(const LinearOperator& A) : base(A), L(A), R(A) {}
+    tfqmr_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L), R(A) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return tfqmr(this->A, x, b, L, R, iter);
+    }
+
+  private:
+    Preconditioner L;
+    RightPreconditioner R;
 };
 
 }  // namespace itl

@@ -37,7 +37,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 # ---- constants (include/mtl4gpu.h) ------------------------------------------
 OK, MAX_ITER = 0, 1
 ERR_NOT_SQUARE, ERR_MISSING_DIAGONAL, ERR_ORTHOGONALITY, ERR_LOGIC, ERR_SIZE, ERR_INVALID, ERR_INTERRUPTED = range(2, 9)
-ERR_FILE, ERR_FORMAT = 9, 10
+ERR_FILE, ERR_FORMAT, BREAKDOWN = 9, 10, 11
 ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3
 F64, F32 = 0, 1
 IDX_I32, IDX_I64, IDX_U32 = 0, 1, 2
@@ -127,6 +127,9 @@ _SIGS = {
     "m4g_batch_download_factor": (ctypes.c_int, [_vp, _vp]),
     "m4g_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "m4g_mm_read": (ctypes.c_int, [ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "m4g_cgs": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_bicg": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_tfqmr": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
     "m4g_mm_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "m4g_batch_cg_ic0": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         _vp, ctypes.c_int, _vp, _vp, _vp]),
@@ -441,6 +444,24 @@ def cg(A, x, b, P, it):
 def bicgstab(A, x, b, P, it):
     """itl::bicgstab(A, x, b, P, iter); returns the iteration's error code."""
     return _solve(_f["m4g_bicgstab"], "m4g_bicgstab", A, x, b, P, it)
+
+
+def cgs(A, x, b, P, it):
+    """itl::cgs(A, x, b, P, iter); returns 0, MAX_ITER or BREAKDOWN."""
+    st = _f["m4g_cgs"](A.h, x.h, b.h, P.h, ctypes.byref(it))
+    return _check(st, "m4g_cgs", allow=(OK, MAX_ITER, BREAKDOWN))
+
+
+def bicg(A, x, b, P, it):
+    """itl::bicg(A, x, b, P, iter); returns 0, MAX_ITER or BREAKDOWN."""
+    st = _f["m4g_bicg"](A.h, x.h, b.h, P.h, ctypes.byref(it))
+    return _check(st, "m4g_bicg", allow=(OK, MAX_ITER, BREAKDOWN))
+
+
+def tfqmr(A, x, b, L, it, R=None):
+    """itl::tfqmr(A, x, b, L, R, iter) (R = None: identity); returns the iteration's error."""
+    st = _f["m4g_tfqmr"](A.h, x.h, b.h, L.h, R.h if R is not None else None, ctypes.byref(it))
+    return _check(st, "m4g_tfqmr", allow=(OK, MAX_ITER))
 
 
 def refined_solve(method, A, x, b, P, it, inner_rtol, inner_max_iter=10000):

@@ -592,4 +592,151 @@ int orc_bicgstab(idx_t n, const idx_t* S, const idx_t* I, const double* D,
     return it->error;
 }
 
+// adjoint_solve(P, x) (pc/solver.hpp:22-52): identity, diagonal (real) and
+// IC(0) (ic_0.hpp:97-103) solve as usual; ILU(0) runs ilu::adjoint_solve.
+static void pc_adjoint_solve(const orc_pc* P, const double* x, double* y)
+{
+    if (P->kind == ORC_PC_ILU0) orc_ilu0_adjoint_solve(P->n, P->LS, P->LI, P->LD, P->US, P->UI, P->UD, x, y);
+    else pc_solve(P, x, y);
+}
+
+// w = adjoint(A) * v for a real CSR A: the transposed view is traversed by
+// A's rows, each row scattering into w (entries of w summed in row order of A).
+static void spmv_adjoint(idx_t n, const idx_t* S, const idx_t* I, const double* D, const double* v, double* w)
+{
+    for (idx_t k = 0; k < n; k++) w[k] = 0.0;
+    for (idx_t r = 0; r < n; r++)
+        for (idx_t e = S[r]; e < S[r + 1]; e++) w[I[e]] += D[e] * v[r];
+}
+
+// itl::cgs (cgs.hpp:24-62).  A zero rho_1 records error 2 ("cgs breakdown")
+// and, as in the reference, the loop goes on.
+int orc_cgs(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b, const orc_pc* P,
+            orc_iteration* it)
+{
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
+    std::vector<double> p(n), phat(n), q(n), qhat(n), vhat(n), u(n), uhat(n), r(n), rtilde(n);
+    orc_residual(n, S, I, D, x, b, r.data());
+    rtilde = r;
+    while (!orc_finished(it, two_norm(n, r.data()))) {
+        ++it->i;
+        rho_1 = dot4(n, rtilde.data(), r.data());
+        if (rho_1 == 0.0) it->error = 2;
+        if (it->i <= 1) {
+            u = r;
+            p = r;
+        } else {
+            beta = rho_1 / rho_2;
+            for (idx_t k = 0; k < n; k++) u[k] = r[k] + beta * q[k];
+            for (idx_t k = 0; k < n; k++) p[k] = u[k] + beta * (q[k] + beta * p[k]);
+        }
+        pc_solve(P, p.data(), phat.data());
+        orc_spmv(n, S, I, D, phat.data(), vhat.data(), 0);
+        alpha = rho_1 / dot4(n, rtilde.data(), vhat.data());
+        for (idx_t k = 0; k < n; k++) q[k] = u[k] - alpha * vhat[k];
+        for (idx_t k = 0; k < n; k++) u[k] += q[k];
+        pc_solve(P, u.data(), uhat.data());
+        for (idx_t k = 0; k < n; k++) x[k] += alpha * uhat[k];
+        orc_spmv(n, S, I, D, uhat.data(), qhat.data(), 0);
+        for (idx_t k = 0; k < n; k++) r[k] -= alpha * qhat[k];
+        rho_2 = rho_1;
+    }
+    return it->error;
+}
+
+// itl::bicg (bicg.hpp:27-68).  A zero rho_1 returns error 2 ("bicg breakdown").
+int orc_bicg(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b, const orc_pc* P,
+             orc_iteration* it)
+{
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
+    std::vector<double> r(n), z(n), p(n), q(n), r_tilde(n), z_tilde(n), p_tilde(n), q_tilde(n);
+    orc_residual(n, S, I, D, x, b, r.data());
+    r_tilde = r;
+    while (!orc_finished(it, two_norm(n, r.data()))) {
+        ++it->i;
+        pc_solve(P, r.data(), z.data());
+        pc_adjoint_solve(P, r_tilde.data(), z_tilde.data());
+        rho_1 = dot4(n, z_tilde.data(), z.data());
+        if (rho_1 == 0.0) return (it->error = 2);
+        if (it->i <= 1) {
+            p = z;
+            p_tilde = z_tilde;
+        } else {
+            beta = rho_1 / rho_2;
+            for (idx_t k = 0; k < n; k++) p[k] = z[k] + beta * p[k];
+            for (idx_t k = 0; k < n; k++) p_tilde[k] = z_tilde[k] + beta * p_tilde[k];
+        }
+        orc_spmv(n, S, I, D, p.data(), q.data(), 0);
+        spmv_adjoint(n, S, I, D, p_tilde.data(), q_tilde.data());
+        alpha = rho_1 / dot4(n, p_tilde.data(), q.data());
+        for (idx_t k = 0; k < n; k++) x[k] += alpha * p[k];
+        for (idx_t k = 0; k < n; k++) r[k] -= alpha * q[k];
+        for (idx_t k = 0; k < n; k++) r_tilde[k] -= alpha * q_tilde[k];
+        rho_2 = rho_1;
+    }
+    return it->error;
+}
+
+// itl::tfqmr (tfqmr.hpp:32-99) with left preconditioner L and right
+// preconditioner R (R = NULL: identity).  Breakdowns return error 1.
+int orc_tfqmr(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b,
+              const orc_pc* L, const orc_pc* R, orc_iteration* it)
+{
+    const orc_pc ident = {ORC_PC_IDENTITY, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (!R) R = &ident;
+    double theta = 0, eta = 0, tau, rho, rhon, sigma, alpha, beta, c;
+    std::vector<double> rt(n), r(n), u1(n), u2(n), y1(n), y2(n), w(n), d(n, 0.0), v(n), tmp(n);
+    pc_solve(R, x, tmp.data());                               // rt = b - A * solve(R, x)
+    orc_residual(n, S, I, D, tmp.data(), b, rt.data());
+    pc_solve(L, rt.data(), r.data());
+    if (orc_finished(it, two_norm(n, rt.data()))) return it->error;
+    y1 = r;
+    w = r;
+    pc_solve(R, y1.data(), tmp.data());
+    orc_spmv(n, S, I, D, tmp.data(), rt.data(), 0);
+    pc_solve(L, rt.data(), u1.data());
+    v = u1;
+    tau = two_norm(n, r.data());
+    rho = tau * tau;
+    while (!orc_finished(it, tau)) {
+        ++it->i;
+        sigma = dot4(n, r.data(), v.data());
+        if (sigma == 0.0) return (it->error = 1);
+        alpha = rho / sigma;
+        for (int j = 1; j < 3; j++) {
+            if (j == 1) {
+                for (idx_t k = 0; k < n; k++) w[k] -= alpha * u1[k];
+                const double s = theta * theta * eta / alpha;
+                for (idx_t k = 0; k < n; k++) d[k] = y1[k] + s * d[k];
+            } else {
+                for (idx_t k = 0; k < n; k++) y2[k] = y1[k] - alpha * v[k];
+                pc_solve(R, y2.data(), tmp.data());
+                orc_spmv(n, S, I, D, tmp.data(), rt.data(), 0);
+                pc_solve(L, rt.data(), u2.data());
+                for (idx_t k = 0; k < n; k++) w[k] -= alpha * u2[k];
+                const double s = theta * theta * eta / alpha;
+                for (idx_t k = 0; k < n; k++) d[k] = y2[k] + s * d[k];
+            }
+            theta = two_norm(n, w.data()) / tau;
+            c = 1.0 / std::sqrt(1.0 + theta * theta);
+            tau *= theta * c;
+            eta = c * c * alpha;
+            for (idx_t k = 0; k < n; k++) x[k] += eta * d[k];
+        }
+        if (rho == 0.0) return (it->error = 1);
+        rhon = dot4(n, r.data(), w.data());
+        beta = rhon / rho;
+        rho = rhon;
+        for (idx_t k = 0; k < n; k++) y1[k] = w[k] + beta * y2[k];
+        pc_solve(R, y1.data(), tmp.data());
+        orc_spmv(n, S, I, D, tmp.data(), rt.data(), 0);
+        pc_solve(L, rt.data(), u1.data());
+        for (idx_t k = 0; k < n; k++) v[k] = u1[k] + beta * (u2[k] + beta * v[k]);
+        // rt = A * x - b (tfqmr.hpp:94) is not used afterwards
+    }
+    pc_solve(R, x, tmp.data());                               // shift back: x = solve(R, x)
+    for (idx_t k = 0; k < n; k++) x[k] = tmp[k];
+    return it->error;
+}
+
 }  // extern "C"

@@ -61,6 +61,10 @@ def _load():
         "orc_diagonal_setup": (None, [_i64, _i64p, _i64p, _f64p, _f64p]),
         "orc_cg": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
         "orc_bicgstab": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
+        "orc_cgs": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
+        "orc_bicg": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
+        "orc_tfqmr": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(PC),
+                                     ctypes.POINTER(Iteration)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -244,7 +248,8 @@ def make_pc(A, kind):
 
 def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0, norm_r0=None,
           history=True):
-    """Runs itl::cg / itl::bicgstab on the oracle; returns dict(x, status, iterations, resid, history)."""
+    """Runs itl::cg / bicgstab / cgs / bicg / tfqmr (right preconditioner: identity) on the
+    oracle; returns dict(x, status, iterations, resid, history)."""
     b = np.ascontiguousarray(b, np.float64)
     x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
     P, keep = make_pc(A, pc)
@@ -256,7 +261,10 @@ def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0,
     if history:
         it.history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         it.history_cap = len(hist)
-    f = lib.orc_cg if method == "cg" else lib.orc_bicgstab
-    st = f(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), ctypes.byref(it))
+    if method == "tfqmr":
+        st = lib.orc_tfqmr(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), None, ctypes.byref(it))
+    else:
+        f = {"cg": lib.orc_cg, "bicgstab": lib.orc_bicgstab, "cgs": lib.orc_cgs, "bicg": lib.orc_bicg}[method]
+        st = f(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), ctypes.byref(it))
     del keep
     return dict(x=x, status=st, iterations=it.i, resid=it.resid, history=hist[: it.i + 1])

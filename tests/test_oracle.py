@@ -344,3 +344,57 @@ def test_openmp_baseline_matches_the_oracle_solve():
     assert abs(its - ref["iterations"]) <= 1
     assert o.two_norm(o.residual(A, x, b)) <= 1e-10 * o.two_norm(b) * 1.0001
     np.testing.assert_allclose(x, ref["x"], rtol=0, atol=1e-8 * np.max(np.abs(ref["x"])))
+
+
+def test_more_krylov_restatements_converge():
+    # cgs / bicg / tfqmr (oracle restatements of cgs.hpp, bicg.hpp, tfqmr.hpp):
+    # on an SPD system without preconditioner BiCG is CG in exact arithmetic,
+    # so its iteration count equals CG's up to one; all three reach the tolerance
+    A = o.laplacian_setup(20, 20)
+    b = np.linspace(-1.0, 1.0, A.n)
+    cg = o.solve("cg", A, b, pc=0, max_iter=2000, rtol=1e-10)
+    for m in ("cgs", "bicg", "tfqmr"):
+        for pc in (0, o.PC_IC0, o.PC_ILU0):
+            r = o.solve(m, A, b, pc=pc, max_iter=2000, rtol=1e-10)
+            assert r["status"] == 0, (m, pc)
+            assert o.two_norm(o.residual(A, r["x"], b)) <= 20 * 1e-10 * o.two_norm(b), (m, pc)
+    bicg = o.solve("bicg", A, b, pc=0, max_iter=2000, rtol=1e-10)
+    assert abs(bicg["iterations"] - cg["iterations"]) <= 1
+    # the oracle's restatement keeps MTL4's rho_1 = dot(z_tilde, z) (bicg.hpp:42): with IC(0) it is
+    # not PCG's r.z, so BiCG needs more iterations than PCG here
+    assert o.solve("bicg", A, b, pc=o.PC_IC0, max_iter=2000, rtol=1e-10)["iterations"] > \
+        o.solve("cg", A, b, pc=o.PC_IC0, max_iter=2000, rtol=1e-10)["iterations"]
+
+
+def test_more_krylov_finite_termination():
+    # exact arithmetic fixes these counts: with a minimal polynomial of degree m
+    # BiCG, CGS and TFQMR reach the exact solution after m steps (barring
+    # breakdown).  diag(1,2,5) blocks: m = 3; A = I + N with N^2 = 0 (nonsymmetric,
+    # so BiCG's shadow sequence needs the true A^T): m = 2
+    d = np.repeat([1.0, 2.0, 5.0], 10)
+    D = o.CSR(np.arange(31, dtype=np.int64), np.arange(30, dtype=np.int64), d)
+    n, h = 40, 20
+    rng = np.random.default_rng(2)
+    S, I, V = [0], [], []
+    for i in range(n):
+        row = [(i, 1.0)]
+        if i < h:
+            row += [(int(j), float(rng.uniform(-1, 1))) for j in rng.choice(np.arange(h, n), 3, replace=False)]
+        row.sort()
+        I += [c for c, _ in row]
+        V += [v for _, v in row]
+        S.append(len(I))
+    N = o.CSR(np.array(S, np.int64), np.array(I, np.int64), np.array(V))
+    for A, m in ((D, 3), (N, 2)):
+        b = rng.uniform(-1, 1, A.n)
+        for meth in ("cgs", "bicg", "tfqmr"):
+            r = o.solve(meth, A, b, pc=0, max_iter=50, rtol=1e-12)
+            assert r["status"] == 0 and r["iterations"] == m, (meth, m, r["iterations"])
+            assert o.two_norm(o.residual(A, r["x"], b)) <= 1e-12 * o.two_norm(b) * 10
+    # ic_0_krylov_test.cpp: 4x4-grid Laplacian, x0 = 0.01, b = A * ones,
+    # cyclic_iteration(b, 150, 1e-4): bicg (I), cgs (ILU), tfqmr (ILU, I) all return 0
+    A = o.laplacian_setup(4, 4)
+    b = o.spmv(A, np.ones(16))
+    for meth, pc in (("bicg", o.PC_IDENTITY), ("cgs", o.PC_ILU0), ("tfqmr", o.PC_ILU0)):
+        r = o.solve(meth, A, b, pc=pc, max_iter=150, rtol=1e-4, x0=np.full(16, 0.01))
+        assert r["status"] == 0, meth
