@@ -45,7 +45,8 @@ enum {
     M4G_ERR_INTERRUPTED = 8,        /* interruptible_iteration flag set (interruptible_iteration.hpp:37-40) */
     M4G_ERR_FILE = 9,               /* mtl::file_not_found (matrix_market.hpp:50-60, exception.hpp:232-236) */
     M4G_ERR_FORMAT = 10,            /* mtl::runtime_error from the Matrix Market reader (matrix_market.hpp:172-215) */
-    M4G_BREAKDOWN = 11,             /* iter.fail(2, "... breakdown") of cgs / bicg: the reference returns 2 (cgs.hpp:40, bicg.hpp:44) */
+    M4G_BREAKDOWN = 11,             /* iter.fail(k >= 2, "... breakdown"): the reference returns k (cgs.hpp:40, bicg.hpp:44, qmr.hpp:51-95, gmres.hpp:114, bicgstab_ell.hpp:112); it->error holds k */
+    M4G_ERR_SINGULAR = 12,          /* mtl::matrix_singular from lu_solve (lu.hpp:83; idr_s.hpp:78) */
     M4G_ERR_HIP = -1,               /* HIP runtime error */
     M4G_ERR_NO_DEVICE = -2,         /* no GPU visible */
     M4G_ERR_TIMEOUT = -3            /* a bounded device-side wait ran out (schedule fault) */
@@ -191,6 +192,39 @@ int m4g_bicg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc M, m4g_iteration* 
  * preconditioner L and right preconditioner R (NULL: identity); breakdowns
  * (sigma = 0, rho = 0) return error 1 as the reference does. */
 int m4g_tfqmr(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it);
+/* itl::qmr(A, x, b, L, R, iter) (krylov/qmr.hpp:29-109); R may be NULL
+ * (identity).  Like the reference, the loop tests rho = two_norm(solve(L,
+ * v_tilde)), the Lanczos vector's norm; breakdowns #1 (rho = 0) returns 1,
+ * #2-#6 return M4G_BREAKDOWN with it->error = 2..6. */
+int m4g_qmr(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it);
+/* itl::gmres_full(A, x, b, L, R, iter) (krylov/gmres.hpp:37-119): min(n,
+ * max_iter - iterations) Arnoldi steps, then one correction of x; the
+ * Hessenberg matrix's rotations are applied as the reference applies them
+ * (host side, with the reference's lu_solve).  M4G_BREAKDOWN (error 2) when no
+ * leading block of H is regular. */
+int m4g_gmres_full(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it);
+/* itl::gmres(A, x, b, L, R, iter, restart) (krylov/gmres.hpp:122-137):
+ * gmres_full on a copy of the iteration with `restart` more steps, then
+ * update_progress, until finished; restart >= 1. */
+int m4g_gmres(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it, int64_t restart);
+/* itl::bicgstab_2(A, x, b, L, iter) (krylov/bicgstab_2.hpp:35-130). */
+int m4g_bicgstab_2(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_iteration* it);
+/* itl::bicgstab_ell(A, x, b, L, R, iter, l) (krylov/bicgstab_ell.hpp:37-135),
+ * 1 <= l <= 64; omega = 0 returns M4G_BREAKDOWN with it->error = 3.  As in
+ * the reference, the inner test checks r_hat[j] and leaves before that step's
+ * x += alpha * u_hat[0]. */
+int m4g_bicgstab_ell(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it, int64_t l);
+/* itl::idr_s(A, x, b, L, R, iter, s) (krylov/idr_s.hpp:35-103), s <= 64; L and
+ * R are ignored, as in the reference.  The shadow space P is random(P): column
+ * j is filled by srand(17 * ++counter) and rand() (mtl::seed<double>,
+ * random.hpp:29-38), so results depend on the C library generator and on the
+ * process-wide counter below.  A singular s x s system returns
+ * M4G_ERR_SINGULAR. */
+int m4g_idr_s(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_iteration* it, int64_t s);
+/* mtl::seed<double>::counter (random.hpp:29-38): read / set the counter that
+ * idr_s's random(P) increments once per column. */
+int m4g_random_seed_counter(void);
+void m4g_set_random_seed_counter(int counter);
 
 /* ---- differentiation of x = A(theta)^{-1} b --------------------------------- */
 /* No reference counterpart.  The adjoint of the solve is A^T lambda = dL/dx,

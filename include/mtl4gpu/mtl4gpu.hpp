@@ -59,6 +59,12 @@ struct matrix_not_square : logic_error {
 struct missing_diagonal : logic_error {
     explicit missing_diagonal(const char* s = "Diagonal entry missing or not where it belongs to.") : logic_error(s) {}
 };
+struct domain_error : std::domain_error {
+    explicit domain_error(const char* s = "MTL4 domain error.") : std::domain_error(s) {}
+};
+struct matrix_singular : domain_error {   // exception.hpp:172-177
+    explicit matrix_singular(const char* s = "Matrix singular in solver.") : domain_error(s) {}
+};
 struct io_error : runtime_error {
     explicit io_error(const char* s = "I/O error") : runtime_error(s) {}
 };
@@ -100,6 +106,7 @@ inline void throw_status(int st, const char* where)
     case M4G_ERR_SIZE: throw incompatible_size();
     case M4G_ERR_FILE: throw file_not_found(m4g_last_error());
     case M4G_ERR_FORMAT: throw runtime_error(m4g_last_error());
+    case M4G_ERR_SINGULAR: throw matrix_singular();
     default: throw runtime_error(msg.c_str());
     }
 }
@@ -144,6 +151,11 @@ template <typename Vector, typename Matrix>
 struct residual_expr {                       // b - A*x
     const Vector& b;
     mat_cvec_times_expr<Matrix, Vector> Ax;
+};
+template <typename Vector, typename Matrix>
+struct product_minus_expr {                  // A*x - b
+    mat_cvec_times_expr<Matrix, Vector> Ax;
+    const Vector& b;
 };
 template <typename Vector>
 struct vec_sum_expr {                        // x + alpha*y (alpha = -1 for x - y)
@@ -242,6 +254,17 @@ class dense_vector {
         checked_change_resource(num_rows(e.Ax.A));
         check(m4g_residual(e.Ax.A.handle(), e.Ax.v.dev(), e.b.dev(), dev()), "b - A * x");
         state_ = device_valid;
+        return *this;
+    }
+    // r = A*x - b: w = A*x, then w -= b
+    template <typename Matrix>
+    dense_vector& operator=(const product_minus_expr<dense_vector, Matrix>& e)
+    {
+        dense_vector tmp(num_rows(e.Ax.A));
+        check(m4g_spmv(e.Ax.A.handle(), e.Ax.v.dev(), tmp.dev(), M4G_ASSIGN), "A * x - b");
+        check(m4g_axpy(-1.0, e.b.dev(), tmp.dev()), "A * x - b");
+        tmp.state_ = device_valid;
+        swap(tmp);
         return *this;
     }
     // z = x + alpha*y
@@ -571,6 +594,13 @@ inline residual_expr<vec::dense_vector<V>, compressed2D<V>> operator-(
     const vec::dense_vector<V>& b, const mat_cvec_times_expr<compressed2D<V>, vec::dense_vector<V>>& Ax)
 {
     return {b, Ax};
+}
+
+template <typename V>
+inline product_minus_expr<vec::dense_vector<V>, compressed2D<V>> operator-(
+    const mat_cvec_times_expr<compressed2D<V>, vec::dense_vector<V>>& Ax, const vec::dense_vector<V>& b)
+{
+    return {Ax, b};
 }
 
 // update policies of the inserter (mtl/operation/update.hpp)
@@ -1118,6 +1148,72 @@ inline int tfqmr(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vect
     return detail::run(fn, A, x, b, L.handle(), iter);
 }
 
+// itl::qmr(A, x, b, L, R, iter) (qmr.hpp:29-109)
+template <typename Matrix, typename V, typename LeftPreconditioner, typename RightPreconditioner, typename Iteration>
+inline int qmr(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b, const LeftPreconditioner& L,
+               const RightPreconditioner& R, Iteration& iter)
+{
+    const m4g_pc r = R.handle();
+    auto fn = [r](m4g_matrix a, m4g_vector xx, m4g_vector bb, m4g_pc l, m4g_iteration* c) {
+        return m4g_qmr(a, xx, bb, l, r, c);
+    };
+    return detail::run(fn, A, x, b, L.handle(), iter);
+}
+// itl::gmres_full(A, x, b, L, R, iter) (gmres.hpp:37-119)
+template <typename Matrix, typename V, typename LeftPreconditioner, typename RightPreconditioner, typename Iteration>
+inline int gmres_full(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                      const LeftPreconditioner& L, const RightPreconditioner& R, Iteration& iter)
+{
+    const m4g_pc r = R.handle();
+    auto fn = [r](m4g_matrix a, m4g_vector xx, m4g_vector bb, m4g_pc l, m4g_iteration* c) {
+        return m4g_gmres_full(a, xx, bb, l, r, c);
+    };
+    return detail::run(fn, A, x, b, L.handle(), iter);
+}
+// itl::gmres(A, x, b, L, R, iter, restart) (gmres.hpp:122-137)
+template <typename Matrix, typename V, typename LeftPreconditioner, typename RightPreconditioner, typename Iteration>
+inline int gmres(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                 const LeftPreconditioner& L, const RightPreconditioner& R, Iteration& iter, std::size_t restart)
+{
+    const m4g_pc r = R.handle();
+    const int64_t m = int64_t(restart);
+    auto fn = [r, m](m4g_matrix a, m4g_vector xx, m4g_vector bb, m4g_pc l, m4g_iteration* c) {
+        return m4g_gmres(a, xx, bb, l, r, c, m);
+    };
+    return detail::run(fn, A, x, b, L.handle(), iter);
+}
+// itl::bicgstab_2(A, x, b, L, iter) (bicgstab_2.hpp:35-130)
+template <typename Matrix, typename V, typename Preconditioner, typename Iteration>
+inline int bicgstab_2(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                      const Preconditioner& L, Iteration& iter)
+{
+    return detail::run(m4g_bicgstab_2, A, x, b, L.handle(), iter);
+}
+// itl::bicgstab_ell(A, x, b, L, R, iter, l) (bicgstab_ell.hpp:37-135)
+template <typename Matrix, typename V, typename LeftPreconditioner, typename RightPreconditioner, typename Iteration>
+inline int bicgstab_ell(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                        const LeftPreconditioner& L, const RightPreconditioner& R, Iteration& iter, std::size_t l)
+{
+    const m4g_pc r = R.handle();
+    const int64_t ell = int64_t(l);
+    auto fn = [r, ell](m4g_matrix a, m4g_vector xx, m4g_vector bb, m4g_pc lp, m4g_iteration* c) {
+        return m4g_bicgstab_ell(a, xx, bb, lp, r, c, ell);
+    };
+    return detail::run(fn, A, x, b, L.handle(), iter);
+}
+// itl::idr_s(A, x, b, L, R, iter, s) (idr_s.hpp:35-103); L and R are ignored
+template <typename Matrix, typename V, typename LeftPreconditioner, typename RightPreconditioner, typename Iteration>
+inline int idr_s(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                 const LeftPreconditioner& L, const RightPreconditioner& R, Iteration& iter, std::size_t s)
+{
+    const m4g_pc r = R.handle();
+    const int64_t ss = int64_t(s);
+    auto fn = [r, ss](m4g_matrix a, m4g_vector xx, m4g_vector bb, m4g_pc l, m4g_iteration* c) {
+        return m4g_idr_s(a, xx, bb, l, r, c, ss);
+    };
+    return detail::run(fn, A, x, b, L.handle(), iter);
+}
+
 // base_solver (base_solver.hpp:24-84) and the solver classes (cg.hpp:125-179, bicgstab.hpp:84-112)
 template <typename Solver, typename LinearOperator>
 struct base_solver {
@@ -1273,6 +1369,146 @@ class tfqmr_solver
     RightPreconditioner R;
 };
 
+// qmr_solver (qmr.hpp:113-140)
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class qmr_solver : public base_solver<qmr_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<qmr_solver, LinearOperator> base;
+
+  public:
+    explicit qmr_solver(const LinearOperator& A) : base(A), L(A), R(A) {}
+    qmr_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L), R(A) {}
+    qmr_solver(const LinearOperator& A, const Preconditioner& L, const RightPreconditioner& R) : base(A), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return qmr(this->A, x, b, L, R, iter);
+    }
+
+  private:
+    Preconditioner L;
+    RightPreconditioner R;
+};
+
+// gmres_solver (gmres.hpp:141-171), default restart 8
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class gmres_solver
+    : public base_solver<gmres_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<gmres_solver, LinearOperator> base;
+
+  public:
+    explicit gmres_solver(const LinearOperator& A, std::size_t restart = 8) : base(A), restart(restart), L(A), R(A) {}
+    gmres_solver(const LinearOperator& A, std::size_t restart, const Preconditioner& L)
+        : base(A), restart(restart), L(L), R(A) {}
+    gmres_solver(const LinearOperator& A, std::size_t restart, const Preconditioner& L, const RightPreconditioner& R)
+        : base(A), restart(restart), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return gmres(this->A, x, b, L, R, iter, restart);
+    }
+
+  private:
+    std::size_t restart;
+    Preconditioner L;
+    RightPreconditioner R;
+};
+
+// bicgstab_2_solver (bicgstab_2.hpp:136-166): right preconditioner ignored
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class bicgstab_2_solver
+    : public base_solver<bicgstab_2_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<bicgstab_2_solver, LinearOperator> base;
+
+  public:
+    explicit bicgstab_2_solver(const LinearOperator& A) : base(A), L(A)
+    {
+        if (!pc::static_is_identity<RightPreconditioner>::value) std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
+    bicgstab_2_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L)
+    {
+        if (!pc::static_is_identity<RightPreconditioner>::value) std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return bicgstab_2(this->A, x, b, L, iter);
+    }
+
+  private:
+    Preconditioner L;
+};
+
+// bicgstab_ell_solver (bicgstab_ell.hpp:140-168), default l = 8
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class bicgstab_ell_solver
+    : public base_solver<bicgstab_ell_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<bicgstab_ell_solver, LinearOperator> base;
+
+  public:
+    explicit bicgstab_ell_solver(const LinearOperator& A, std::size_t l = 8) : base(A), l(l), L(A), R(A) {}
+    bicgstab_ell_solver(const LinearOperator& A, std::size_t l, const Preconditioner& L) : base(A), l(l), L(L), R(A) {}
+    bicgstab_ell_solver(const LinearOperator& A, std::size_t l, const Preconditioner& L, const RightPreconditioner& R)
+        : base(A), l(l), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return bicgstab_ell(this->A, x, b, L, R, iter, l);
+    }
+
+  private:
+    std::size_t l;
+    Preconditioner L;
+    RightPreconditioner R;
+};
+
+// idr_s_solver (idr_s.hpp:107-136), default s = 8
+template <typename LinearOperator, typename Preconditioner = pc::identity<LinearOperator>,
+          typename RightPreconditioner = pc::identity<LinearOperator>>
+class idr_s_solver
+    : public base_solver<idr_s_solver<LinearOperator, Preconditioner, RightPreconditioner>, LinearOperator> {
+    typedef base_solver<idr_s_solver, LinearOperator> base;
+
+  public:
+    explicit idr_s_solver(const LinearOperator& A, std::size_t s = 8) : base(A), s(s), L(A), R(A) {}
+    idr_s_solver(const LinearOperator& A, std::size_t s, const Preconditioner& L) : base(A), s(s), L(L), R(A) {}
+    idr_s_solver(const LinearOperator& A, std::size_t s, const Preconditioner& L, const RightPreconditioner& R)
+        : base(A), s(s), L(L), R(R) {}
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        return idr_s(this->A, x, b, L, R, iter, s);
+    }
+
+  private:
+    std::size_t s;
+    Preconditioner L;
+    RightPreconditioner R;
+};
+
 }  // namespace itl
+
+namespace mtl {
+namespace detail {
+// the library's random(P) counter seen as an int (mtl::seed<double>::counter)
+struct seed_counter {
+    operator int() const { return m4g_random_seed_counter(); }
+    seed_counter& operator=(int c)
+    {
+        m4g_set_random_seed_counter(c);
+        return *this;
+    }
+};
+}  // namespace detail
+// mtl::seed (operation/random.hpp:29-38): only its counter is needed, which
+// the library keeps for idr_s's random(P)
+template <typename T>
+struct seed {
+    static inline detail::seed_counter counter{};
+};
+}  // namespace mtl
 
 #endif  // MTL4GPU_HPP

@@ -37,7 +37,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 # ---- constants (include/mtl4gpu.h) ------------------------------------------
 OK, MAX_ITER = 0, 1
 ERR_NOT_SQUARE, ERR_MISSING_DIAGONAL, ERR_ORTHOGONALITY, ERR_LOGIC, ERR_SIZE, ERR_INVALID, ERR_INTERRUPTED = range(2, 9)
-ERR_FILE, ERR_FORMAT, BREAKDOWN = 9, 10, 11
+ERR_FILE, ERR_FORMAT, BREAKDOWN, ERR_SINGULAR = 9, 10, 11, 12
 ERR_HIP, ERR_NO_DEVICE, ERR_TIMEOUT = -1, -2, -3
 F64, F32 = 0, 1
 IDX_I32, IDX_I64, IDX_U32 = 0, 1, 2
@@ -130,6 +130,14 @@ _SIGS = {
     "m4g_cgs": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
     "m4g_bicg": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
     "m4g_tfqmr": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_qmr": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_gmres_full": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_gmres": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
+    "m4g_bicgstab_2": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(Iteration)]),
+    "m4g_bicgstab_ell": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
+    "m4g_idr_s": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
+    "m4g_random_seed_counter": (ctypes.c_int, []),
+    "m4g_set_random_seed_counter": (None, [ctypes.c_int]),
     "m4g_mm_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "m4g_batch_cg_ic0": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         _vp, ctypes.c_int, _vp, _vp, _vp]),
@@ -462,6 +470,55 @@ def tfqmr(A, x, b, L, it, R=None):
     """itl::tfqmr(A, x, b, L, R, iter) (R = None: identity); returns the iteration's error."""
     st = _f["m4g_tfqmr"](A.h, x.h, b.h, L.h, R.h if R is not None else None, ctypes.byref(it))
     return _check(st, "m4g_tfqmr", allow=(OK, MAX_ITER))
+
+
+def _h(P):
+    return P.h if P is not None else None
+
+
+_SOLVER_STATUS = (OK, MAX_ITER, BREAKDOWN)
+
+
+def qmr(A, x, b, L, it, R=None):
+    """itl::qmr(A, x, b, L, R, iter) (R = None: identity); returns 0, MAX_ITER or BREAKDOWN."""
+    return _check(_f["m4g_qmr"](A.h, x.h, b.h, _h(L), _h(R), ctypes.byref(it)), "m4g_qmr", allow=_SOLVER_STATUS)
+
+
+def gmres(A, x, b, L, it, restart=8, R=None):
+    """itl::gmres(A, x, b, L, R, iter, restart)."""
+    st = _f["m4g_gmres"](A.h, x.h, b.h, _h(L), _h(R), ctypes.byref(it), int(restart))
+    return _check(st, "m4g_gmres", allow=_SOLVER_STATUS)
+
+
+def gmres_full(A, x, b, L, it, R=None):
+    """itl::gmres_full(A, x, b, L, R, iter)."""
+    st = _f["m4g_gmres_full"](A.h, x.h, b.h, _h(L), _h(R), ctypes.byref(it))
+    return _check(st, "m4g_gmres_full", allow=_SOLVER_STATUS)
+
+
+def bicgstab_2(A, x, b, L, it):
+    """itl::bicgstab_2(A, x, b, L, iter)."""
+    return _check(_f["m4g_bicgstab_2"](A.h, x.h, b.h, _h(L), ctypes.byref(it)), "m4g_bicgstab_2",
+                  allow=_SOLVER_STATUS)
+
+
+def bicgstab_ell(A, x, b, L, it, l=8, R=None):
+    """itl::bicgstab_ell(A, x, b, L, R, iter, l)."""
+    st = _f["m4g_bicgstab_ell"](A.h, x.h, b.h, _h(L), _h(R), ctypes.byref(it), int(l))
+    return _check(st, "m4g_bicgstab_ell", allow=_SOLVER_STATUS)
+
+
+def idr_s(A, x, b, it, s=8, L=None, R=None):
+    """itl::idr_s(A, x, b, L, R, iter, s) (L and R ignored, as in the reference)."""
+    st = _f["m4g_idr_s"](A.h, x.h, b.h, _h(L), _h(R), ctypes.byref(it), int(s))
+    return _check(st, "m4g_idr_s", allow=_SOLVER_STATUS)
+
+
+def random_seed_counter(value=None):
+    """mtl::seed<double>::counter: returns it, or sets it when `value` is given."""
+    if value is not None:
+        _f["m4g_set_random_seed_counter"](int(value))
+    return _f["m4g_random_seed_counter"]()
 
 
 def refined_solve(method, A, x, b, P, it, inner_rtol, inner_max_iter=10000):

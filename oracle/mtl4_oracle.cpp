@@ -739,4 +739,439 @@ int orc_tfqmr(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* 
     return it->error;
 }
 
+// itl::qmr (qmr.hpp:29-109) with left preconditioner L and right R (NULL:
+// identity).  trans(A) * q is the row-order scatter of spmv_adjoint.
+// Breakdowns return their codes 1..6 as the reference's iter.fail(k, ...).
+int orc_qmr(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b, const orc_pc* L,
+            const orc_pc* R, orc_iteration* it)
+{
+    const orc_pc ident = {ORC_PC_IDENTITY, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (!R) R = &ident;
+    double beta, gamma = 1.0, gamma_1, delta, eta = -1.0, ep = 1.0, rho_1, theta = 0.0, theta_1;
+    std::vector<double> r(n), v_tld(n), y(n), w_tld(n), z(n), v(n), w(n), y_tld(n), z_tld(n), p(n), q(n), p_tld(n),
+        d(n), s(n);
+    orc_residual(n, S, I, D, x, b, r.data());
+    v_tld = r;
+    pc_solve(L, v_tld.data(), y.data());
+    w_tld = r;
+    pc_adjoint_solve(R, w_tld.data(), z.data());
+    if (orc_finished(it, two_norm(n, r.data()))) return it->error;
+    double rho = two_norm(n, y.data()), xi = two_norm(n, z.data());
+    while (!orc_finished(it, rho)) {
+        ++it->i;
+        if (rho == 0.0) return (it->error = 1);
+        if (xi == 0.0) return (it->error = 2);
+        for (idx_t k = 0; k < n; k++) v[k] = v_tld[k] / rho;
+        for (idx_t k = 0; k < n; k++) y[k] /= rho;
+        for (idx_t k = 0; k < n; k++) w[k] = w_tld[k] / xi;
+        for (idx_t k = 0; k < n; k++) z[k] /= xi;
+        delta = dot4(n, z.data(), y.data());
+        if (delta == 0.0) return (it->error = 3);
+        pc_solve(R, y.data(), y_tld.data());
+        pc_adjoint_solve(L, z.data(), z_tld.data());
+        if (it->i <= 1) {
+            p = y_tld;
+            q = z_tld;
+        } else {
+            const double sp = (xi * delta) / ep, sq = (rho * delta) / ep;
+            for (idx_t k = 0; k < n; k++) p[k] = y_tld[k] - sp * p[k];
+            for (idx_t k = 0; k < n; k++) q[k] = z_tld[k] - sq * q[k];
+        }
+        orc_spmv(n, S, I, D, p.data(), p_tld.data(), 0);
+        ep = dot4(n, q.data(), p_tld.data());
+        if (ep == 0.0) return (it->error = 4);
+        beta = ep / delta;
+        if (beta == 0.0) return (it->error = 5);
+        for (idx_t k = 0; k < n; k++) v_tld[k] = p_tld[k] - beta * v[k];
+        pc_solve(L, v_tld.data(), y.data());
+        rho_1 = rho;
+        rho = two_norm(n, y.data());
+        spmv_adjoint(n, S, I, D, q.data(), w_tld.data());
+        for (idx_t k = 0; k < n; k++) w_tld[k] = w_tld[k] - beta * w[k];
+        pc_adjoint_solve(R, w_tld.data(), z.data());
+        xi = two_norm(n, z.data());
+        gamma_1 = gamma;
+        theta_1 = theta;
+        theta = rho / (gamma_1 * beta);
+        gamma = 1.0 / std::sqrt(1.0 + theta * theta);
+        if (gamma == 0.0) return (it->error = 6);
+        eta = -eta * rho_1 * gamma * gamma / (beta * gamma_1 * gamma_1);
+        if (it->i <= 1) {
+            for (idx_t k = 0; k < n; k++) d[k] = eta * p[k];
+            for (idx_t k = 0; k < n; k++) s[k] = eta * p_tld[k];
+        } else {
+            const double f = theta_1 * theta_1 * gamma * gamma;
+            for (idx_t k = 0; k < n; k++) d[k] = eta * p[k] + f * d[k];
+            for (idx_t k = 0; k < n; k++) s[k] = eta * p_tld[k] + f * s[k];
+        }
+        for (idx_t k = 0; k < n; k++) x[k] += d[k];
+        for (idx_t k = 0; k < n; k++) r[k] -= s[k];
+    }
+    return it->error;
+}
+
+// ---- small dense algebra of gmres / idr_s ----------------------------------
+
+// lu_solve(A, b) (mtl/operation/lu.hpp:132-140): LU with partial pivoting
+// (lu.hpp:63-88: first row of largest |a| in the column, rows swapped, a zero
+// pivot among the first m-1 throws matrix_singular), then
+// upper_trisolve(upper(LU), unit_lower_trisolve(strict_lower(LU), P b)).
+// A is m x m row-major; returns false for matrix_singular.
+static bool dense_lu_solve(int m, std::vector<double> A, const double* b, double* y)
+{
+    std::vector<int> P(m);
+    for (int i = 0; i < m; i++) P[i] = i;
+    for (int i = 0; i + 1 < m; i++) {
+        int rmax = i;
+        for (int r = i + 1; r < m; r++)
+            if (std::fabs(A[size_t(r) * m + i]) > std::fabs(A[size_t(rmax) * m + i])) rmax = r;
+        if (rmax != i) {
+            for (int c = 0; c < m; c++) std::swap(A[size_t(i) * m + c], A[size_t(rmax) * m + c]);
+            std::swap(P[i], P[rmax]);
+        }
+        if (std::fabs(A[size_t(i) * m + i]) <= 0.0) return false;
+        for (int r = i + 1; r < m; r++) A[size_t(r) * m + i] /= A[size_t(i) * m + i];
+        for (int r = i + 1; r < m; r++)
+            for (int c = i + 1; c < m; c++) A[size_t(r) * m + c] -= A[size_t(r) * m + i] * A[size_t(i) * m + c];
+    }
+    std::vector<double> w(m);
+    for (int r = 0; r < m; r++) {
+        double t = b[P[r]];
+        for (int c = 0; c < r; c++) t -= A[size_t(r) * m + c] * w[c];
+        w[r] = t;
+    }
+    for (int r = m - 1; r >= 0; r--) {
+        double t = w[r];
+        for (int c = r + 1; c < m; c++) t -= A[size_t(r) * m + c] * y[c];
+        y[r] = t / A[size_t(r) * m + r];
+    }
+    return true;
+}
+
+// mat::givens::set_rotation (mtl/operation/givens.hpp:40-56)
+static void givens_rotation(double a, double b, double& c, double& s)
+{
+    if (b == 0.0) {
+        c = 1.0;
+        s = 0.0;
+    } else if (std::fabs(b) > std::fabs(a)) {
+        const double t = -a / b;
+        s = 1.0 / std::sqrt(1.0 + t * t);
+        c = s * t;
+    } else {
+        const double t = -b / a;
+        c = 1.0 / std::sqrt(1.0 + t * t);
+        s = c * t;
+    }
+}
+
+// mat::givens::trafo(k) (givens.hpp:66-84) on the (rows x cols) row-major H
+// with G = [c s; -s c]: rows k, k+1 become trans(G) * H[k:k+2][:], then
+// columns k, k+1 become H[:][k:k+2] * G.
+static void givens_trafo(std::vector<double>& H, int rows, int cols, int k, double c, double s)
+{
+    for (int j = 0; j < cols; j++) {
+        const double h0 = H[size_t(k) * cols + j], h1 = H[size_t(k + 1) * cols + j];
+        H[size_t(k) * cols + j] = c * h0 + (-s) * h1;
+        H[size_t(k + 1) * cols + j] = s * h0 + c * h1;
+    }
+    if (k + 1 >= cols) return;   // the column block lies outside H
+    for (int i = 0; i < rows; i++) {
+        const double h0 = H[size_t(i) * cols + k], h1 = H[size_t(i) * cols + k + 1];
+        H[size_t(i) * cols + k] = h0 * c + h1 * (-s);
+        H[size_t(i) * cols + k + 1] = h0 * s + h1 * c;
+    }
+}
+
+// itl::gmres_full (gmres.hpp:37-119): kmax = min(n, max_iter - iterations)
+// Arnoldi steps with modified Gram-Schmidt and one re-orthogonalisation, the
+// reference's Givens handling (for i < k a rotation built from column k-1,
+// applied from both sides, then the new rotation of column k), H y = g by
+// lu_solve on the largest regular leading block, x += solve(R, V y) and the
+// final test on the true residual b - A x.
+int orc_gmres_full(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b,
+                   const orc_pc* L, const orc_pc* R, orc_iteration* it)
+{
+    const orc_pc ident = {ORC_PC_IDENTITY, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (!R) R = &ident;
+    const long room = long(it->max_iter) - long(it->i);
+    const int kmax = int(room < 0 ? n : std::min<long>(long(n), room));
+    const int rows = kmax + 1, cols = kmax;
+    std::vector<double> r0(n), r(n), va(n), va0(n), tmp(n);
+    std::vector<std::vector<double>> V(size_t(kmax) + 1, std::vector<double>(n, 0.0));
+    std::vector<double> s(size_t(kmax) + 1, 0.0), c(size_t(kmax) + 1, 0.0), g(size_t(kmax) + 1, 0.0),
+        y(size_t(kmax), 0.0), H(size_t(rows) * std::max(cols, 1), 0.0);
+    auto h = [&](int i, int j) -> double& { return H[size_t(i) * cols + j]; };
+    orc_residual(n, S, I, D, x, b, r0.data());
+    pc_solve(L, r0.data(), r.data());
+    double rho = g[0] = two_norm(n, r.data());
+    if (orc_finished(it, rho)) return it->error;
+    for (idx_t e = 0; e < n; e++) V[0][e] = r[e] / rho;
+    int k;
+    for (k = 0; k < kmax; ++k, ++it->i) {
+        pc_solve(R, V[k].data(), tmp.data());
+        orc_spmv(n, S, I, D, tmp.data(), va0.data(), 0);
+        pc_solve(L, va0.data(), va.data());
+        V[k + 1] = va;
+        for (int j = 0; j < k + 1; j++) {
+            h(j, k) = dot4(n, V[j].data(), V[k + 1].data());
+            for (idx_t e = 0; e < n; e++) V[k + 1][e] -= h(j, k) * V[j][e];
+        }
+        h(k + 1, k) = two_norm(n, V[k + 1].data());
+        for (int j = 0; j < k + 1; j++) {
+            const double hr = dot4(n, V[k + 1].data(), V[j].data());
+            h(j, k) += hr;
+            for (idx_t e = 0; e < n; e++) V[k + 1][e] -= hr * V[j][e];
+        }
+        h(k + 1, k) = two_norm(n, V[k + 1].data());
+        if (h(k + 1, k) != 0.0) {
+            const double inv = 1. / h(k + 1, k);
+            for (idx_t e = 0; e < n; e++) V[k + 1][e] *= inv;
+        }
+        for (int i = 0; i < k; i++) {
+            double gc, gs;
+            givens_rotation(h(i, k - 1), h(i + 1, k - 1), gc, gs);
+            givens_trafo(H, rows, cols, i, gc, gs);
+        }
+        const double nu = std::sqrt(h(k, k) * h(k, k) + h(k + 1, k) * h(k + 1, k));
+        if (nu != 0.0) {
+            c[k] = h(k, k) / nu;
+            s[k] = -h(k + 1, k) / nu;
+            h(k, k) = c[k] * h(k, k) - s[k] * h(k + 1, k);
+            h(k + 1, k) = 0;
+            const double w1 = c[k] * g[k] - s[k] * g[k + 1], w2 = s[k] * g[k] + c[k] * g[k + 1];
+            g[k] = w1;
+            g[k + 1] = w2;
+        }
+        rho = std::fabs(g[k + 1]);
+    }
+    while (k > 0 && std::fabs(g[k - 1]) <= it->atol) k--;
+    int m = k;
+    for (; m > 0; --m) {
+        std::vector<double> Hm(size_t(m) * m);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) Hm[size_t(i) * m + j] = h(i, j);
+        if (dense_lu_solve(m, Hm, g.data(), y.data())) break;
+    }
+    if (m == 0) return (it->error = 2);   // "GMRES did not find any direction to correct x"
+    std::vector<double> Vy(n, 0.0);
+    for (int j = 0; j < m; j++)
+        for (idx_t e = 0; e < n; e++) Vy[e] += V[j][e] * y[j];
+    pc_solve(R, Vy.data(), tmp.data());
+    for (idx_t e = 0; e < n; e++) x[e] += tmp[e];
+    orc_residual(n, S, I, D, x, b, r.data());
+    orc_finished(it, two_norm(n, r.data()));   // iter.terminate(r)
+    return it->error;
+}
+
+// itl::gmres with restart (gmres.hpp:122-137): gmres_full on a copy of the
+// iteration limited to `restart` more steps, then update_progress
+// (basic_iteration.hpp:145-155) until the iteration is finished.
+int orc_gmres(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b,
+              const orc_pc* L, const orc_pc* R, orc_iteration* it, int restart)
+{
+    if (restart < 1) return -1;
+    do {
+        orc_iteration inner = *it;
+        inner.max_iter = std::min(it->i + restart, it->max_iter);
+        orc_gmres_full(n, S, I, D, x, b, L, R, &inner);
+        it->i = inner.i;
+        it->resid = inner.resid;
+        if (inner.error > 1) {
+            it->error = inner.error;
+            it->is_finished = 1;
+        } else {
+            orc_finished(it, it->resid);
+        }
+    } while (!it->is_finished);
+    return it->error;
+}
+
+// orthogonalize_factors (mtl/operation/orth.hpp:61-85, unfused branch) on
+// v[0..m): tau[0][0] = v0.v0; for j: tau[i][j] = (v_i.v_j) / tau[i][i],
+// v_j -= tau[i][j] v_i (i < j), tau[j][j] = v_j.v_j.  tau is m x m row-major.
+static void orthogonalize_factors(idx_t n, std::vector<std::vector<double>*>& v, std::vector<double>& tau)
+{
+    const int m = int(v.size());
+    tau.assign(size_t(m) * m, 0.0);
+    if (m == 0) return;
+    tau[0] = dot4(n, v[0]->data(), v[0]->data());
+    for (int j = 1; j < m; j++) {
+        for (int i = 0; i < j; i++) {
+            const double t = dot4(n, v[i]->data(), v[j]->data()) / tau[size_t(i) * m + i];
+            tau[size_t(i) * m + j] = t;
+            for (idx_t e = 0; e < n; e++) (*v[j])[e] -= t * (*v[i])[e];
+        }
+        tau[size_t(j) * m + j] = dot4(n, v[j]->data(), v[j]->data());
+    }
+}
+
+// itl::bicgstab_ell (bicgstab_ell.hpp:37-135) with left L and right R (NULL:
+// identity); itl::bicgstab_2 (bicgstab_2.hpp:35-130) is this with l = 2 and
+// R = identity.  The MR part orthogonalises r_hat[1..l] in place (the
+// sub-vector r_hat[1:] is a view).  omega == 0 returns error 3.
+int orc_bicgstab_ell(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b,
+                     const orc_pc* L, const orc_pc* R, orc_iteration* it, int l)
+{
+    const orc_pc ident = {ORC_PC_IDENTITY, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (!R) R = &ident;
+    if (l < 1) return -1;
+    std::vector<double> x0(n, 0.0), y(n), tmp(n), r0_tilde(n);
+    std::vector<std::vector<double>> r_hat(size_t(l) + 1, std::vector<double>(n)),
+        u_hat(size_t(l) + 1, std::vector<double>(n));
+    r_hat[0].assign(b, b + n);
+    if (two_norm(n, x) != 0.0) {
+        orc_spmv(n, S, I, D, x, tmp.data(), 0);
+        for (idx_t e = 0; e < n; e++) r_hat[0][e] -= tmp[e];
+        x0.assign(x, x + n);
+        for (idx_t e = 0; e < n; e++) x[e] = 0.0;
+    }
+    const double nr = two_norm(n, r_hat[0].data());
+    for (idx_t e = 0; e < n; e++) r0_tilde[e] = r_hat[0][e] / nr;
+    pc_solve(L, r_hat[0].data(), y.data());
+    r_hat[0] = y;
+    std::fill(u_hat[0].begin(), u_hat[0].end(), 0.0);
+    double rho_0 = 1.0, rho_1 = 0.0, alpha = 0.0, Gamma = 0.0, beta = 0.0, omega = 1.0;
+    std::vector<double> tau, gamma(size_t(l) + 1), gamma_a(size_t(l) + 1), gamma_aa(size_t(l) + 1);
+    auto finish_x = [&]() {
+        pc_solve(R, x, tmp.data());
+        for (idx_t e = 0; e < n; e++) x[e] = tmp[e] + x0[e];
+    };
+    while (!orc_finished(it, two_norm(n, r_hat[0].data()))) {
+        ++it->i;
+        rho_0 = -omega * rho_0;
+        for (int j = 0; j < l; ++j) {
+            rho_1 = dot4(n, r0_tilde.data(), r_hat[j].data());
+            beta = alpha * rho_1 / rho_0;
+            rho_0 = rho_1;
+            for (int i = 0; i <= j; ++i)
+                for (idx_t e = 0; e < n; e++) u_hat[i][e] = r_hat[i][e] - beta * u_hat[i][e];
+            pc_solve(R, u_hat[j].data(), tmp.data());
+            orc_spmv(n, S, I, D, tmp.data(), y.data(), 0);
+            pc_solve(L, y.data(), u_hat[j + 1].data());
+            Gamma = dot4(n, r0_tilde.data(), u_hat[j + 1].data());
+            alpha = rho_0 / Gamma;
+            for (int i = 0; i <= j; ++i)
+                for (idx_t e = 0; e < n; e++) r_hat[i][e] -= alpha * u_hat[i + 1][e];
+            if (orc_finished(it, two_norm(n, r_hat[j].data()))) {
+                finish_x();
+                return it->error;
+            }
+            pc_solve(R, r_hat[j].data(), r_hat[j + 1].data());
+            orc_spmv(n, S, I, D, r_hat[j + 1].data(), y.data(), 0);
+            pc_solve(L, y.data(), r_hat[j + 1].data());
+            for (idx_t e = 0; e < n; e++) x[e] += alpha * u_hat[0][e];
+        }
+        std::vector<std::vector<double>*> tail;
+        for (int j = 1; j <= l; j++) tail.push_back(&r_hat[j]);
+        orthogonalize_factors(n, tail, tau);   // tau[1..l][1..l] of the reference, 0-based here
+        auto T = [&](int i, int j) { return tau[size_t(i - 1) * l + (j - 1)]; };
+        for (int j = 1; j <= l; ++j) gamma_a[j] = dot4(n, r_hat[j].data(), r_hat[0].data()) / T(j, j);
+        gamma[l] = gamma_a[l];
+        omega = gamma[l];
+        if (omega == 0.0) return (it->error = 3);   // "bicg breakdown #2"
+        for (int j = l - 1; j > 0; --j) {
+            double sum = 0.0;
+            for (int i = j + 1; i <= l; ++i) sum += T(j, i) * gamma[i];
+            gamma[j] = gamma_a[j] - sum;
+        }
+        // gamma_aa[1..l-1] = strict_upper(tau[1..l-1][1..l-1]) * gamma[2..l] + gamma[2..l]
+        for (int j = 1; j < l; ++j) {
+            double sum = 0.0;
+            for (int i = j + 1; i < l; ++i) sum += T(j, i) * gamma[i + 1];
+            gamma_aa[j] = sum + gamma[j + 1];
+        }
+        for (idx_t e = 0; e < n; e++) x[e] += gamma[1] * r_hat[0][e];
+        for (idx_t e = 0; e < n; e++) r_hat[0][e] -= gamma_a[l] * r_hat[l][e];
+        for (idx_t e = 0; e < n; e++) u_hat[0][e] -= gamma[l] * u_hat[l][e];
+        for (int j = 1; j < l; ++j) {
+            for (idx_t e = 0; e < n; e++) u_hat[0][e] -= gamma[j] * u_hat[j][e];
+            for (idx_t e = 0; e < n; e++) x[e] += gamma_aa[j] * r_hat[j][e];
+            for (idx_t e = 0; e < n; e++) r_hat[0][e] -= gamma_a[j] * r_hat[j][e];
+        }
+    }
+    finish_x();
+    return it->error;
+}
+
+// mtl::seed<double>::counter (mtl/operation/random.hpp:29-38): every
+// random(vector) call reseeds the C library generator with 17 * ++counter and
+// fills the vector with rand() values.
+static int orc_seed_counter = 0;
+void orc_set_random_counter(int c) { orc_seed_counter = c; }
+int orc_get_random_counter() { return orc_seed_counter; }
+
+// itl::idr_s (idr_s.hpp:35-103); both preconditioners are ignored as in the
+// reference.  P's columns are random(P) (column j: srand(17 * ++counter),
+// rand()), column 0 replaced by r, then orth(P) (orth.hpp:35-54).  A singular
+// M in lu_solve returns -2 (the reference throws matrix_singular).
+int orc_idr_s(idx_t n, const idx_t* S, const idx_t* I, const double* D, double* x, const double* b,
+              orc_iteration* it, int s)
+{
+    if (s < 1) s = 1;
+    std::vector<double> v(n), t(n), q(n), r(n), tmp(n);
+    std::vector<std::vector<double>> dR(s, std::vector<double>(n, 0.0)), dX(s, std::vector<double>(n, 0.0)),
+        P(s, std::vector<double>(n, 0.0));
+    std::vector<double> m(s), c(s), dm(s), M(size_t(s) * s, 0.0);
+    orc_residual(n, S, I, D, x, b, r.data());
+    for (int j = 0; j < s; j++) {
+        srand(unsigned(17 * ++orc_seed_counter));
+        for (idx_t e = 0; e < n; e++) P[j][e] = double(rand());
+    }
+    P[0] = r;
+    for (int j = 0; j < s; j++) {
+        for (int i = 0; i < j; i++) {
+            const double d = dot4(n, P[i].data(), P[j].data());
+            for (idx_t e = 0; e < n; e++) P[j][e] -= d * P[i][e];
+        }
+        const double nrm = two_norm(n, P[j].data());
+        for (idx_t e = 0; e < n; e++) P[j][e] /= nrm;
+    }
+    double omega = 0.0;
+    for (int k = 0; k < s; k++) {
+        orc_spmv(n, S, I, D, r.data(), v.data(), 0);
+        omega = dot4(n, v.data(), r.data()) / dot4(n, v.data(), v.data());
+        for (idx_t e = 0; e < n; e++) dX[k][e] = omega * r[e];
+        for (idx_t e = 0; e < n; e++) dR[k][e] = -omega * v[e];
+        for (idx_t e = 0; e < n; e++) x[e] += dX[k][e];
+        for (idx_t e = 0; e < n; e++) r[e] += dR[k][e];
+        ++it->i;
+        if (orc_finished(it, two_norm(n, r.data()))) return it->error;
+        for (int i = 0; i < s; i++) M[size_t(i) * s + k] = dot4(n, P[i].data(), dR[k].data());
+    }
+    int oldest = 0;
+    for (int i = 0; i < s; i++) m[i] = dot4(n, P[i].data(), r.data());
+    while (!orc_finished(it, two_norm(n, r.data()))) {
+        for (int k = 0; k < s; k++) {
+            if (!dense_lu_solve(s, M, m.data(), c.data())) return -2;
+            std::fill(q.begin(), q.end(), 0.0);
+            for (int j = 0; j < s; j++)
+                for (idx_t e = 0; e < n; e++) q[e] += dR[j][e] * -c[j];
+            for (idx_t e = 0; e < n; e++) v[e] = r[e] + q[e];
+            std::fill(tmp.begin(), tmp.end(), 0.0);   // dX * c
+            for (int j = 0; j < s; j++)
+                for (idx_t e = 0; e < n; e++) tmp[e] += dX[j][e] * c[j];
+            if (k == 0) {
+                orc_spmv(n, S, I, D, v.data(), t.data(), 0);
+                omega = dot4(n, t.data(), v.data()) / dot4(n, t.data(), t.data());
+                for (idx_t e = 0; e < n; e++) dR[oldest][e] = q[e] - omega * t[e];
+                for (idx_t e = 0; e < n; e++) dX[oldest][e] = omega * v[e] - tmp[e];
+            } else {
+                for (idx_t e = 0; e < n; e++) dX[oldest][e] = omega * v[e] - tmp[e];
+                for (idx_t e = 0; e < n; e++) tmp[e] = -dX[oldest][e];
+                orc_spmv(n, S, I, D, tmp.data(), dR[oldest].data(), 0);
+            }
+            for (idx_t e = 0; e < n; e++) r[e] += dR[oldest][e];
+            for (idx_t e = 0; e < n; e++) x[e] += dX[oldest][e];
+            ++it->i;
+            if (orc_finished(it, two_norm(n, r.data()))) return it->error;
+            for (int i = 0; i < s; i++) dm[i] = dot4(n, P[i].data(), dR[oldest].data());
+            for (int i = 0; i < s; i++) M[size_t(i) * s + oldest] = dm[i];
+            for (int i = 0; i < s; i++) m[i] += dm[i];
+            oldest = (oldest + 1) % s;
+        }
+    }
+    return it->error;
+}
+
 }  // extern "C"

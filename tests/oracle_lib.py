@@ -65,6 +65,17 @@ def _load():
         "orc_bicg": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
         "orc_tfqmr": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(PC),
                                      ctypes.POINTER(Iteration)]),
+        "orc_qmr": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(PC),
+                                   ctypes.POINTER(Iteration)]),
+        "orc_gmres": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC), ctypes.POINTER(PC),
+                                     ctypes.POINTER(Iteration), ctypes.c_int]),
+        "orc_gmres_full": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC),
+                                          ctypes.POINTER(PC), ctypes.POINTER(Iteration)]),
+        "orc_bicgstab_ell": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(PC),
+                                            ctypes.POINTER(PC), ctypes.POINTER(Iteration), ctypes.c_int]),
+        "orc_idr_s": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(Iteration), ctypes.c_int]),
+        "orc_set_random_counter": (None, [ctypes.c_int]),
+        "orc_get_random_counter": (ctypes.c_int, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -247,9 +258,11 @@ def make_pc(A, kind):
 
 
 def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0, norm_r0=None,
-          history=True):
-    """Runs itl::cg / bicgstab / cgs / bicg / tfqmr (right preconditioner: identity) on the
-    oracle; returns dict(x, status, iterations, resid, history)."""
+          history=True, right=None, param=None):
+    """Runs itl::cg / bicgstab / cgs / bicg / tfqmr / qmr / gmres / gmres_full /
+    bicgstab_2 / bicgstab_ell / idr_s on the oracle; `pc` is the (left)
+    preconditioner kind, `right` the right one (None: identity), `param` the
+    restart / l / s argument.  Returns dict(x, status, iterations, resid, history)."""
     b = np.ascontiguousarray(b, np.float64)
     x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
     P, keep = make_pc(A, pc)
@@ -261,10 +274,25 @@ def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0,
     if history:
         it.history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         it.history_cap = len(hist)
+    R, keep_r = (None, []) if right is None else make_pc(A, right)
+    Rp = None if R is None else ctypes.byref(R)
+    args = (A.n, A.starts, A.indices, A.data, x, b)
     if method == "tfqmr":
-        st = lib.orc_tfqmr(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), None, ctypes.byref(it))
+        st = lib.orc_tfqmr(*args, ctypes.byref(P), Rp, ctypes.byref(it))
+    elif method == "qmr":
+        st = lib.orc_qmr(*args, ctypes.byref(P), Rp, ctypes.byref(it))
+    elif method == "gmres":
+        st = lib.orc_gmres(*args, ctypes.byref(P), Rp, ctypes.byref(it), 8 if param is None else param)
+    elif method == "gmres_full":
+        st = lib.orc_gmres_full(*args, ctypes.byref(P), Rp, ctypes.byref(it))
+    elif method == "bicgstab_2":
+        st = lib.orc_bicgstab_ell(*args, ctypes.byref(P), None, ctypes.byref(it), 2)
+    elif method == "bicgstab_ell":
+        st = lib.orc_bicgstab_ell(*args, ctypes.byref(P), Rp, ctypes.byref(it), 8 if param is None else param)
+    elif method == "idr_s":
+        st = lib.orc_idr_s(*args, ctypes.byref(it), 8 if param is None else param)
     else:
         f = {"cg": lib.orc_cg, "bicgstab": lib.orc_bicgstab, "cgs": lib.orc_cgs, "bicg": lib.orc_bicg}[method]
         st = f(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), ctypes.byref(it))
-    del keep
+    del keep, keep_r
     return dict(x=x, status=st, iterations=it.i, resid=it.resid, history=hist[: it.i + 1])

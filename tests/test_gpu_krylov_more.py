@@ -1,7 +1,11 @@
-"""itl::cgs, itl::bicg and itl::tfqmr on the GPU against the oracle's
-restatements (oracle/mtl4_oracle.cpp: orc_cgs, orc_bicg, orc_tfqmr): same
-iteration count within 2 % (at least 1; dots are reduced in a different
-order), true residual at the target, solution close to the oracle's."""
+"""itl::cgs, bicg, tfqmr, qmr, gmres, bicgstab_2, bicgstab_ell and idr_s on the
+GPU against the oracle's restatements (oracle/mtl4_oracle.cpp).  cgs, bicg and
+tfqmr: same status, iteration count within 2 % (at least 1; dots are reduced
+in a different order), solution close to the oracle's.  The others follow the
+reference's own control flow (QMR tests the Lanczos vector's norm,
+BiCGStab(l) leaves before its last x update, GMRES rotates H its own way,
+IDR(s) takes s steps per cycle), whose outcome moves with rounding: they are
+held to the spread of the oracle over rounding-level changes of b."""
 import numpy as np
 import pytest
 
@@ -73,3 +77,132 @@ def test_zero_rhs_and_max_iter(g):
         it, _ = g.make_iteration(2, 1e-14, 0.0, 1.0, history=False)
         x.fill(0.0)
         assert fn(Ag, x, g.Vector(data=np.ones(n)), P, it) == g.MAX_ITER and it.iterations == 2
+
+
+MORE = [
+    # (method, param, right preconditioner?)
+    ("qmr", None, False),
+    ("qmr", None, True),
+    ("gmres", 1, False),
+    ("gmres", 8, False),
+    ("gmres", 4, True),
+    ("gmres_full", None, False),
+    ("bicgstab_2", None, False),
+    ("bicgstab_ell", 1, False),
+    ("bicgstab_ell", 4, False),
+    ("bicgstab_ell", 3, True),
+    ("idr_s", 1, False),
+    ("idr_s", 4, False),
+]
+
+
+def _run_gpu(g, method, param, Ag, x, bv, P, R, it):
+    if method == "qmr":
+        return g.qmr(Ag, x, bv, P, it, R=R)
+    if method == "gmres":
+        return g.gmres(Ag, x, bv, P, it, restart=param, R=R)
+    if method == "gmres_full":
+        return g.gmres_full(Ag, x, bv, P, it, R=R)
+    if method == "bicgstab_2":
+        return g.bicgstab_2(Ag, x, bv, P, it)
+    if method == "bicgstab_ell":
+        return g.bicgstab_ell(Ag, x, bv, P, it, l=param, R=R)
+    if method == "idr_s":
+        return g.idr_s(Ag, x, bv, it, s=param, L=P, R=R)
+    raise ValueError(method)
+
+
+def _oracle_spread(A, b, method, okind, right, param, max_iter, rtol, runs=6):
+    """The oracle on b and on b * (1 + 1e-15 * noise) (runs - 1 draws): how
+    far the reference's own outcome moves under rounding-level changes."""
+    out = []
+    for k in range(runs):
+        bb = b if k == 0 else b * (1.0 + 1e-15 * np.random.default_rng(k).standard_normal(A.n))
+        o.lib.orc_set_random_counter(0)
+        out.append(o.solve(method, A, bb, pc=okind, max_iter=max_iter, rtol=rtol, history=False,
+                           right=okind if right else None, param=param))
+    return out
+
+
+@pytest.mark.parametrize("method,param,right", MORE, ids=[f"{m}-{p}-{'R' if r else 'L'}" for m, p, r in MORE])
+@pytest.mark.parametrize("case", [c[0] for c in CASES])
+def test_more_solvers_match_oracle(g, method, param, right, case):
+    # Iteration counts of BiCGStab(l), IDR(s) and MTL4's GMRES move by several
+    # steps when b changes by 1e-15 (e.g. IDR(4) on vc5: 208-259), so no
+    # differently ordered dot can hold a fixed 2 % to one oracle run.  The bar
+    # is the oracle's own spread over six such runs: status among theirs,
+    # iterations within [min, max] +- 2 %, and x within 1e-6 (or 20x their x
+    # spread) where that spread is below 1e-3; beyond that (GMRES(8),
+    # gmres_full, IDR(1)) x is chaotic and not compared.
+    name, make, pcs = next(c for c in CASES if c[0] == case)
+    S, I, V = make(g)
+    A = o.CSR(S, I, V)
+    n = A.n
+    b = g.rng_uniform(9, 1, n)
+    nb = o.two_norm(b)
+    rtol = 1e-9
+    max_iter = 80 if method in ("qmr", "gmres", "gmres_full") else 400
+    Ag = g.Matrix(S, I, V)
+    for pc in pcs:
+        print(f"{method}({param}) {name} {pc} right={right}", flush=True)
+        gname, okind = PCS[pc]
+        runs = _oracle_spread(A, b, method, okind, right, param, max_iter, rtol)
+        ref = runs[0]
+        P = g.PC(Ag, getattr(g, gname))
+        x = g.Vector(n)
+        x.fill(0.0)
+        it, _ = g.make_iteration(max_iter, rtol, 0.0, nb, history=False)
+        g.random_seed_counter(0)
+        st = _run_gpu(g, method, param, Ag, x, g.Vector(data=b), P, P if right else None, it)
+        xs = x.numpy()
+        if st == g.OK and method in ("gmres", "gmres_full", "idr_s"):
+            # these test the true residual b - A x before they stop
+            assert o.two_norm(o.residual(A, xs, b)) <= 20 * rtol * nb
+        scale = max(1.0, np.max(np.abs(ref["x"])))
+        spread = max(np.max(np.abs(r["x"] - ref["x"])) for r in runs[1:])
+        if not (np.all(np.isfinite(ref["x"])) and spread <= 1e-3 * scale):
+            # chaotic in the reference itself (e.g. IDR(1) on the Laplacian
+            # converges for about 1 in 5 such perturbations): the GPU's outcome
+            # must be one the method can produce, stated honestly
+            print(f"  chaotic: the oracle's x moves by {spread:.1e}; GPU status {st}, {it.iterations} its")
+            assert st in (g.OK, g.MAX_ITER, g.BREAKDOWN)
+            if st == g.MAX_ITER:
+                assert it.iterations == max_iter
+            continue
+        to_status = lambda e: {0: g.OK, 1: g.MAX_ITER}.get(e, g.BREAKDOWN)
+        assert st in {to_status(r["status"]) for r in runs}, (st, [r["status"] for r in runs])
+        if st == g.BREAKDOWN:
+            assert it.error in {r["status"] for r in runs}
+        its = [r["iterations"] for r in runs]
+        slack = max(1, int(np.ceil(0.02 * max(its))))
+        assert min(its) - slack <= it.iterations <= max(its) + slack, (it.iterations, its)
+        tol = max(1e-6 * scale, 20 * spread)
+        assert np.max(np.abs(xs - ref["x"])) <= tol, (np.max(np.abs(xs - ref["x"])), tol)
+
+
+def test_idr_s_seed_counter(g):
+    # random(P) advances mtl::seed<double>::counter once per column
+    S, I, V = g.laplacian2d(10, 10)
+    Ag = g.Matrix(S, I, V)
+    n = len(S) - 1
+    b = np.ones(n)
+    g.random_seed_counter(5)
+    x = g.Vector(n)
+    x.fill(0.0)
+    it, _ = g.make_iteration(300, 1e-8, 0.0, o.two_norm(b), history=False)
+    assert g.idr_s(Ag, x, g.Vector(data=b), it, s=4) == 0
+    assert g.random_seed_counter() == 9
+
+
+def test_more_solvers_argument_errors(g):
+    S, I, V = g.laplacian2d(6, 6)
+    Ag = g.Matrix(S, I, V)
+    n = len(S) - 1
+    x, bv = g.Vector(n), g.Vector(data=np.ones(n))
+    it, _ = g.make_iteration(10, 1e-8, 0.0, 1.0, history=False)
+    with pytest.raises(g.M4GError) as e:
+        g.gmres(Ag, x, bv, None, it, restart=0)
+    assert e.value.status == g.ERR_INVALID
+    with pytest.raises(g.M4GError) as e:
+        g.bicgstab_ell(Ag, x, bv, None, it, l=0)
+    assert e.value.status == g.ERR_INVALID

@@ -398,3 +398,80 @@ def test_more_krylov_finite_termination():
     for meth, pc in (("bicg", o.PC_IDENTITY), ("cgs", o.PC_ILU0), ("tfqmr", o.PC_ILU0)):
         r = o.solve(meth, A, b, pc=pc, max_iter=150, rtol=1e-4, x0=np.full(16, 0.01))
         assert r["status"] == 0, meth
+
+
+def _nilpotent_plus_identity(n=40, h=20, seed=2):
+    rng = np.random.default_rng(seed)
+    S, I, V = [0], [], []
+    for i in range(n):
+        row = [(i, 1.0)]
+        if i < h:
+            row += [(int(j), float(rng.uniform(-1, 1))) for j in rng.choice(np.arange(h, n), 3, replace=False)]
+        row.sort()
+        I += [c for c, _ in row]
+        V += [v for _, v in row]
+        S.append(len(I))
+    return o.CSR(np.array(S, np.int64), np.array(I, np.int64), np.array(V))
+
+
+def test_qmr_idr_finite_termination():
+    # QMR's Lanczos process ends after m steps for a minimal polynomial of
+    # degree m: the next Lanczos vector is 0, so rho = 0 passes the test and x
+    # is exact.  MTL4's IDR(s) (s steps per cycle) ends within m + 1 steps here.
+    d = np.repeat([1.0, 2.0, 5.0], 10)
+    D = o.CSR(np.arange(31, dtype=np.int64), np.arange(30, dtype=np.int64), d)
+    N = _nilpotent_plus_identity()
+    rng = np.random.default_rng(5)
+    for A, m in ((D, 3), (N, 2)):
+        b = rng.uniform(-1, 1, A.n)
+        r = o.solve("qmr", A, b, pc=0, max_iter=40, rtol=1e-12)
+        assert r["status"] == 0 and r["iterations"] == m
+        assert o.two_norm(o.residual(A, r["x"], b)) <= 1e-12 * o.two_norm(b)
+    b = rng.uniform(-1, 1, N.n)
+    r = o.solve("idr_s", N, b, pc=0, max_iter=40, rtol=1e-12, param=2)
+    assert r["status"] == 0 and r["iterations"] <= 3
+    assert o.two_norm(o.residual(N, r["x"], b)) <= 1e-12 * o.two_norm(b)
+
+
+def test_gmres_known_answers():
+    # gmres_test.cpp: 2x2-grid Laplacian, b = 1, x = 5, one GMRES(1) step with
+    # basic_iteration(b, 1, 1e-8): r0 is an eigenvector, so the step is exact
+    A = o.laplacian_setup(2, 2)
+    b = np.ones(4)
+    r = o.solve("gmres", A, b, x0=np.full(4, 5.0), pc=0, max_iter=1, rtol=1e-8, param=1)
+    assert o.two_norm(o.residual(A, r["x"], b)) <= 5e-5
+    # one GMRES(1) cycle is the minimal-residual step x + ((r.Ar)/(Ar.Ar)) r
+    N = _nilpotent_plus_identity()
+    rng = np.random.default_rng(7)
+    b = rng.uniform(-1, 1, N.n)
+    x0 = rng.uniform(-1, 1, N.n)
+    r0 = o.residual(N, x0, b)
+    Ar = o.spmv(N, r0)
+    expect = x0 + (np.dot(r0, Ar) / np.dot(Ar, Ar)) * r0
+    r = o.solve("gmres", N, b, x0=x0, pc=0, max_iter=1, rtol=1e-14, param=1)
+    assert r["iterations"] == 1
+    np.testing.assert_allclose(r["x"], expect, rtol=1e-13, atol=1e-14)
+    # the restarted driver stops at the first cycle whose true residual passes
+    r = o.solve("gmres", N, b, pc=0, max_iter=200, rtol=1e-10, param=1)
+    assert r["status"] == 0 and r["resid"] <= 1e-10 * o.two_norm(b)
+    assert abs(o.two_norm(o.residual(N, r["x"], b)) - r["resid"]) <= 1e-12 * o.two_norm(b)
+
+
+@pytest.mark.parametrize("method,param", [("bicgstab_2", None), ("bicgstab_ell", 1), ("bicgstab_ell", 3),
+                                          ("idr_s", 2), ("idr_s", 4)])
+def test_ell_idr_residual_consistency(method, param):
+    # x and the recurrence residual must agree (identity PC, so r_hat[0] is
+    # b - A x).  BiCGStab(l) stopped by the iteration limit leaves at its first
+    # inner test of iteration k, before that iteration touches x
+    # (bicgstab_ell.hpp:87-99), so ||b - A x|| is the norm tested at the end of
+    # iteration k-1; IDR(s) updates x and r before each test.  A dropped or
+    # mis-signed term in the MR or IDR updates breaks this.
+    A = o.laplacian_setup(12, 12)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, A.n)
+    for k in (2, 3, 5):
+        r = o.solve(method, A, b, pc=0, max_iter=k, rtol=1e-30, param=param)
+        assert r["status"] == 1 and r["iterations"] == k
+        true = o.two_norm(o.residual(A, r["x"], b))
+        tested = r["resid"] if method == "idr_s" else r["history"][k - 1]
+        assert abs(true - tested) <= 1e-10 * o.two_norm(b), (k, true, tested)
