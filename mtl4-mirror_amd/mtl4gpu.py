@@ -137,6 +137,18 @@ _SIGS = {
     "m4g_bicgstab_ell": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
     "m4g_idr_s": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
     "m4g_random_seed_counter": (ctypes.c_int, []),
+    # alternative formats (mtl4gpu_formats.h)
+    "m4g_ell_create": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _c_i64, _vp, _vp, _vp]),
+    "m4g_ell_destroy": (ctypes.c_int, [_vp]),
+    "m4g_ell_shape": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i64p, _c_i64p, _c_i64p]),
+    "m4g_ell_download": (ctypes.c_int, [_vp, _vp, _vp]),
+    "m4g_ell_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "m4g_banded_create": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _c_i64, _c_i64, _vp, _vp]),
+    "m4g_banded_from_csr": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _c_i64, _vp, _vp, _vp]),
+    "m4g_banded_destroy": (ctypes.c_int, [_vp]),
+    "m4g_banded_shape": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i64p]),
+    "m4g_banded_download": (ctypes.c_int, [_vp, _vp, _vp]),
+    "m4g_banded_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "m4g_set_random_seed_counter": (None, [ctypes.c_int]),
     "m4g_mm_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "m4g_batch_cg_ic0": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -221,6 +233,92 @@ def set_stream(stream_ptr):
 # ---------------------------------------------------------------------------
 # handles
 # ---------------------------------------------------------------------------
+class Ell:
+    """Device-resident mtl::mat::ell_matrix<double> built from a sorted CSR
+    (the ell_matrix_inserter's layout)."""
+
+    def __init__(self, starts, indices, values, ncols=None):
+        S = np.ascontiguousarray(starts, np.int64)
+        I = np.ascontiguousarray(indices, np.int64)
+        V = np.ascontiguousarray(values, np.float64)
+        self.nrows = len(S) - 1
+        self.ncols = self.nrows if ncols is None else int(ncols)
+        h = ctypes.c_void_p()
+        _check(_f["m4g_ell_create"](ctypes.byref(h), self.nrows, self.ncols, _ptr(S), _ptr(I), _ptr(V)),
+               "m4g_ell_create")
+        self.h = h
+        a = [ctypes.c_int64() for _ in range(5)]
+        _check(_f["m4g_ell_shape"](self.h, *[ctypes.byref(x) for x in a]), "m4g_ell_shape")
+        self.slots, self.stride, self.nnz = a[2].value, a[3].value, a[4].value
+
+    def download(self):
+        """(ref_minor(), ref_data()): stride * slots entries each."""
+        n = self.stride * self.slots
+        I = np.zeros(n, np.int64)
+        D = np.zeros(n, np.float64)
+        _check(_f["m4g_ell_download"](self.h, _ptr(I), _ptr(D)), "m4g_ell_download")
+        return I, D
+
+    def spmv(self, v, w, mode=ASSIGN):
+        _check(_f["m4g_ell_spmv"](self.h, v.h, w.h, mode), "m4g_ell_spmv")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _f["m4g_ell_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Banded:
+    """Device-resident mtl::mat::sparse_banded<double>: from a CSR (the
+    inserter) or, with bands=..., from the reference layout data[r * nb + b]."""
+
+    def __init__(self, starts=None, indices=None, values=None, nrows=None, ncols=None, bands=None, data=None):
+        h = ctypes.c_void_p()
+        if bands is not None:
+            B = np.ascontiguousarray(bands, np.int64)
+            D = np.ascontiguousarray(data, np.float64)
+            _check(_f["m4g_banded_create"](ctypes.byref(h), int(nrows), int(ncols), len(B), _ptr(B), _ptr(D)),
+                   "m4g_banded_create")
+        else:
+            S = np.ascontiguousarray(starts, np.int64)
+            I = np.ascontiguousarray(indices, np.int64)
+            V = np.ascontiguousarray(values, np.float64)
+            nr = len(S) - 1
+            _check(_f["m4g_banded_from_csr"](ctypes.byref(h), nr, nr if ncols is None else int(ncols), _ptr(S),
+                                             _ptr(I), _ptr(V)), "m4g_banded_from_csr")
+        self.h = h
+        a = [ctypes.c_int64() for _ in range(3)]
+        _check(_f["m4g_banded_shape"](self.h, *[ctypes.byref(x) for x in a]), "m4g_banded_shape")
+        self.nrows, self.ncols, self.nbands = a[0].value, a[1].value, a[2].value
+
+    def download(self):
+        """(ref_bands(), data) in the reference's row-major layout."""
+        B = np.zeros(self.nbands, np.int64)
+        D = np.zeros(self.nrows * self.nbands, np.float64)
+        _check(_f["m4g_banded_download"](self.h, _ptr(B), _ptr(D)), "m4g_banded_download")
+        return B, D
+
+    def spmv(self, v, w, mode=ASSIGN):
+        _check(_f["m4g_banded_spmv"](self.h, v.h, w.h, mode), "m4g_banded_spmv")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _f["m4g_banded_destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Matrix:
     """Device-resident compressed2D built from host CSR arrays (copied)."""
 

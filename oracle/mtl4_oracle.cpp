@@ -1094,6 +1094,141 @@ int orc_bicgstab_ell(idx_t n, const idx_t* S, const idx_t* I, const double* D, d
     return it->error;
 }
 
+// ---- ell_matrix and sparse_banded -----------------------------------------
+
+// ell_matrix_inserter's conversion (mtl/matrix/ell_matrix.hpp:146-176) of a
+// sorted CSR: stride = num_rows rounded up to 32 (set_stride, :43-44), slots =
+// longest row; slot i of row r at r + i * stride; padding repeats the row's
+// last column index (0 for an empty row) with value 0.  With indices == NULL
+// only slots and stride are returned.
+void orc_ell_layout(idx_t nrows, const idx_t* S, const idx_t* I, const double* D, idx_t* slots_out,
+                    idx_t* stride_out, idx_t* indices, double* data)
+{
+    const idx_t stride = (nrows + 31) / 32 * 32;
+    idx_t slots = 0;
+    for (idx_t r = 0; r < nrows; r++) slots = std::max(slots, S[r + 1] - S[r]);
+    *slots_out = slots;
+    *stride_out = stride;
+    if (!indices) return;
+    for (idx_t r = 0; r < nrows; r++) {
+        idx_t k = r, patch_index = 0;
+        for (idx_t j = S[r]; j < S[r + 1]; ++j, k += stride) {
+            patch_index = indices[k] = I[j];
+            data[k] = D[j];
+        }
+        for (idx_t j = 0; j < slots - (S[r + 1] - S[r]); ++j, k += stride) {
+            indices[k] = patch_index;
+            data[k] = 0.0;
+        }
+    }
+}
+
+// smat_cvec_mult(ell_matrix) (mtl/operation/mat_vec_mult.hpp:692-704);
+// mode 0 assign, 1 +=, 2 -= (Assign::first_update).
+void orc_ell_spmv(idx_t nrows, idx_t stride, idx_t slots, const idx_t* indices, const double* data,
+                  const double* v, double* w, int mode)
+{
+    for (idx_t r = 0; r < nrows; ++r) {
+        double s = 0.0;
+        for (idx_t k = r, i = 0; i < slots; ++i, k += stride) s += data[k] * v[indices[k]];
+        if (mode == 0) w[r] = s;
+        else if (mode == 1) w[r] += s;
+        else w[r] -= s;
+    }
+}
+
+// sparse_banded_inserter (mtl/matrix/sparse_banded.hpp:168-258) fed with a
+// CSR's entries in row order: a new offset is inserted at its sorted place
+// (find_dia :150-156, insert_dia :188-198), update_store keeps the last value,
+// the destructor lays the diagonals out row-major (:210-219).  With bands ==
+// NULL only the band count is returned.
+void orc_banded_layout(idx_t nrows, const idx_t* S, const idx_t* I, const double* D, idx_t* nb_out, idx_t* bands,
+                       double* data)
+{
+    std::vector<idx_t> bs;
+    std::vector<std::vector<double>> diagonals;
+    for (idx_t r = 0; r < nrows; r++)
+        for (idx_t j = S[r]; j < S[r + 1]; j++) {
+            const idx_t dia = I[j] - r;
+            size_t b = 0;
+            while (b < bs.size() && dia > bs[b]) b++;
+            if (b == bs.size() || dia != bs[b]) {
+                diagonals.push_back(std::vector<double>(size_t(nrows), 0.0));
+                bs.push_back(dia);
+                for (size_t i = diagonals.size() - 1; i > b; i--) {
+                    std::swap(diagonals[i - 1], diagonals[i]);
+                    std::swap(bs[i - 1], bs[i]);
+                }
+            }
+            diagonals[b][size_t(r)] = D[j];
+        }
+    *nb_out = idx_t(bs.size());
+    if (!bands) return;
+    for (size_t b = 0; b < bs.size(); b++) bands[b] = bs[b];
+    double* p = data;
+    for (idx_t r = 0; r < nrows; r++)
+        for (size_t b = 0; b < bs.size(); b++) *p++ = diagonals[b][size_t(r)];
+}
+
+// smat_cvec_mult(sparse_banded) (mtl/operation/mat_vec_mult.hpp:709-809),
+// the reference's segment loop: rows [from, to) share the band range
+// [begin_pos, end_pos]; each row sums those bands from 0 in ascending order.
+// Leading rows with no band and rows past the lowest band's end are set to 0
+// by an assignment and not touched by += / -=.  (With += / -= and only
+// strictly lower bands the reference reads begin_rows[nb]; this restatement
+// steps back to the last band there, as the assignment branch does.)
+void orc_banded_spmv(idx_t nr, idx_t nc, idx_t nb, const idx_t* bands, const double* data, const double* v,
+                     double* w, int mode)
+{
+    const bool init_to_zero = mode == 0;
+    auto update = [&](idx_t r, double t) {
+        if (mode == 0) w[r] = t;
+        else if (mode == 1) w[r] += t;
+        else w[r] -= t;
+    };
+    if (nr == 0) return;
+    if (nb == 0) {
+        if (init_to_zero)
+            for (idx_t r = 0; r < nr; r++) w[r] = 0.0;
+        return;
+    }
+    std::vector<idx_t> begin_rows(static_cast<size_t>(nb)), end_rows(static_cast<size_t>(nb));
+    for (idx_t b = 0; b < nb; b++) {
+        begin_rows[size_t(b)] = std::max<idx_t>(0, -bands[b]);
+        end_rows[size_t(b)] = std::min<idx_t>(nr, nc - bands[b]);
+    }
+    idx_t begin_pos = 0, end_pos = nb - 1;
+    while (begin_pos < nb && begin_rows[size_t(begin_pos)] > 0) begin_pos++;
+    if (begin_pos == nb) {
+        --begin_pos;
+        if (init_to_zero)
+            for (idx_t r = 0; r < begin_rows[size_t(begin_pos)]; r++) w[r] = 0.0;
+    }
+    idx_t from = begin_rows[size_t(begin_pos)];
+    while (begin_pos > 0 && begin_rows[size_t(begin_pos - 1)] == from) begin_pos--;
+    for (bool active = true; active;) {
+        const idx_t to = begin_pos > 0 && begin_rows[size_t(begin_pos - 1)] <= end_rows[size_t(end_pos)]
+                             ? begin_rows[size_t(begin_pos - 1)]
+                             : end_rows[size_t(end_pos)];
+        for (idx_t r = from; r < to; r++) {
+            double tmp = 0.0;
+            const double* Ap = data + r * nb + begin_pos;
+            for (idx_t b = begin_pos; b <= end_pos; ++b, ++Ap) tmp += *Ap * v[r + bands[b]];
+            update(r, tmp);
+        }
+        if (begin_pos > 0) {
+            if (begin_rows[size_t(begin_pos - 1)] == to) begin_pos--;
+            if (end_rows[size_t(end_pos)] == to) end_pos--;
+        } else {
+            if (end_rows[0] == to) active = false;
+            else end_pos--;
+        }
+        from = to;
+    }
+    if (end_rows[0] < nr && init_to_zero)
+        for (idx_t r = end_rows[0]; r < nr; r++) w[r] = 0.0;
+}
+
 // mtl::seed<double>::counter (mtl/operation/random.hpp:29-38): every
 // random(vector) call reseeds the C library generator with 17 * ++counter and
 // fills the vector with rand() values.

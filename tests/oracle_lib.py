@@ -75,6 +75,12 @@ def _load():
                                             ctypes.POINTER(PC), ctypes.POINTER(Iteration), ctypes.c_int]),
         "orc_idr_s": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(Iteration), ctypes.c_int]),
         "orc_set_random_counter": (None, [ctypes.c_int]),
+        "orc_ell_layout": (None, [_i64, _i64p, _i64p, _f64p, ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p]),
+        "orc_ell_spmv": (None, [_i64, _i64, _i64, _i64p, _f64p, _f64p, _f64p, ctypes.c_int]),
+        "orc_banded_layout": (None, [_i64, _i64p, _i64p, _f64p, ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+        "orc_banded_spmv": (None, [_i64, _i64, _i64, _i64p, _f64p, _f64p, _f64p, ctypes.c_int]),
         "orc_get_random_counter": (ctypes.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -296,3 +302,39 @@ def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0,
         st = f(A.n, A.starts, A.indices, A.data, x, b, ctypes.byref(P), ctypes.byref(it))
     del keep, keep_r
     return dict(x=x, status=st, iterations=it.i, resid=it.resid, history=hist[: it.i + 1])
+
+
+def ell_layout(A):
+    """ell_matrix_inserter's layout of CSR A: (slots, stride, indices, data)."""
+    slots, stride = ctypes.c_int64(), ctypes.c_int64()
+    lib.orc_ell_layout(A.n, A.starts, A.indices, A.data, ctypes.byref(slots), ctypes.byref(stride), None, None)
+    total = slots.value * stride.value
+    I = np.zeros(total, np.int64)
+    D = np.zeros(total, np.float64)
+    lib.orc_ell_layout(A.n, A.starts, A.indices, A.data, ctypes.byref(slots), ctypes.byref(stride),
+                       I.ctypes.data, D.ctypes.data)
+    return slots.value, stride.value, I, D
+
+
+def ell_spmv(nrows, slots, stride, I, D, v, w=None, mode=0):
+    w = np.zeros(nrows) if w is None else np.array(w, np.float64)
+    lib.orc_ell_spmv(nrows, stride, slots, np.ascontiguousarray(I, np.int64), np.ascontiguousarray(D, np.float64),
+                     np.ascontiguousarray(v, np.float64), w, mode)
+    return w
+
+
+def banded_layout(A):
+    """sparse_banded_inserter's layout of CSR A: (bands, data[r * nb + b])."""
+    nb = ctypes.c_int64()
+    lib.orc_banded_layout(A.n, A.starts, A.indices, A.data, ctypes.byref(nb), None, None)
+    B = np.zeros(nb.value, np.int64)
+    D = np.zeros(A.n * nb.value, np.float64)
+    lib.orc_banded_layout(A.n, A.starts, A.indices, A.data, ctypes.byref(nb), B.ctypes.data, D.ctypes.data)
+    return B, D
+
+
+def banded_spmv(nrows, ncols, B, D, v, w=None, mode=0):
+    w = np.zeros(nrows) if w is None else np.array(w, np.float64)
+    lib.orc_banded_spmv(nrows, ncols, len(B), np.ascontiguousarray(B, np.int64), np.ascontiguousarray(D, np.float64),
+                        np.ascontiguousarray(v, np.float64), w, mode)
+    return w

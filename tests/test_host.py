@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(g.LIB_PATH)
     names = set()
-    for h in ("mtl4gpu.h", "mtl4gpu_analysis.h", "mtl4gpu_gen.h", "mtl4gpu_io.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(m4g_\w+)\s*\(", text))
