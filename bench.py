@@ -316,6 +316,24 @@ def main():
     padded = int(rl.reshape(nslc, 64).max(axis=1).sum()) * 64
     spmv_moved = 8 * padded + 2 * padded + 16 * n + 16 * nslc
 
+    # the same matrix as ell_matrix and as sparse_banded (7 diagonals): each
+    # product's own algorithmic bytes (stored values and indices, x and w
+    # once) and the time against the CSR bytes of the same product
+    fmt = {}
+    for name, make, fbytes in (
+            ("ell", lambda: g.Ell(S, I, V), lambda m: 12 * m.slots * n + 16 * n),
+            ("banded", lambda: g.Banded(S, I, V), lambda m: 8 * m.nbands * n + 16 * n + 8 * m.nbands)):
+        M = make()
+        fw = time_launches(timer, lambda: M.spmv(xt, y), 100)
+        ff = time_launches(timer, lambda: M.spmv(xt, y), 30, flush=lambda: g.two_norm(flushbuf))
+        b = fbytes(M)
+        fmt[name] = {"bytes": b, "warm_ms_median": float(np.median(fw)), "flushed_ms_median": float(np.median(ff)),
+                     "flushed_gbs": round(b / (np.median(ff) * 1e-3) / 1e9, 1),
+                     "flushed_frac": round(b / (np.median(ff) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "csr_bytes_flushed_gbs": round(spmv_bytes / (np.median(ff) * 1e-3) / 1e9, 1)}
+        M.close()
+    g.spmv(A, xt, y)
+
     e_factor = (nnz + n) // 2                                     # entries of L (= U^T), diagonal included
     sweep_bytes = 12 * e_factor + 4 * (n + 1) + 16 * n
     P.apply_async(xt, y)                                          # first use allocates the PC's temporary
@@ -396,6 +414,7 @@ def main():
                  "flushed_gbs": round(spmv_flushed_gbs, 1), "flushed_frac": round(spmv_flushed_gbs / HBM_PEAK_GBS, 4),
                  "index_format": "int16 column - row per entry (SELL-64 slices)", "moved_bytes": spmv_moved,
                  "moved_flushed_gbs": round(spmv_moved / (np.median(flushed) * 1e-3) / 1e9, 1)},
+        "spmv_formats": fmt,
         "sweep": {"bytes": sweep_bytes, "ms": sweep_ms, "gbs": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1),
                   "tiles": pcstats["tiles_lower"], "steps": pcstats["steps_lower"],
                   "tile": [pcstats["tile_nx"], pcstats["tile_ny"], pcstats["tile_nz"]]},

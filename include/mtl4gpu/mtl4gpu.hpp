@@ -20,6 +20,7 @@
 #define MTL4GPU_HPP
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -35,6 +36,7 @@
 #include "../mtl4gpu.h"
 #include "../mtl4gpu_gen.h"
 #include "../mtl4gpu_io.h"
+#include "../mtl4gpu_formats.h"
 
 namespace mtl {
 
@@ -229,21 +231,21 @@ class dense_vector {
     dense_vector& operator=(const mat_cvec_times_expr<Matrix, dense_vector>& e)
     {
         checked_change_resource(num_rows(e.A));
-        check(m4g_spmv(e.A.handle(), e.v.dev(), dev(), M4G_ASSIGN), "A * v");
+        check(e.A.mult_device(e.v.dev(), dev(), M4G_ASSIGN), "A * v");
         state_ = device_valid;
         return *this;
     }
     template <typename Matrix>
     dense_vector& operator+=(const mat_cvec_times_expr<Matrix, dense_vector>& e)
     {
-        check(m4g_spmv(e.A.handle(), e.v.dev(), dev(), M4G_PLUS), "+= A * v");
+        check(e.A.mult_device(e.v.dev(), dev(), M4G_PLUS), "+= A * v");
         state_ = device_valid;
         return *this;
     }
     template <typename Matrix>
     dense_vector& operator-=(const mat_cvec_times_expr<Matrix, dense_vector>& e)
     {
-        check(m4g_spmv(e.A.handle(), e.v.dev(), dev(), M4G_MINUS), "-= A * v");
+        check(e.A.mult_device(e.v.dev(), dev(), M4G_MINUS), "-= A * v");
         state_ = device_valid;
         return *this;
     }
@@ -557,6 +559,32 @@ class compressed2D {
         if (!h_) const_cast<compressed2D*>(this)->upload();
         return h_;
     }
+    // w (=, +=, -=) A * v on the device
+    int mult_device(m4g_vector v, m4g_vector w, int mode) const { return m4g_spmv(handle(), v, w, mode); }
+
+    // the host rows, as the inserter and the format conversions read them
+    const std::vector<int64_t>& host_starts() const { return starts_; }
+    const std::vector<int64_t>& host_indices() const { return indices_; }
+    const std::vector<Value>& host_values() const { return data_; }
+
+    // copy from another matrix type of this library (ell_matrix, sparse_banded)
+    template <typename Src, typename = decltype(std::declval<const Src&>().host_starts()),
+              typename = typename std::enable_if<!std::is_same<Src, compressed2D>::value>::type>
+    explicit compressed2D(const Src& src) : nrows_(src.num_rows()), ncols_(src.num_cols())
+    {
+        assign_csr(src.host_starts(), src.host_indices(),
+                   std::vector<Value>(src.host_values().begin(), src.host_values().end()));
+    }
+    template <typename Src, typename = decltype(std::declval<const Src&>().host_starts()),
+              typename = typename std::enable_if<!std::is_same<Src, compressed2D>::value>::type>
+    compressed2D& operator=(const Src& src)
+    {
+        nrows_ = src.num_rows();
+        ncols_ = src.num_cols();
+        assign_csr(src.host_starts(), src.host_indices(),
+                   std::vector<Value>(src.host_values().begin(), src.host_values().end()));
+        return *this;
+    }
 
   private:
     void release()
@@ -582,8 +610,266 @@ template <typename V> inline std::size_t num_rows(const compressed2D<V>& A) { re
 template <typename V> inline std::size_t num_cols(const compressed2D<V>& A) { return A.num_cols(); }
 template <typename V> inline std::size_t size(const compressed2D<V>& A) { return A.num_rows() * A.num_cols(); }
 
+namespace detail {
+// Host rows of a matrix kept beside its device format: what the inserter
+// reads and rebuilds, and what element access and conversions use.
+template <typename Value>
+struct host_rows {
+    std::size_t nrows_ = 0, ncols_ = 0;
+    std::vector<int64_t> starts_{0}, indices_;
+    std::vector<Value> values_;
+    void reset(std::size_t m, std::size_t n)
+    {
+        nrows_ = m;
+        ncols_ = n;
+        starts_.assign(m + 1, 0);
+        indices_.clear();
+        values_.clear();
+    }
+    Value at(std::size_t r, std::size_t c) const
+    {
+        auto b = indices_.begin() + starts_[r], e = indices_.begin() + starts_[r + 1];
+        auto p = std::lower_bound(b, e, int64_t(c));
+        return (p != e && *p == int64_t(c)) ? values_[p - indices_.begin()] : Value(0);
+    }
+};
+}  // namespace detail
+
+// ell_matrix<Value> (mtl/matrix/ell_matrix.hpp:31-177): row-major ELLPACK with
+// the reference's slot layout, held on the device (mtl4gpu_formats.h).
+template <typename Value = double>
+class ell_matrix {
+    static_assert(std::is_same<Value, double>::value, "ell_matrix holds double values on the device");
+
+  public:
+    typedef Value value_type;
+    typedef std::size_t size_type;
+    static const unsigned alignment = 32;
+
+    ell_matrix() = default;
+    ell_matrix(size_type m, size_type n) { rows_.reset(m, n); }
+    ell_matrix(const ell_matrix& o) : rows_(o.rows_) { upload(); }
+    template <typename Src, typename = decltype(std::declval<const Src&>().host_starts()),
+              typename = typename std::enable_if<!std::is_same<Src, ell_matrix>::value>::type>
+    explicit ell_matrix(const Src& src)
+    {
+        *this = src;
+    }
+    ell_matrix& operator=(const ell_matrix& o)
+    {
+        if (this != &o) {
+            rows_ = o.rows_;
+            upload();
+        }
+        return *this;
+    }
+    template <typename Src, typename = decltype(std::declval<const Src&>().host_starts()),
+              typename = typename std::enable_if<!std::is_same<Src, ell_matrix>::value>::type>
+    ell_matrix& operator=(const Src& src)
+    {
+        rows_.reset(src.num_rows(), src.num_cols());
+        assign_csr(src.host_starts(), src.host_indices(),
+                   std::vector<Value>(src.host_values().begin(), src.host_values().end()));
+        return *this;
+    }
+    ~ell_matrix() { release(); }
+
+    void change_dim(size_type m, size_type n)
+    {
+        if (m != num_rows() || n != num_cols()) {
+            rows_.reset(m, n);
+            upload();
+        }
+    }
+    void make_empty()
+    {
+        rows_.reset(num_rows(), num_cols());
+        upload();
+    }
+    void assign_csr(std::vector<int64_t> S, std::vector<int64_t> I, std::vector<Value> D)
+    {
+        rows_.starts_ = std::move(S);
+        rows_.indices_ = std::move(I);
+        rows_.values_ = std::move(D);
+        upload();
+    }
+
+    size_type num_rows() const { return rows_.nrows_; }
+    size_type num_cols() const { return rows_.ncols_; }
+    size_type nnz() const { return rows_.indices_.size(); }
+    size_type slots() const { return slots_; }
+    size_type stride() const { return stride_; }
+    // the slot layout (ref_minor / ref_data, ell_matrix.hpp:77-80)
+    const std::vector<size_type>& ref_minor() const { return minor_; }
+    const std::vector<value_type>& ref_data() const { return data_; }
+    Value operator()(size_type r, size_type c) const { return rows_.at(r, c); }
+    struct row_proxy {
+        const ell_matrix& A;
+        size_type r;
+        Value operator[](size_type c) const { return A(r, c); }
+    };
+    row_proxy operator[](size_type r) const { return {*this, r}; }
+
+    const std::vector<int64_t>& host_starts() const { return rows_.starts_; }
+    const std::vector<int64_t>& host_indices() const { return rows_.indices_; }
+    const std::vector<Value>& host_values() const { return rows_.values_; }
+    m4g_ell handle() const { return h_; }
+    int mult_device(m4g_vector v, m4g_vector w, int mode) const { return m4g_ell_spmv(h_, v, w, mode); }
+
+  private:
+    void release()
+    {
+        if (h_) m4g_ell_destroy(h_);
+        h_ = nullptr;
+    }
+    void upload()
+    {
+        release();
+        mtl::detail::throw_status(m4g_ell_create(&h_, int64_t(num_rows()), int64_t(num_cols()), rows_.starts_.data(),
+                                                 rows_.indices_.data(), rows_.values_.data()),
+                                  "ell_matrix");
+        int64_t sl = 0, st = 0;
+        m4g_ell_shape(h_, nullptr, nullptr, &sl, &st, nullptr);
+        slots_ = size_type(sl);
+        stride_ = size_type(st);
+        std::vector<int64_t> idx(slots_ * stride_);
+        data_.assign(slots_ * stride_, Value(0));
+        mtl::detail::throw_status(m4g_ell_download(h_, idx.data(), data_.data()), "ell_matrix");
+        minor_.assign(idx.begin(), idx.end());
+    }
+
+    detail::host_rows<Value> rows_;
+    size_type slots_ = 0, stride_ = 0;
+    std::vector<size_type> minor_;
+    std::vector<value_type> data_;
+    m4g_ell h_ = nullptr;
+};
+
+// sparse_banded<Value> (mtl/matrix/sparse_banded.hpp:38-258): the stored
+// diagonals of the matrix, held band-major on the device (mtl4gpu_formats.h).
+template <typename Value = double>
+class sparse_banded {
+    static_assert(std::is_same<Value, double>::value, "sparse_banded holds double values on the device");
+
+  public:
+    typedef Value value_type;
+    typedef std::size_t size_type;
+    typedef long band_size_type;
+
+    explicit sparse_banded(size_type m = 0, size_type n = 0) { rows_.reset(m, n); }
+    sparse_banded(const sparse_banded& o) : rows_(o.rows_) { upload(); }
+    template <typename Src, typename = decltype(std::declval<const Src&>().host_starts()),
+              typename = typename std::enable_if<!std::is_same<Src, sparse_banded>::value>::type>
+    explicit sparse_banded(const Src& src)
+    {
+        *this = src;
+    }
+    sparse_banded& operator=(const sparse_banded& o)
+    {
+        if (this != &o) {
+            rows_ = o.rows_;
+            upload();
+        }
+        return *this;
+    }
+    template <typename Src, typename = decltype(std::declval<const Src&>().host_starts()),
+              typename = typename std::enable_if<!std::is_same<Src, sparse_banded>::value>::type>
+    sparse_banded& operator=(const Src& src)
+    {
+        rows_.reset(src.num_rows(), src.num_cols());
+        assign_csr(src.host_starts(), src.host_indices(),
+                   std::vector<Value>(src.host_values().begin(), src.host_values().end()));
+        return *this;
+    }
+    ~sparse_banded() { release(); }
+
+    void change_dim(size_type m, size_type n)
+    {
+        rows_.reset(m, n);
+        upload();
+    }
+    void make_empty()
+    {
+        rows_.reset(num_rows(), num_cols());
+        upload();
+    }
+    void assign_csr(std::vector<int64_t> S, std::vector<int64_t> I, std::vector<Value> D)
+    {
+        rows_.starts_ = std::move(S);
+        rows_.indices_ = std::move(I);
+        rows_.values_ = std::move(D);
+        upload();
+    }
+
+    size_type num_rows() const { return rows_.nrows_; }
+    size_type num_cols() const { return rows_.ncols_; }
+    // bands times rows, the stored entries (sparse_banded.hpp:125)
+    size_type nnz() const { return bands_.size() * num_rows(); }
+    const std::vector<band_size_type>& ref_bands() const { return bands_; }
+    const value_type* ref_data() const { return data_.data(); }
+    Value operator()(size_type r, size_type c) const { return rows_.at(r, c); }
+    struct row_proxy {
+        const sparse_banded& A;
+        size_type r;
+        Value operator[](size_type c) const { return A(r, c); }
+    };
+    row_proxy operator[](size_type r) const { return {*this, r}; }
+
+    const std::vector<int64_t>& host_starts() const { return rows_.starts_; }
+    const std::vector<int64_t>& host_indices() const { return rows_.indices_; }
+    const std::vector<Value>& host_values() const { return rows_.values_; }
+    m4g_banded handle() const { return h_; }
+    int mult_device(m4g_vector v, m4g_vector w, int mode) const { return m4g_banded_spmv(h_, v, w, mode); }
+
+  private:
+    void release()
+    {
+        if (h_) m4g_banded_destroy(h_);
+        h_ = nullptr;
+    }
+    void upload()
+    {
+        release();
+        mtl::detail::throw_status(m4g_banded_from_csr(&h_, int64_t(num_rows()), int64_t(num_cols()),
+                                                      rows_.starts_.data(), rows_.indices_.data(),
+                                                      rows_.values_.data()),
+                                  "sparse_banded");
+        int64_t nb = 0;
+        m4g_banded_shape(h_, nullptr, nullptr, &nb);
+        std::vector<int64_t> b(static_cast<size_t>(nb));
+        data_.assign(size_t(nb) * num_rows(), Value(0));
+        mtl::detail::throw_status(m4g_banded_download(h_, b.data(), data_.data()), "sparse_banded");
+        bands_.assign(b.begin(), b.end());
+    }
+
+    detail::host_rows<Value> rows_;
+    std::vector<band_size_type> bands_;
+    std::vector<value_type> data_;
+    m4g_banded h_ = nullptr;
+};
+
+template <typename V> inline std::size_t num_rows(const ell_matrix<V>& A) { return A.num_rows(); }
+template <typename V> inline std::size_t num_cols(const ell_matrix<V>& A) { return A.num_cols(); }
+template <typename V> inline std::size_t size(const ell_matrix<V>& A) { return A.num_rows() * A.num_cols(); }
+template <typename V> inline std::size_t num_rows(const sparse_banded<V>& A) { return A.num_rows(); }
+template <typename V> inline std::size_t num_cols(const sparse_banded<V>& A) { return A.num_cols(); }
+template <typename V> inline std::size_t size(const sparse_banded<V>& A) { return A.num_rows() * A.num_cols(); }
+
 template <typename V>
 inline mat_cvec_times_expr<compressed2D<V>, vec::dense_vector<V>> operator*(const compressed2D<V>& A,
+                                                                            const vec::dense_vector<V>& v)
+{
+    return {A, v};
+}
+
+template <typename V>
+inline mat_cvec_times_expr<ell_matrix<V>, vec::dense_vector<V>> operator*(const ell_matrix<V>& A,
+                                                                          const vec::dense_vector<V>& v)
+{
+    return {A, v};
+}
+template <typename V>
+inline mat_cvec_times_expr<sparse_banded<V>, vec::dense_vector<V>> operator*(const sparse_banded<V>& A,
                                                                             const vec::dense_vector<V>& v)
 {
     return {A, v};
@@ -618,8 +904,8 @@ class inserter {
     explicit inserter(Matrix& A, std::size_t = 5) : A_(A)
     {
         for (std::size_t r = 0; r < num_rows(A); r++)
-            for (int64_t p = A.ref_major()[r]; p < A.ref_major()[r + 1]; p++)
-                rows_[{int64_t(r), A.ref_minor()[p]}] = A.data()[p];
+            for (int64_t p = A.host_starts()[r]; p < A.host_starts()[r + 1]; p++)
+                rows_[{int64_t(r), A.host_indices()[p]}] = A.host_values()[p];
     }
     ~inserter()
     {
@@ -670,6 +956,22 @@ inline void laplacian_setup(compressed2D<V>& A, unsigned m, unsigned n)
     A.assign_csr(std::move(S), std::move(I), std::vector<V>(D.begin(), D.end()));
 }
 
+// laplacian_setup for the alternative formats: the same rows
+template <typename V>
+inline void laplacian_setup(ell_matrix<V>& A, unsigned m, unsigned n)
+{
+    compressed2D<V> C;
+    laplacian_setup(C, m, n);
+    A = C;
+}
+template <typename V>
+inline void laplacian_setup(sparse_banded<V>& A, unsigned m, unsigned n)
+{
+    compressed2D<V> C;
+    laplacian_setup(C, m, n);
+    A = C;
+}
+
 // trans(A): transposed copy (rows sorted)
 template <typename V>
 inline compressed2D<V> trans(const compressed2D<V>& A)
@@ -705,6 +1007,8 @@ inline std::ostream& operator<<(std::ostream& os, const compressed2D<V>& A)
 }  // namespace mat
 
 using mat::compressed2D;
+using mat::ell_matrix;
+using mat::sparse_banded;
 using mat::inserter;
 using mat::laplacian_setup;
 using mat::num_cols;

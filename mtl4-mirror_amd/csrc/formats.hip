@@ -70,9 +70,14 @@ __global__ __launch_bounds__(256) void k_banded_spmv(int64_t nrows, int64_t ncol
     const int64_t lowest = bands[0], highest = bands[nbands - 1];
     for (int64_t r = blockIdx.x * int64_t(256) + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * 256) {
         double s = 0.0;
-        for (int b = 0; b < nbands; ++b) {
-            const int64_t c = r + bands[b];
-            if (c >= 0 && c < ncols) s += val[b * nrows + r] * x[c];
+        if (r + lowest >= 0 && r + highest < ncols) {
+            // interior row: every band is inside the matrix
+            for (int b = 0; b < nbands; ++b) s += val[b * nrows + r] * x[r + bands[b]];
+        } else {
+            for (int b = 0; b < nbands; ++b) {
+                const int64_t c = r + bands[b];
+                if (c >= 0 && c < ncols) s += val[b * nrows + r] * x[c];
+            }
         }
         if (MODE == M4G_ASSIGN || (r + highest >= 0 && r + lowest < ncols)) first_update<MODE>(w, r, s);
     }
