@@ -226,6 +226,20 @@ int m4g_idr_s(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_
 int m4g_random_seed_counter(void);
 void m4g_set_random_seed_counter(int counter);
 
+/* ---- assembly on the device (compressed2D_inserter semantics) -------------- */
+/* Updaters of mtl/operation/update.hpp:20-115: the first value of a position
+ * stores init(v) (v; -v for minus), each later one applies the operation. */
+enum { M4G_UPDATE_STORE = 0, M4G_UPDATE_PLUS = 1, M4G_UPDATE_MINUS = 2, M4G_UPDATE_TIMES = 3 };
+/* mat::inserter<compressed2D, Updater> (compressed2D.hpp:662-1151) fed with
+ * `count` triplets (rows[i], cols[i], values[i]) in insertion order, built on
+ * the device: entries of one position are folded in insertion order; with
+ * `existing` != NULL its entries are present before the first insertion (the
+ * inserter keeps a matrix's entries), and *out is a new matrix (existing is
+ * left unchanged).  on_device = 1: the three arrays are device pointers.
+ * Rows come out sorted; an index out of range returns M4G_ERR_INVALID. */
+int m4g_matrix_assemble(m4g_matrix* out, int64_t nrows, int64_t ncols, int64_t count, const int64_t* rows,
+                        const int64_t* cols, const double* values, int on_device, int updater, m4g_matrix existing);
+
 /* ---- differentiation of x = A(theta)^{-1} b --------------------------------- */
 /* No reference counterpart.  The adjoint of the solve is A^T lambda = dL/dx,
  * which runs through the same Krylov kernels on the transposed matrix (or on A

@@ -43,6 +43,7 @@ F64, F32 = 0, 1
 IDX_I32, IDX_I64, IDX_U32 = 0, 1, 2
 ASSIGN, PLUS, MINUS = 0, 1, 2
 PC_IDENTITY, PC_DIAGONAL, PC_IC0, PC_ILU0 = 0, 1, 2, 3
+UPDATE_STORE, UPDATE_PLUS, UPDATE_MINUS, UPDATE_TIMES = 0, 1, 2, 3
 METHOD_CG, METHOD_BICGSTAB = 0, 1
 
 _c_i64 = ctypes.c_int64
@@ -137,6 +138,8 @@ _SIGS = {
     "m4g_bicgstab_ell": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
     "m4g_idr_s": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Iteration), _c_i64]),
     "m4g_random_seed_counter": (ctypes.c_int, []),
+    "m4g_matrix_assemble": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, ctypes.c_int,
+                                           ctypes.c_int, _vp]),
     # alternative formats (mtl4gpu_formats.h)
     "m4g_ell_create": (ctypes.c_int, [ctypes.POINTER(_vp), _c_i64, _c_i64, _vp, _vp, _vp]),
     "m4g_ell_destroy": (ctypes.c_int, [_vp]),
@@ -370,6 +373,31 @@ class Matrix:
             self.close()
         except Exception:
             pass
+
+
+def assemble(nrows, ncols, rows, cols, values, updater=UPDATE_STORE, existing=None):
+    """mat::inserter<compressed2D, Updater> on the device: the triplets (numpy
+    arrays, or int64/float64 torch tensors on the GPU) folded in insertion order
+    into a new Matrix, starting from the entries of `existing`."""
+    h = ctypes.c_void_p()
+    if hasattr(values, "data_ptr"):
+        count = int(values.numel())
+        ptrs = [ctypes.c_void_p(t.data_ptr()) for t in (rows, cols, values)]
+        on_device = 1
+    else:
+        arrs = (np.ascontiguousarray(rows, np.int64), np.ascontiguousarray(cols, np.int64),
+                np.ascontiguousarray(values, np.float64))
+        count = len(arrs[2])
+        ptrs = [_ptr(a) for a in arrs]
+        on_device = 0
+    _check(_f["m4g_matrix_assemble"](ctypes.byref(h), int(nrows), int(ncols), count, *ptrs, on_device, int(updater),
+                                     existing.h if existing is not None else None), "m4g_matrix_assemble")
+    M = Matrix.__new__(Matrix)
+    M.h, M.nrows, M.ncols = h, int(nrows), int(ncols)
+    nnz = ctypes.c_int64()
+    _check(_f["m4g_matrix_dims"](h, None, None, ctypes.byref(nnz)), "m4g_matrix_dims")
+    M.nnz = nnz.value
+    return M
 
 
 class Vector:

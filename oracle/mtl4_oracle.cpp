@@ -1094,6 +1094,59 @@ int orc_bicgstab_ell(idx_t n, const idx_t* S, const idx_t* I, const double* D, d
     return it->error;
 }
 
+// ---- assembly: compressed2D_inserter --------------------------------------
+
+// mat::inserter<compressed2D, Updater> (mtl/matrix/compressed2D.hpp:662-1151,
+// modify :955-985): the matrix's entries are present first; each inserted
+// value either updates a present entry with the updater (update.hpp:20-115:
+// store x = v, plus x += v, minus x -= v, times x *= v) or creates it with
+// updater.init(v) (v; -v for minus).  Rows are sorted by column at the end.
+// updater: 0 store, 1 plus, 2 minus, 3 times.  With outS == NULL only the
+// entry count is returned.
+idx_t orc_assemble(idx_t nrows, idx_t count, const idx_t* rows, const idx_t* cols, const double* vals, int updater,
+                   const idx_t* exS, const idx_t* exI, const double* exD, idx_t* outS, idx_t* outI, double* outD)
+{
+    std::vector<std::vector<std::pair<idx_t, double>>> R(static_cast<size_t>(nrows));
+    auto find = [&](idx_t r, idx_t c) -> double* {
+        for (auto& e : R[size_t(r)])
+            if (e.first == c) return &e.second;
+        return nullptr;
+    };
+    if (exS)
+        for (idx_t r = 0; r < nrows; r++)
+            for (idx_t j = exS[r]; j < exS[r + 1]; j++) R[size_t(r)].push_back({exI[j], exD[j]});
+    for (idx_t i = 0; i < count; i++) {
+        double* x = find(rows[i], cols[i]);
+        const double v = vals[i];
+        if (x) {
+            if (updater == 0) *x = v;
+            else if (updater == 1) *x += v;
+            else if (updater == 2) *x -= v;
+            else *x *= v;
+        } else {
+            R[size_t(rows[i])].push_back({cols[i], updater == 2 ? -v : v});
+        }
+    }
+    idx_t nnz = 0;
+    for (auto& row : R) nnz += idx_t(row.size());
+    if (!outS) return nnz;
+    idx_t k = 0;
+    outS[0] = 0;
+    for (idx_t r = 0; r < nrows; r++) {
+        auto row = R[size_t(r)];
+        std::sort(row.begin(), row.end(), [](const std::pair<idx_t, double>& a, const std::pair<idx_t, double>& b) {
+            return a.first < b.first;
+        });
+        for (auto& e : row) {
+            outI[k] = e.first;
+            outD[k] = e.second;
+            k++;
+        }
+        outS[r + 1] = k;
+    }
+    return nnz;
+}
+
 // ---- ell_matrix and sparse_banded -----------------------------------------
 
 // ell_matrix_inserter's conversion (mtl/matrix/ell_matrix.hpp:146-176) of a

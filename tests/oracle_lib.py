@@ -75,6 +75,8 @@ def _load():
                                             ctypes.POINTER(PC), ctypes.POINTER(Iteration), ctypes.c_int]),
         "orc_idr_s": (ctypes.c_int, [_i64, _i64p, _i64p, _f64p, _f64p, _f64p, ctypes.POINTER(Iteration), ctypes.c_int]),
         "orc_set_random_counter": (None, [ctypes.c_int]),
+        "orc_assemble": (_i64, [_i64, _i64, _i64p, _i64p, _f64p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "orc_ell_layout": (None, [_i64, _i64p, _i64p, _f64p, ctypes.POINTER(ctypes.c_int64),
                                   ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p]),
         "orc_ell_spmv": (None, [_i64, _i64, _i64, _i64p, _f64p, _f64p, _f64p, ctypes.c_int]),
@@ -338,3 +340,23 @@ def banded_spmv(nrows, ncols, B, D, v, w=None, mode=0):
     lib.orc_banded_spmv(nrows, ncols, len(B), np.ascontiguousarray(B, np.int64), np.ascontiguousarray(D, np.float64),
                         np.ascontiguousarray(v, np.float64), w, mode)
     return w
+
+
+def assemble(nrows, rows, cols, vals, updater=0, existing=None):
+    """mat::inserter<compressed2D, Updater> fed with the triplets in order,
+    starting from `existing` (a CSR or None); returns a CSR."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    vals = np.ascontiguousarray(vals, np.float64)
+    ex = (None, None, None)
+    if existing is not None:
+        ex = (existing.starts.ctypes.data, existing.indices.ctypes.data,
+              np.ascontiguousarray(existing.data, np.float64).ctypes.data)
+    keep = existing
+    nnz = lib.orc_assemble(nrows, len(rows), rows, cols, vals, updater, *ex, None, None, None)
+    S = np.zeros(nrows + 1, np.int64)
+    I = np.zeros(nnz, np.int64)
+    D = np.zeros(nnz, np.float64)
+    lib.orc_assemble(nrows, len(rows), rows, cols, vals, updater, *ex, S.ctypes.data, I.ctypes.data, D.ctypes.data)
+    del keep
+    return CSR(S, I, D)
