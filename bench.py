@@ -193,13 +193,93 @@ def assembly(g, N, cpu=True):
            "note": "wall time of m4g_matrix_assemble with the triplets resident in HBM: sort, fold, CSR, "
                    "host pattern copy and SELL build"}
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib as o
+        o = _oracle()
         t0 = time.perf_counter()
         o.assemble(n, r, c, v, 1)
         out["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         out["cpu_kind"] = "oracle restatement of compressed2D_inserter, 1 thread"
     return out
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    return oracle_lib
+
+
+def krylov_family(g, N=64, iters=20, cpu=True):
+    """The other ITL solvers on the N^3 7-point Poisson system with IC(0)
+    (IDR(s): no preconditioner, as in the reference): time per iteration of a
+    fixed `iters`-iteration run on the GPU (scalars on the host, one read-back
+    per reduction) and of the oracle's restatement on one host core."""
+    S, I, V = g.poisson3d(N, N, N)
+    n = len(S) - 1
+    A = g.Matrix(S, I, V)
+    P = g.PC(A, g.PC_IC0)
+    bh = g.rng_uniform(3, 1, n)
+    b = g.Vector(data=bh)
+    nb = float(np.linalg.norm(bh))
+    runs = {
+        "cgs": lambda x, it: g.cgs(A, x, b, P, it),
+        "bicg": lambda x, it: g.bicg(A, x, b, P, it),
+        "tfqmr": lambda x, it: g.tfqmr(A, x, b, P, it),
+        "qmr": lambda x, it: g.qmr(A, x, b, P, it),
+        "gmres(8)": lambda x, it: g.gmres(A, x, b, P, it, restart=8),
+        "bicgstab_2": lambda x, it: g.bicgstab_2(A, x, b, P, it),
+        "bicgstab_ell(4)": lambda x, it: g.bicgstab_ell(A, x, b, P, it, l=4),
+        "idr_s(4)": lambda x, it: g.idr_s(A, x, b, it, s=4),
+    }
+    o = _oracle() if cpu else None
+    Ao = o.CSR(S, I, V) if cpu else None
+    out = {"workload": f"poisson3d_{N}", "n": n, "pc": "ic_0 (idr_s: none)", "iterations_timed": iters, "solvers": {}}
+    for name, fn in runs.items():
+        x = g.Vector(n)
+        x.fill(0.0)
+        it, _ = g.make_iteration(2, 1e-30, 0.0, nb, history=False)
+        fn(x, it)                                           # warm-up
+        x.fill(0.0)
+        it, _ = g.make_iteration(iters, 1e-30, 0.0, nb, history=False)
+        g.synchronize()
+        t0 = time.perf_counter()
+        fn(x, it)
+        g.synchronize()
+        gpu = (time.perf_counter() - t0) * 1e3 / max(1, it.iterations)
+        rec = {"gpu_ms_per_iteration": round(gpu, 3)}
+        if cpu:
+            meth = name.split("(")[0]
+            param = int(name.split("(")[1][:-1]) if "(" in name else None
+            pck = 0 if meth == "idr_s" else o.PC_IC0
+            t = []
+            for k in (1, 4):             # the difference removes the factorisation and setup
+                t0 = time.perf_counter()
+                r = o.solve(meth, Ao, bh, pc=pck, max_iter=k, rtol=1e-30, history=False, param=param)
+                t.append(((time.perf_counter() - t0) * 1e3, r["iterations"]))
+            rec["cpu_ms_per_iteration"] = round((t[1][0] - t[0][0]) / max(1, t[1][1] - t[0][1]), 2)
+        out["solvers"][name] = rec
+    if cpu:
+        out["cpu_kind"] = "oracle restatement, 1 thread: (time of 4 iterations - time of 1) / 3"
+    return out
+
+
+def matrix_market(g, N=64):
+    """m4g_mm_write / m4g_mm_read of the N^3 Laplacian (coordinate real general)."""
+    import tempfile
+    S, I, V = g.poisson3d(N, N, N)
+    path = os.path.join(tempfile.gettempdir(), f"m4g_bench_{os.getpid()}.mtx")
+    try:
+        t0 = time.perf_counter()
+        g.mm_write(path, S, I, V)
+        w = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        S2, I2, V2, nr, nc = g.mm_read(path)
+        r = time.perf_counter() - t0
+        ok = bool(np.array_equal(S2, S) and np.array_equal(I2, I) and np.array_equal(V2, V))
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    return {"workload": f"poisson3d_{N}", "file_mb": round(size / 1e6, 1), "nnz": int(S[-1]),
+            "write_mb_s": round(size / 1e6 / w, 1), "read_mb_s": round(size / 1e6 / r, 1), "round_trip_exact": ok}
 
 
 def mixed_precision(g, N, steps=2):
@@ -269,6 +349,7 @@ def main():
     ap.add_argument("--mixed-grid", type=int, default=96)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-system workload")
     ap.add_argument("--no-assembly", action="store_true", help="skip the device assembly measurement")
+    ap.add_argument("--no-krylov-family", action="store_true", help="skip the cgs ... idr_s timings")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: a calibration axpy of known size, 5 SpMVs and 5 IC(0) applications")
     args = ap.parse_args()
@@ -381,9 +462,24 @@ def main():
                      "flushed_frac": round(fb / (np.median(ff) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "csr_bytes_flushed_gbs": round(spmv_bytes / (np.median(ff) * 1e-3) / 1e9, 1)}
         M.close()
+    if not args.no_cpu:
+        o = _oracle()
+        Ao = o.CSR(S, I, V)
+        xh = xt.numpy()
+        sl, st, EI, ED = o.ell_layout(Ao)
+        BB, BD = o.banded_layout(Ao)
+        for name, fn in (("ell", lambda: o.ell_spmv(n, sl, st, EI, ED, xh)),
+                         ("banded", lambda: o.banded_spmv(n, n, BB, BD, xh))):
+            t0 = time.perf_counter()
+            for _ in range(3):
+                fn()
+            fmt[name]["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3 / 3, 2)
+        fmt["cpu_kind"] = "oracle restatement of the reference loops, 1 thread, mean of 3"
     g.spmv(A, xt, y)
 
     asm = None if args.no_assembly else assembly(g, N, cpu=not args.no_cpu)
+    kfam = None if args.no_krylov_family else krylov_family(g, cpu=not args.no_cpu)
+    mmio = matrix_market(g)
 
     e_factor = (nnz + n) // 2                                     # entries of L (= U^T), diagonal included
     sweep_bytes = 12 * e_factor + 4 * (n + 1) + 16 * n
@@ -467,6 +563,8 @@ def main():
                  "moved_flushed_gbs": round(spmv_moved / (np.median(flushed) * 1e-3) / 1e9, 1)},
         "spmv_formats": fmt,
         "assembly": asm,
+        "krylov_family": kfam,
+        "matrix_market": mmio,
         "sweep": {"bytes": sweep_bytes, "ms": sweep_ms, "gbs": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1),
                   "tiles": pcstats["tiles_lower"], "steps": pcstats["steps_lower"],
                   "tile": [pcstats["tile_nx"], pcstats["tile_ny"], pcstats["tile_nz"]]},
