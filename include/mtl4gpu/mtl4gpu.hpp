@@ -166,7 +166,55 @@ struct vec_sum_expr {                        // x + alpha*y (alpha = -1 for x - 
     const Vector& y;
 };
 
+namespace detail {
+// true for the library's matrix types, which carry a device handle
+template <typename M, typename = void>
+struct has_handle : std::false_type {};
+template <typename M>
+struct has_handle<M, std::void_t<decltype(std::declval<const M&>().handle())>> : std::true_type {};
+}  // namespace detail
+
+// assign modes of the operator protocol (mtl/operation/assign_mode.hpp:22-135)
+namespace assign {
+struct assign_sum {
+    template <typename T, typename U> static T& apply(T& x, const U& y) { return x = y; }
+};
+struct plus_sum {
+    template <typename T, typename U> static T& apply(T& x, const U& y) { return x += y; }
+};
+struct minus_sum {
+    template <typename T, typename U> static T& apply(T& x, const U& y) { return x -= y; }
+};
+}  // namespace assign
+
+namespace detail {
+template <typename Assign> struct assign_code;
+template <> struct assign_code<assign::assign_sum> { static const int value = M4G_ASSIGN; };
+template <> struct assign_code<assign::plus_sum> { static const int value = M4G_PLUS; };
+template <> struct assign_code<assign::minus_sum> { static const int value = M4G_MINUS; };
+}  // namespace detail
+
+// algebraic shapes: a user operator declares ashape_aux<Op>::type = nonscal
+// (poisson2D_dirichlet.hpp:96-101); the drop-in only needs the names
+namespace ashape {
+struct scal {};
+struct nonscal {};
+template <typename T> struct ashape_aux { typedef nonscal type; };
+}  // namespace ashape
+
 namespace vec {
+
+// A * v for a matrix-free operator: the product is computed where it goes,
+// through A.mult(v, w, Assign) (mtl/vector/mat_cvec_multiplier.hpp:25-69)
+template <typename Matrix, typename VectorIn>
+struct mat_cvec_multiplier {
+    mat_cvec_multiplier(const Matrix& A, const VectorIn& v) : A(A), v(v) {}
+    template <typename VectorOut> void assign_to(VectorOut& w) const { A.mult(v, w, mtl::assign::assign_sum()); }
+    template <typename VectorOut> void increment_it(VectorOut& w) const { A.mult(v, w, mtl::assign::plus_sum()); }
+    template <typename VectorOut> void decrement_it(VectorOut& w) const { A.mult(v, w, mtl::assign::minus_sum()); }
+    const Matrix& A;
+    const VectorIn& v;
+};
 
 // ---------------------------------------------------------------------------
 // dense_vector: device-resident, with an on-demand host mirror
@@ -306,6 +354,25 @@ class dense_vector {
         a.assign_to(*this);
         return *this;
     }
+    template <typename Assigner, typename = decltype(std::declval<const Assigner&>().assign_to(std::declval<dense_vector&>())),
+              typename = void>
+    dense_vector(const Assigner& a)
+    {
+        a.assign_to(*this);
+    }
+    // incrementer / decrementer protocol (w += A * v, w -= A * v for operators)
+    template <typename Incr, typename = decltype(std::declval<const Incr&>().increment_it(std::declval<dense_vector&>()))>
+    dense_vector& operator+=(const Incr& a)
+    {
+        a.increment_it(*this);
+        return *this;
+    }
+    template <typename Decr, typename = decltype(std::declval<const Decr&>().decrement_it(std::declval<dense_vector&>()))>
+    dense_vector& operator-=(const Decr& a)
+    {
+        a.decrement_it(*this);
+        return *this;
+    }
 
     size_type size() const { return size_type(n_); }
     void change_resource(size_type n)
@@ -393,6 +460,15 @@ inline std::ostream& operator<<(std::ostream& os, const dense_vector<V>& v)
     os << "[";
     for (std::size_t i = 0; i < v.size(); i++) os << v[i] << (i + 1 < v.size() ? "," : "");
     return os << "]";
+}
+
+// one_norm(v) = sum |v_i| (mtl/operation/one_norm.hpp), summed on the host mirror
+template <typename V>
+inline V one_norm(const dense_vector<V>& v)
+{
+    V s = 0;
+    for (std::size_t i = 0; i < v.size(); i++) s += std::abs(v[i]);
+    return s;
 }
 
 // iota(v, offset): v[i] = i + offset
@@ -972,6 +1048,36 @@ inline void laplacian_setup(sparse_banded<V>& A, unsigned m, unsigned n)
     A = C;
 }
 
+// poisson2D_dirichlet (mtl/matrix/poisson2D_dirichlet.hpp:24-102): the
+// 5-point Laplacian of an m x n grid as a matrix-free operator, applied on
+// the device (m4g_poisson2d_dirichlet_mult); operator* gives the
+// mat_cvec_multiplier, so w = A * v, w += A * v and the solvers work on it.
+struct poisson2D_dirichlet {
+    typedef double value_type;
+    typedef std::size_t size_type;
+    poisson2D_dirichlet(int m, int n) : m(m), n(n), s(m * n) {}
+    template <typename VectorIn, typename VectorOut, typename Assign>
+    void mult(const VectorIn& v, VectorOut& w, Assign) const
+    {
+        MTL_THROW_IF(int(size(v)) != s, incompatible_size());
+        if (size(w) == 0) w.change_resource(std::size_t(s));
+        MTL_THROW_IF(size(v) != size(w), incompatible_size());
+        mtl::detail::throw_status(
+            m4g_poisson2d_dirichlet_mult(m, n, v.dev(), w.dev(), mtl::detail::assign_code<Assign>::value),
+            "poisson2D_dirichlet");
+        w.device_modified();
+    }
+    template <typename VectorIn>
+    vec::mat_cvec_multiplier<poisson2D_dirichlet, VectorIn> operator*(const VectorIn& v) const
+    {
+        return vec::mat_cvec_multiplier<poisson2D_dirichlet, VectorIn>(*this, v);
+    }
+    int m, n, s;
+};
+inline std::size_t size(const poisson2D_dirichlet& A) { return std::size_t(A.s) * std::size_t(A.s); }
+inline std::size_t num_rows(const poisson2D_dirichlet& A) { return std::size_t(A.s); }
+inline std::size_t num_cols(const poisson2D_dirichlet& A) { return std::size_t(A.s); }
+
 // trans(A): transposed copy (rows sorted)
 template <typename V>
 inline compressed2D<V> trans(const compressed2D<V>& A)
@@ -1007,6 +1113,7 @@ inline std::ostream& operator<<(std::ostream& os, const compressed2D<V>& A)
 }  // namespace mat
 
 using mat::compressed2D;
+using mat::poisson2D_dirichlet;
 using mat::ell_matrix;
 using mat::sparse_banded;
 using mat::inserter;
@@ -1015,6 +1122,7 @@ using mat::num_cols;
 using mat::num_rows;
 using mat::trans;
 using vec::size;
+using vec::one_norm;
 
 template <typename T> struct Collection { typedef typename T::value_type value_type; typedef std::size_t size_type; };
 
@@ -1282,14 +1390,16 @@ class device_pc {
     void solve(const mtl::dense_vector<V>& x, mtl::dense_vector<V>& y) const
     {
         y.checked_change_resource(x.size());
-        mtl::detail::throw_status(m4g_pc_solve(h_, x.dev(), y.dev()), "pc solve");
+        if (!h_) mtl::detail::throw_status(m4g_vector_copy(y.dev(), x.dev()), "pc solve");   // identity of an operator
+        else mtl::detail::throw_status(m4g_pc_solve(h_, x.dev(), y.dev()), "pc solve");
         y.device_modified();
     }
     template <typename V>
     void adjoint_solve(const mtl::dense_vector<V>& x, mtl::dense_vector<V>& y) const
     {
         y.checked_change_resource(x.size());
-        mtl::detail::throw_status(m4g_pc_adjoint_solve(h_, x.dev(), y.dev()), "pc adjoint_solve");
+        if (!h_) mtl::detail::throw_status(m4g_vector_copy(y.dev(), x.dev()), "pc adjoint_solve");
+        else mtl::detail::throw_status(m4g_pc_adjoint_solve(h_, x.dev(), y.dev()), "pc adjoint_solve");
         y.device_modified();
     }
     template <typename V>
@@ -1304,7 +1414,13 @@ class device_pc {
     template <typename Matrix>
     device_pc(const Matrix& A, int kind, int ftype)
     {
-        mtl::detail::throw_status(m4g_pc_create(&h_, A.handle(), kind, ftype), "preconditioner");
+        if constexpr (mtl::detail::has_handle<Matrix>::value) {
+            mtl::detail::throw_status(m4g_pc_create(&h_, A.handle(), kind, ftype), "preconditioner");
+        } else {
+            // a matrix-free operator: only the identity needs no matrix entries
+            MTL_THROW_IF(kind != M4G_PC_IDENTITY,
+                         mtl::logic_error("this preconditioner needs the matrix entries (a compressed2D)"));
+        }
     }
     template <typename V>
     mtl::mat::compressed2D<V> factor(int which) const
@@ -1322,7 +1438,7 @@ class device_pc {
     std::size_t n_ = 0;
 };
 
-template <typename Matrix, typename Value = typename Matrix::value_type>
+template <typename Matrix, typename Value = typename mtl::Collection<Matrix>::value_type>
 class identity : public device_pc {                 // pc/identity.hpp:24-71
   public:
     typedef Value value_type;
@@ -1398,18 +1514,75 @@ inline int run(Fn fn, const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense
 }
 }  // namespace detail
 
+namespace detail {
+// stands for "no preconditioner" in the operator CG (cg.hpp:39-72)
+struct no_preconditioner {
+    template <typename V> void solve(const V&, V&) const {}
+};
+}  // namespace detail
+namespace pc {
+inline bool is_identity(const detail::no_preconditioner&) { return true; }
+}  // namespace pc
+namespace detail {
+// itl::cg for a linear operator without a device matrix (a matrix-free
+// operator such as poisson2D_dirichlet or a user type with mult): the
+// reference's steps on device vectors, q = A * p through the operator's
+// multiplier.  Without a preconditioner cg.hpp:39-72, otherwise :75-112.
+template <typename LinearOperator, typename V, typename Preconditioner, typename Iteration>
+int cg_operator(const LinearOperator& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b,
+                const Preconditioner* L, Iteration& iter)
+{
+    typedef mtl::dense_vector<V> Vector;
+    typedef typename Iteration::real Real;
+    auto ok = [](int st, const char* where) { mtl::detail::throw_status(st, where); };
+    const std::size_t n = x.size();
+    Vector p(n), q(n), r(n), z(n);
+    q = A * x;                                                   // r = b - A*x
+    ok(m4g_vector_copy(r.dev(), b.dev()), "cg");
+    ok(m4g_axpy(-1.0, q.dev(), r.dev()), "cg");
+    r.device_modified();
+    double rho = 0, rho_1 = 0, rr = 0, alpha = 0, alpha_1 = 0;
+    const bool plain = !L || pc::is_identity(*L);
+    if (plain) ok(m4g_dot(r.dev(), r.dev(), &rho), "cg");
+    else ok(m4g_dot(r.dev(), r.dev(), &rr), "cg");
+    while (!iter.finished(Real(std::sqrt(std::abs(plain ? rho : rr))))) {
+        ++iter;
+        const Vector* src = &r;
+        if (!plain) {
+            L->solve(r, z);                                      // z = solve(L, r)
+            ok(m4g_dot(r.dev(), z.dev(), &rho), "cg");
+            src = &z;
+        }
+        if (iter.first()) ok(m4g_vector_copy(p.dev(), src->dev()), "cg");
+        else ok(m4g_xpay(src->dev(), rho / rho_1, p.dev()), "cg");   // p = z + (rho / rho_1) * p
+        p.device_modified();
+        q = A * p;
+        ok(m4g_dot(p.dev(), q.dev(), &alpha_1), "cg");
+        alpha = rho / alpha_1;
+        ok(m4g_axpy(alpha, p.dev(), x.dev()), "cg");             // x += alpha * p
+        x.device_modified();
+        rho_1 = rho;
+        ok(m4g_axpy_unary_dot(alpha, q.dev(), r.dev(), plain ? &rho : &rr), "cg");   // r -= alpha * q; rr = r.r
+        r.device_modified();
+    }
+    return iter;
+}
+}  // namespace detail
+
 // itl::cg(A, x, b, P, iter) (cg.hpp:75-112); identity PC -> cg.hpp:39-72
 template <typename Matrix, typename V, typename Preconditioner, typename Iteration>
 inline int cg(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b, const Preconditioner& L,
               Iteration& iter)
 {
-    return detail::run(m4g_cg, A, x, b, L.handle(), iter);
+    if constexpr (mtl::detail::has_handle<Matrix>::value) return detail::run(m4g_cg, A, x, b, L.handle(), iter);
+    else return detail::cg_operator(A, x, b, &L, iter);
 }
 // no preconditioner (cg.hpp:39-72)
 template <typename Matrix, typename V, typename Iteration>
 inline int cg(const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense_vector<V>& b, Iteration& iter)
 {
-    return detail::run(m4g_cg, A, x, b, nullptr, iter);
+    if constexpr (mtl::detail::has_handle<Matrix>::value) return detail::run(m4g_cg, A, x, b, nullptr, iter);
+    else return detail::cg_operator(A, x, b, static_cast<const detail::no_preconditioner*>(nullptr), iter);
 }
 // right preconditioner ignored (cg.hpp:115-121)
 template <typename Matrix, typename V, typename Preconditioner, typename RightPreconditioner, typename Iteration>

@@ -55,6 +55,13 @@ int m4g_banded_download(m4g_banded B, int64_t* bands, double* data);
  * with no such band are set to 0 (assign) or left alone (+=, -=). */
 int m4g_banded_spmv(m4g_banded B, m4g_vector v, m4g_vector w, int mode);
 
+/* ---- matrix-free operator ------------------------------------------------ */
+/* mtl::mat::poisson2D_dirichlet(m, n).mult(v, w, Assign)
+ * (mtl/matrix/poisson2D_dirichlet.hpp:27-80): the 5-point Laplacian on an
+ * m x n grid (m, n >= 2) with Dirichlet borders, no matrix stored; each entry
+ * in the reference's operand order; mode as above (Assign::apply). */
+int m4g_poisson2d_dirichlet_mult(int64_t m, int64_t n, m4g_vector v, m4g_vector w, int mode);
+
 #ifdef __cplusplus
 }
 #endif

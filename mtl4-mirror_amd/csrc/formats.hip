@@ -83,6 +83,32 @@ __global__ __launch_bounds__(256) void k_banded_spmv(int64_t nrows, int64_t ncol
     }
 }
 
+// poisson2D_dirichlet::mult (mtl/matrix/poisson2D_dirichlet.hpp:27-80): the
+// 5-point operator on an m x n grid without a stored matrix; each w[k] is the
+// reference's expression for its region (inner, border, corner) with the
+// same operand order, then Assign::apply.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_poisson2d_dirichlet(int64_t m, int64_t n, const double* __restrict__ v,
+                                                             double* __restrict__ w)
+{
+    const int64_t total = m * n;
+    for (int64_t k = blockIdx.x * int64_t(256) + threadIdx.x; k < total; k += int64_t(gridDim.x) * 256) {
+        const int64_t i = k / n, j = k - i * n;
+        const bool top = i == 0, bottom = i == m - 1, left = j == 0, right = j == n - 1;
+        double x;
+        if (top && left) x = 4 * v[0] - v[1] - v[n];
+        else if (top && right) x = 4 * v[n - 1] - v[n - 2] - v[2 * n - 1];
+        else if (bottom && left) x = 4 * v[k] - v[k - n] - v[k + 1];
+        else if (bottom && right) x = 4 * v[k] - v[k - 1] - v[k - n];
+        else if (top) x = 4 * v[k] - v[k + n] - v[k - 1] - v[k + 1];
+        else if (bottom) x = 4 * v[k] - v[k - n] - v[k - 1] - v[k + 1];
+        else if (left) x = 4 * v[k] - v[k - n] - v[k + n] - v[k + 1];
+        else if (right) x = 4 * v[k] - v[k - n] - v[k + n] - v[k - 1];
+        else x = 4 * v[k] - v[k - n] - v[k + n] - v[k - 1] - v[k + 1];
+        first_update<MODE>(w, k, x);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_zero_rows(int64_t n, double* __restrict__ w)
 {
     for (int64_t r = blockIdx.x * int64_t(256) + threadIdx.x; r < n; r += int64_t(gridDim.x) * 256) w[r] = 0.0;
@@ -336,6 +362,26 @@ int m4g_banded_spmv(m4g_banded B, m4g_vector v, m4g_vector w, int mode)
         else
             hipLaunchKernelGGL(k_banded_spmv<M4G_MINUS>, grid, block, 0, s, B->nrows, B->ncols, nb, B->bands.ptr,
                                B->val.ptr, x, y);
+        M4G_HIP(hipGetLastError());
+        return M4G_OK;
+    });
+}
+
+int m4g_poisson2d_dirichlet_mult(int64_t m, int64_t n, m4g_vector v, m4g_vector w, int mode)
+{
+    return guard([&] {
+        M4G_REQUIRE(m >= 2 && n >= 2, M4G_ERR_INVALID, "m4g_poisson2d_dirichlet_mult: grid must be at least 2 x 2");
+        check_vectors(m * n, m * n, v, w, mode, "m4g_poisson2d_dirichlet_mult");
+        const dim3 grid(rows_grid(m * n)), block(256);
+        const double* x = (const double*)v->ptr;
+        double* y = (double*)w->ptr;
+        hipStream_t s = current_stream();
+        if (mode == M4G_ASSIGN)
+            hipLaunchKernelGGL(k_poisson2d_dirichlet<M4G_ASSIGN>, grid, block, 0, s, m, n, x, y);
+        else if (mode == M4G_PLUS)
+            hipLaunchKernelGGL(k_poisson2d_dirichlet<M4G_PLUS>, grid, block, 0, s, m, n, x, y);
+        else
+            hipLaunchKernelGGL(k_poisson2d_dirichlet<M4G_MINUS>, grid, block, 0, s, m, n, x, y);
         M4G_HIP(hipGetLastError());
         return M4G_OK;
     });

@@ -152,6 +152,7 @@ _SIGS = {
     "m4g_banded_shape": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i64p]),
     "m4g_banded_download": (ctypes.c_int, [_vp, _vp, _vp]),
     "m4g_banded_spmv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "m4g_poisson2d_dirichlet_mult": (ctypes.c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_int]),
     "m4g_set_random_seed_counter": (None, [ctypes.c_int]),
     "m4g_mm_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "m4g_batch_cg_ic0": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -320,6 +321,11 @@ class Banded:
             self.close()
         except Exception:
             pass
+
+
+def poisson2d_dirichlet_mult(m, n, v, w, mode=ASSIGN):
+    """mtl::mat::poisson2D_dirichlet(m, n).mult(v, w, Assign): the matrix-free 5-point operator."""
+    _check(_f["m4g_poisson2d_dirichlet_mult"](int(m), int(n), v.h, w.h, mode), "m4g_poisson2d_dirichlet_mult")
 
 
 class Matrix:

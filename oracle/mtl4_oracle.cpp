@@ -1147,6 +1147,40 @@ idx_t orc_assemble(idx_t nrows, idx_t count, const idx_t* rows, const idx_t* col
     return nnz;
 }
 
+// ---- matrix-free operator ---------------------------------------------------
+
+// poisson2D_dirichlet::mult (mtl/matrix/poisson2D_dirichlet.hpp:27-80), loop by
+// loop: the inner domain in blocks of four columns then the remainder, the
+// four borders, the four corners; mode 0 assign, 1 +=, 2 -= (Assign::apply).
+void orc_poisson2d_dirichlet_mult(int m, int n, const double* v, double* w, int mode)
+{
+    auto apply = [&](idx_t k, double x) {
+        if (mode == 0) w[k] = x;
+        else if (mode == 1) w[k] += x;
+        else w[k] -= x;
+    };
+    const int nb = n < 3 ? 1 : (n - 2) / 4 * 4 + 1;
+    for (int i = 1; i < m - 1; i++) {
+        const int kmax = i * n + nb;
+        for (int k = i * n + 1; k < kmax; k += 4) {
+            const double v0 = v[k], v1 = v[k + 1], v2 = v[k + 2], v3 = v[k + 3];
+            apply(k, 4 * v0 - v[k - n] - v[k + n] - v[k - 1] - v1);
+            apply(k + 1, 4 * v1 - v[k - n + 1] - v[k + n + 1] - v0 - v2);
+            apply(k + 2, 4 * v2 - v[k - n + 2] - v[k + n + 2] - v1 - v3);
+            apply(k + 3, 4 * v3 - v[k - n + 3] - v[k + n + 3] - v2 - v[k + 4]);
+        }
+        for (int j = nb, k = i * n + j; j < n - 1; j++, k++) apply(k, 4 * v[k] - v[k - n] - v[k + n] - v[k - 1] - v[k + 1]);
+    }
+    for (int j = 1; j < n - 1; j++) apply(j, 4 * v[j] - v[j + n] - v[j - 1] - v[j + 1]);
+    for (int j = 1, k = (m - 1) * n + j; j < n - 1; j++, k++) apply(k, 4 * v[k] - v[k - n] - v[k - 1] - v[k + 1]);
+    for (int i = 1, k = n; i < m - 1; i++, k += n) apply(k, 4 * v[k] - v[k - n] - v[k + n] - v[k + 1]);
+    for (int i = 1, k = n + n - 1; i < m - 1; i++, k += n) apply(k, 4 * v[k] - v[k - n] - v[k + n] - v[k - 1]);
+    apply(0, 4 * v[0] - v[1] - v[n]);
+    apply(n - 1, 4 * v[n - 1] - v[n - 2] - v[2 * n - 1]);
+    apply(idx_t(m - 1) * n, 4 * v[(m - 1) * n] - v[(m - 2) * n] - v[(m - 1) * n + 1]);
+    apply(idx_t(m) * n - 1, 4 * v[m * n - 1] - v[m * n - 2] - v[m * n - n - 1]);
+}
+
 // ---- ell_matrix and sparse_banded -----------------------------------------
 
 // ell_matrix_inserter's conversion (mtl/matrix/ell_matrix.hpp:146-176) of a

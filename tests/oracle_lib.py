@@ -77,6 +77,7 @@ def _load():
         "orc_set_random_counter": (None, [ctypes.c_int]),
         "orc_assemble": (_i64, [_i64, _i64, _i64p, _i64p, _f64p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+        "orc_poisson2d_dirichlet_mult": (None, [ctypes.c_int, ctypes.c_int, _f64p, _f64p, ctypes.c_int]),
         "orc_ell_layout": (None, [_i64, _i64p, _i64p, _f64p, ctypes.POINTER(ctypes.c_int64),
                                   ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p]),
         "orc_ell_spmv": (None, [_i64, _i64, _i64, _i64p, _f64p, _f64p, _f64p, ctypes.c_int]),
@@ -360,3 +361,10 @@ def assemble(nrows, rows, cols, vals, updater=0, existing=None):
     lib.orc_assemble(nrows, len(rows), rows, cols, vals, updater, *ex, S.ctypes.data, I.ctypes.data, D.ctypes.data)
     del keep
     return CSR(S, I, D)
+
+
+def poisson2d_dirichlet_mult(m, n, v, w=None, mode=0):
+    """poisson2D_dirichlet(m, n).mult(v, w, Assign) on the oracle."""
+    w = np.zeros(m * n) if w is None else np.array(w, np.float64)
+    lib.orc_poisson2d_dirichlet_mult(m, n, np.ascontiguousarray(v, np.float64), w, mode)
+    return w

@@ -134,3 +134,19 @@ def test_banded_rectangle_values():
     assert list(D[2 * nb:3 * nb]) == [0, 7, 6, 5]
     x = np.arange(11.0)
     np.testing.assert_array_equal(o.banded_spmv(6, 11, B, D, x)[:3], [1 * 4 + 2 * 0, 3 * 5 + 4 * 1, 5 * 6 + 6 * 2 + 7 * 0])
+
+
+@pytest.mark.parametrize("m,n", [(2, 2), (2, 7), (4, 5), (9, 3), (13, 17), (30, 31)])
+def test_poisson2d_dirichlet_operator(m, n):
+    # matrix_free_1_test.cpp: the operator equals laplacian_setup's matrix times
+    # v; on integer vectors every operation is exact, so the two agree bit for
+    # bit although they add in different orders
+    A = o.laplacian_setup(m, n)
+    v = np.arange(m * n, dtype=np.float64)                   # iota(v)
+    ref = o.spmv(A, v)
+    np.testing.assert_array_equal(o.poisson2d_dirichlet_mult(m, n, v), ref)
+    x = np.random.default_rng(m * 100 + n).uniform(-1, 1, m * n)
+    np.testing.assert_allclose(o.poisson2d_dirichlet_mult(m, n, x), o.spmv(A, x), rtol=0, atol=1e-14)
+    w0 = np.random.default_rng(1).uniform(-1, 1, m * n)
+    np.testing.assert_array_equal(o.poisson2d_dirichlet_mult(m, n, v, w0, 1), w0 + ref)
+    np.testing.assert_array_equal(o.poisson2d_dirichlet_mult(m, n, v, w0, 2), w0 - ref)

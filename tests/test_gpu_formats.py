@@ -138,3 +138,18 @@ def test_format_errors(g):
     np.testing.assert_array_equal(w.numpy(), [1, 2, 3])
     Z.spmv(g.Vector(data=np.ones(3)), w, g.ASSIGN)
     np.testing.assert_array_equal(w.numpy(), [0, 0, 0])
+
+
+@pytest.mark.parametrize("m,n", [(2, 2), (2, 9), (7, 3), (33, 65), (512, 384)])
+def test_poisson2d_dirichlet_operator(g, m, n):
+    # the matrix-free operator (poisson2D_dirichlet::mult): every mode bit-identical to the oracle's loops
+    rng = np.random.default_rng(m + n)
+    x = rng.uniform(-1, 1, m * n)
+    w0 = rng.uniform(-1, 1, m * n)
+    for mode in (g.ASSIGN, g.PLUS, g.MINUS):
+        ref = o.poisson2d_dirichlet_mult(m, n, x, w0, mode)
+        w = g.Vector(data=w0)
+        g.poisson2d_dirichlet_mult(m, n, g.Vector(data=x), w, mode)
+        np.testing.assert_array_equal(w.numpy(), ref)
+    with pytest.raises(g.M4GError):
+        g.poisson2d_dirichlet_mult(1, n, g.Vector(n), g.Vector(n))
