@@ -612,8 +612,39 @@ class compressed2D {
     size_type nnz() const { return size_type(indices_.size()); }
     size_type num_rows() const { return nrows_; }
     size_type num_cols() const { return ncols_; }
+    // raw views (compressed2D.hpp:609-625); indices are 64-bit like the
+    // default size_type.  A non-const view may be written through, so it
+    // drops the device copy, which is rebuilt from the host rows on next use.
     const std::vector<int64_t>& ref_major() const { return starts_; }
     const std::vector<int64_t>& ref_minor() const { return indices_; }
+    std::vector<int64_t>& ref_major()
+    {
+        release();
+        return starts_;
+    }
+    std::vector<int64_t>& ref_minor()
+    {
+        release();
+        return indices_;
+    }
+    const int64_t* address_major() const { return starts_.data(); }
+    const int64_t* address_minor() const { return indices_.data(); }
+    const Value* address_data() const { return data_.data(); }
+    int64_t* address_major()
+    {
+        release();
+        return starts_.data();
+    }
+    int64_t* address_minor()
+    {
+        release();
+        return indices_.data();
+    }
+    Value* address_data()
+    {
+        release();
+        return data_.data();
+    }
     const std::vector<Value>& data() const { return data_; }
 
     // A[r][c]: 0 when absent (compressed2D.hpp:536-547)
