@@ -20,6 +20,8 @@
 #define MTL4GPU_HPP
 
 #include <algorithm>
+#include <thread>
+#include <atomic>
 #include <type_traits>
 #include <cmath>
 #include <complex>
@@ -1409,20 +1411,15 @@ struct solver {
 
 class device_pc {
   public:
-    device_pc(const device_pc&) = delete;
-    device_pc& operator=(const device_pc&) = delete;
-    ~device_pc()
-    {
-        if (h_) m4g_pc_destroy(h_);
-    }
-    m4g_pc handle() const { return h_; }
+    // copies share the device factor (solver classes hold their preconditioner by value)
+    m4g_pc handle() const { return h_.get(); }
 
     template <typename V>
     void solve(const mtl::dense_vector<V>& x, mtl::dense_vector<V>& y) const
     {
         y.checked_change_resource(x.size());
         if (!h_) mtl::detail::throw_status(m4g_vector_copy(y.dev(), x.dev()), "pc solve");   // identity of an operator
-        else mtl::detail::throw_status(m4g_pc_solve(h_, x.dev(), y.dev()), "pc solve");
+        else mtl::detail::throw_status(m4g_pc_solve(h_.get(), x.dev(), y.dev()), "pc solve");
         y.device_modified();
     }
     template <typename V>
@@ -1430,7 +1427,7 @@ class device_pc {
     {
         y.checked_change_resource(x.size());
         if (!h_) mtl::detail::throw_status(m4g_vector_copy(y.dev(), x.dev()), "pc adjoint_solve");
-        else mtl::detail::throw_status(m4g_pc_adjoint_solve(h_, x.dev(), y.dev()), "pc adjoint_solve");
+        else mtl::detail::throw_status(m4g_pc_adjoint_solve(h_.get(), x.dev(), y.dev()), "pc adjoint_solve");
         y.device_modified();
     }
     template <typename V>
@@ -1446,7 +1443,9 @@ class device_pc {
     device_pc(const Matrix& A, int kind, int ftype)
     {
         if constexpr (mtl::detail::has_handle<Matrix>::value) {
-            mtl::detail::throw_status(m4g_pc_create(&h_, A.handle(), kind, ftype), "preconditioner");
+            m4g_pc p = nullptr;
+            mtl::detail::throw_status(m4g_pc_create(&p, A.handle(), kind, ftype), "preconditioner");
+            h_.reset(p, [](m4g_pc q) { m4g_pc_destroy(q); });
         } else {
             // a matrix-free operator: only the identity needs no matrix entries
             MTL_THROW_IF(kind != M4G_PC_IDENTITY,
@@ -1457,15 +1456,15 @@ class device_pc {
     mtl::mat::compressed2D<V> factor(int which) const
     {
         int64_t nnz = 0;
-        mtl::detail::throw_status(m4g_pc_factor_nnz(h_, which, &nnz), "factor");
+        mtl::detail::throw_status(m4g_pc_factor_nnz(h_.get(), which, &nnz), "factor");
         std::vector<int64_t> S(n_ + 1), I(nnz);
         std::vector<double> D(nnz);
-        mtl::detail::throw_status(m4g_pc_download_factor(h_, which, S.data(), I.data(), D.data()), "factor");
+        mtl::detail::throw_status(m4g_pc_download_factor(h_.get(), which, S.data(), I.data(), D.data()), "factor");
         mtl::mat::compressed2D<V> F(n_, n_);
         F.assign_csr(std::move(S), std::move(I), std::vector<V>(D.begin(), D.end()));
         return F;
     }
-    m4g_pc h_ = nullptr;
+    std::shared_ptr<m4g_pc_s> h_;
     std::size_t n_ = 0;
 };
 
@@ -1536,7 +1535,7 @@ inline int run(Fn fn, const Matrix& A, mtl::dense_vector<V>& x, const mtl::dense
     iter.to_c(c, hist);
     int st = fn(A.handle(), x.dev(), b.dev(), P, &c);
     x.device_modified();
-    if (st == M4G_OK || st == M4G_MAX_ITER || st == M4G_BREAKDOWN) {   // breakdown: iter.fail(2, ...)
+    if (st == M4G_OK || st == M4G_MAX_ITER || st == M4G_BREAKDOWN || st == M4G_ERR_INTERRUPTED) {
         iter.from_c(c, hist);
         return iter;
     }
@@ -1777,6 +1776,10 @@ class cg_solver : public base_solver<cg_solver<LinearOperator, Preconditioner, R
     {
         if (!pc::static_is_identity<RightPreconditioner>::value) std::cerr << "Right Preconditioner ignored!" << std::endl;
     }
+    cg_solver(const LinearOperator& A, const Preconditioner& L) : base(A), L(L)
+    {
+        if (!pc::static_is_identity<RightPreconditioner>::value) std::cerr << "Right Preconditioner ignored!" << std::endl;
+    }
     template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
     int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
     {
@@ -1996,6 +1999,117 @@ class idr_s_solver
     Preconditioner L;
     RightPreconditioner R;
 };
+
+// make_cg_solver(A, L) (cg.hpp:174-179)
+template <typename LinearOperator, typename Preconditioner>
+inline cg_solver<LinearOperator, Preconditioner, pc::identity<LinearOperator>> make_cg_solver(const LinearOperator& A,
+                                                                                                const Preconditioner& L)
+{
+    return cg_solver<LinearOperator, Preconditioner, pc::identity<LinearOperator>>(A, L);
+}
+
+// interruptible_iteration (iteration/interruptible_iteration.hpp:21-52): a
+// basic_iteration plus a flag another thread may raise.  Device solves poll
+// the flag between iteration batches (the device-resident drivers) or at every
+// residual test (the host-driven ones); an interrupted solve returns the
+// wrapped iteration's error code, as the reference's finished() does.
+class interruptible_iteration {
+  public:
+    typedef double real;
+    interruptible_iteration(basic_iteration<double>& iter) : iter(&iter), interrupted(false) {}
+    interruptible_iteration() : iter(nullptr), interrupted(false) {}
+    void set_iter(basic_iteration<double>& new_iter) { iter = &new_iter; }
+    template <typename T>
+    bool finished(const T& r)
+    {
+        return iter->finished(r) || interrupted.load();
+    }
+    void interrupt()
+    {
+        flag_ = 1;
+        interrupted = true;
+    }
+    bool is_interrupted() const { return interrupted.load(); }
+    operator int() const { return *iter; }
+    int error_code() const { return iter->error_code(); }
+    interruptible_iteration& operator++()
+    {
+        ++*iter;
+        return *this;
+    }
+    bool first() const { return iter->first(); }
+    int iterations() const { return iter->iterations(); }
+
+    void to_c(m4g_iteration& c, std::vector<double>& hist) const
+    {
+        iter->to_c(c, hist);
+        c.interrupt = &flag_;
+    }
+    void from_c(const m4g_iteration& c, const std::vector<double>& hist)
+    {
+        m4g_iteration d = c;
+        if (d.error == M4G_ERR_INTERRUPTED) d.error = iter->error_code();
+        iter->from_c(d, hist);
+    }
+
+  private:
+    basic_iteration<double>* iter;
+    std::atomic<bool> interrupted;
+    volatile int flag_ = 0;
+};
+
+// async_executor (krylov/async_executor.hpp:23-74): runs a solver on its own
+// host thread with an interruptible iteration; the thread's library calls go
+// to that thread's stream of the same device.
+template <typename Solver>
+class async_executor {
+  public:
+    async_executor(const Solver& solver) : my_solver(solver), my_iter(), my_thread() {}
+    ~async_executor()
+    {
+        if (my_thread.joinable()) my_thread.join();
+    }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    void start_solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        my_iter.set_iter(iter);
+        my_thread = std::thread([this, &x, &b]() { my_solver.solve(x, b, my_iter); });
+    }
+    template <typename HilbertSpaceX, typename HilbertSpaceB, typename Iteration>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b, Iteration& iter) const
+    {
+        start_solve(x, b, iter);
+        return const_cast<async_executor*>(this)->wait();
+    }
+    template <typename HilbertSpaceX, typename HilbertSpaceB>
+    int solve(HilbertSpaceX& x, const HilbertSpaceB& b) const
+    {
+        basic_iteration<double> iter(x, 1, 0, 0);
+        return solve(x, b, iter);
+    }
+    int wait()
+    {
+        my_thread.join();
+        return my_iter.error_code();
+    }
+    bool is_finished() const { return !my_thread.joinable(); }
+    int interrupt()
+    {
+        my_iter.interrupt();
+        return wait();
+    }
+
+  private:
+    Solver my_solver;
+    mutable interruptible_iteration my_iter;
+    mutable std::thread my_thread;
+};
+
+template <typename Solver>
+inline async_executor<Solver> make_async_executor(const Solver& solver)
+{
+    return async_executor<Solver>(solver);
+}
 
 }  // namespace itl
 

@@ -181,13 +181,18 @@ struct Ops {
     }
 };
 
-// basic_iteration::finished(r) (basic_iteration.hpp:52-65, 85-93)
+// basic_iteration::finished(r) (basic_iteration.hpp:52-65, 85-93); a raised
+// interrupt flag finishes too (interruptible_iteration.hpp:37-40)
 bool finished(m4g_iteration* it, double r)
 {
     it->resid = r;
     if (it->history && it->iterations < it->history_capacity) it->history[it->iterations] = r;
     const bool conv = it->norm_r0 == 0 ? r <= it->atol : (r <= it->rtol * it->norm_r0 || r <= it->atol);
     if (conv) return true;
+    if (it->interrupt && *it->interrupt) {
+        it->error = M4G_ERR_INTERRUPTED;
+        return true;
+    }
     if (it->iterations >= it->max_iter) {
         it->error = M4G_MAX_ITER;
         return true;
@@ -211,6 +216,7 @@ int sweep_fault(m4g_pc_s* P)
 int status_of(m4g_iteration* it, int fault)
 {
     if (fault) return M4G_ERR_TIMEOUT;
+    if (it->error == M4G_ERR_INTERRUPTED) return M4G_ERR_INTERRUPTED;
     if (it->error >= 2) return M4G_BREAKDOWN;
     return it->error;
 }

@@ -206,3 +206,24 @@ def test_more_solvers_argument_errors(g):
     with pytest.raises(g.M4GError) as e:
         g.bicgstab_ell(Ag, x, bv, None, it, l=0)
     assert e.value.status == g.ERR_INVALID
+
+
+def test_interrupt_flag_stops_host_driven_solvers(g):
+    # interruptible_iteration: a raised flag finishes the loop at the next
+    # residual test; the C ABI reports M4G_ERR_INTERRUPTED
+    import ctypes
+    S, I, V = g.laplacian2d(40, 40)
+    Ag = g.Matrix(S, I, V)
+    n = len(S) - 1
+    flag = ctypes.c_int(1)
+    for fn in (lambda x, it: g.gmres(Ag, x, g.Vector(data=np.ones(n)), None, it, restart=4),
+               lambda x, it: g.qmr(Ag, x, g.Vector(data=np.ones(n)), None, it),
+               lambda x, it: g.idr_s(Ag, x, g.Vector(data=np.ones(n)), it, s=2)):
+        x = g.Vector(n)
+        x.fill(0.0)
+        it, _ = g.make_iteration(500, 1e-30, 0.0, 1.0, history=False)
+        it.interrupt = ctypes.pointer(flag)
+        with pytest.raises(g.M4GError) as e:
+            fn(x, it)
+        assert e.value.status == g.ERR_INTERRUPTED
+        assert it.iterations <= 2
