@@ -71,12 +71,16 @@ def time_launches(timer, launch, reps, flush=None):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of the kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel of known
-    size); None when there is no such measurement."""
-    path = os.path.join(ROOT, "profiles", "k_sweep_k_sell_pipe_poisson3d_128_hbm_traffic.md")
+    FETCH_SIZE / WRITE_SIZE summaries (calibrated against a copy kernel of
+    known size); None when there is no such measurement."""
+    if kernel.startswith("k_ell") or kernel.startswith("k_banded"):
+        path = os.path.join(ROOT, "profiles", "k_ell_spmv_k_banded_spmv_poisson3d_128_hbm_traffic.md")
+        name = "k_ell_spmv" if kernel.startswith("k_ell") else "k_banded_spmv"
+    else:
+        path = os.path.join(ROOT, "profiles", "k_sweep_k_sell_pipe_poisson3d_128_hbm_traffic.md")
+        name = "k_sweep" if kernel.startswith("k_sweep") else "k_sell_pipe"
     if not os.path.exists(path):
         return None
-    name = "k_sweep" if kernel.startswith("k_sweep") else "k_sell_pipe"
     for line in open(path):
         parts = line.split()
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == name:
@@ -387,8 +391,14 @@ def main():
             g.spmv(A, xt, y)
         for _ in range(5):
             P.apply_async(xt, y)
+        E, Bm = g.Ell(S, I, V), g.Banded(S, I, V)
+        for _ in range(5):
+            E.spmv(xt, y)
+        for _ in range(5):
+            Bm.spmv(xt, y)
         g.synchronize()
-        print(json.dumps({"kernels_only": True, "n": n, "nnz": nnz, "calibration_axpy_n": 1 << 25}), flush=True)
+        print(json.dumps({"kernels_only": True, "n": n, "nnz": nnz, "calibration_axpy_n": 1 << 25,
+                          "ell_slots": E.slots, "banded_bands": Bm.nbands}), flush=True)
         return
 
     # ---- IC(0) factorisation (once) ----
@@ -460,7 +470,8 @@ def main():
         fmt[name] = {"bytes": fb, "warm_ms_median": float(np.median(fw)), "flushed_ms_median": float(np.median(ff)),
                      "flushed_gbs": round(fb / (np.median(ff) * 1e-3) / 1e9, 1),
                      "flushed_frac": round(fb / (np.median(ff) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "csr_bytes_flushed_gbs": round(spmv_bytes / (np.median(ff) * 1e-3) / 1e9, 1)}
+                     "csr_bytes_flushed_gbs": round(spmv_bytes / (np.median(ff) * 1e-3) / 1e9, 1),
+                     "traffic": measured_traffic("k_ell_spmv" if name == "ell" else "k_banded_spmv")}
         M.close()
     if not args.no_cpu:
         o = _oracle()
