@@ -286,6 +286,67 @@ def matrix_market(g, N=64):
             "write_mb_s": round(size / 1e6 / w, 1), "read_mb_s": round(size / 1e6 / r, 1), "round_trip_exact": ok}
 
 
+def differentiation(g, side=256, steps=3, cpu=True):
+    """x = A(theta)^{-1} b on the side x side edge-conductivity diffusion grid,
+    theta ~ lognormal(0,1), b and x_target ~ N(0,1), L = 1/2 |x - x_target|^2.
+    GPU: the forward (assembly of A's values, IC(0) refactorisation, IC(0)-PCG
+    to 1e-10) and the backward (adjoint IC(0)-PCG, pattern gradient, chain rule
+    to theta) through the autograd binding, each the mean of `steps` runs.
+    CPU: the oracle's IC(0)-PCG for the forward and the adjoint solve (each
+    factorising) plus the closed-form gradient, one host thread."""
+    import torch
+    import mtl4gpu_torch as gt
+    sysm = gt.EdgeSystem(side, side, rtol=1e-10)
+    ne, n = sysm.n_edges, sysm.n
+    theta_h, b_h, xt_h = g.rng_lognormal(5, 1, ne), g.rng_normal(5, 2, n), g.rng_normal(5, 3, n)
+    b = torch.tensor(b_h, device="cuda")
+    x_target = torch.tensor(xt_h, device="cuda")
+    fwd, bwd = [], []
+    for k in range(steps + 1):                              # the first pass is the warm-up
+        theta = torch.tensor(theta_h, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = gt.solve(theta, b, sysm)
+        loss = 0.5 * torch.sum((x - x_target) ** 2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fwd_its = sysm.last_iterations
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if k:
+            fwd.append((t1 - t0) * 1e3)
+            bwd.append((t2 - t1) * 1e3)
+    adj_its = sysm.last_iterations
+    grad = theta.grad.cpu().numpy()
+    out = {"workload": f"edge_diffusion2d_{side}", "n": n, "nnz": sysm.nnz, "n_edges": ne, "rtol": 1e-10,
+           "forward_ms": round(float(np.mean(fwd)), 3), "backward_ms": round(float(np.mean(bwd)), 3),
+           "forward_iterations": fwd_its, "adjoint_iterations": adj_its}
+    if cpu:
+        o = _oracle()
+        e = sysm.e
+        t0 = time.perf_counter()
+        vals = np.zeros(sysm.nnz)
+        np.add.at(vals, np.repeat(np.arange(sysm.nnz), np.diff(e["aptr"])), e["acoef"] * theta_h[e["aidx"]])
+        Ao = o.CSR(e["S"], e["I"], vals)
+        rf = o.solve("cg", Ao, b_h, pc=o.PC_IC0, max_iter=20000, rtol=1e-10, history=False)
+        t1 = time.perf_counter()
+        xc = rf["x"]
+        ra = o.solve("cg", Ao, xc - xt_h, pc=o.PC_IC0, max_iter=20000, rtol=1e-10, history=False)
+        lam = ra["x"]
+        ei, ej = e["ei"], e["ej"]
+        inner = ej >= 0
+        gc = np.empty(ne)
+        gc[inner] = -(lam[ei[inner]] - lam[ej[inner]]) * (xc[ei[inner]] - xc[ej[inner]])
+        gc[~inner] = -lam[ei[~inner]] * xc[ei[~inner]]
+        t2 = time.perf_counter()
+        out.update(cpu_forward_ms=round((t1 - t0) * 1e3, 1), cpu_backward_ms=round((t2 - t1) * 1e3, 1),
+                   cpu_iterations=[rf["iterations"], ra["iterations"]],
+                   grad_rel_err_vs_cpu=float(np.max(np.abs(grad - gc)) / np.max(np.abs(gc))),
+                   cpu_kind="oracle IC(0)-PCG (factorisation included) + closed-form gradient, 1 thread")
+    return out
+
+
 def mixed_precision(g, N, steps=2):
     """27-point convection-diffusion on an N^3 grid: plain FP64 ILU(0)-BiCGStab
     to 1e-10 vs FP32 ILU(0)-BiCGStab to 1e-4 inside FP64 iterative refinement
@@ -354,6 +415,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-system workload")
     ap.add_argument("--no-assembly", action="store_true", help="skip the device assembly measurement")
     ap.add_argument("--no-krylov-family", action="store_true", help="skip the cgs ... idr_s timings")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation (adjoint) timings")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: a calibration axpy of known size, 5 SpMVs and 5 IC(0) applications")
     args = ap.parse_args()
@@ -545,6 +607,7 @@ def main():
 
     mixed = None if args.no_mixed else mixed_precision(g, args.mixed_grid)
     batch = None if args.no_batch else batched(g)
+    diff = None if args.no_diff else differentiation(g, cpu=not args.no_cpu)
 
     out = {
         "metric": METRIC,
@@ -588,6 +651,7 @@ def main():
         "mixed_vs_fp64_speedup": mixed["speedup"] if mixed else None,
         "mixed": mixed,
         "batched": batch,
+        "differentiation": diff,
     }
     print(json.dumps(out), flush=True)
 

@@ -55,7 +55,7 @@ _vp = ctypes.c_void_p
 class PcStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "levels_lower", "levels_upper", "tiles_lower", "tiles_upper", "steps_lower", "steps_upper",
-        "grid_nx", "grid_ny", "grid_nz", "tile_nx", "tile_ny", "tile_nz")]
+        "grid_nx", "grid_ny", "grid_nz", "tile_nx", "tile_ny", "tile_nz", "line_bands")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -6,7 +6,9 @@ its stored values are M theta for a fixed sparse assembly map M.  The forward
 pass assembles the values on the GPU (SpMV with M), refactorises IC(0) and
 runs IC(0)-PCG; the backward pass solves the adjoint system A^T lambda = dL/dx
 with the same kernels (A is symmetric), forms dL/dA_ij = -lambda_i x_j on the
-pattern (m4g_pattern_gradient) and maps it to theta with M^T (SpMV).
+pattern (m4g_pattern_gradient) and maps it to theta with M^T (SpMV).  The
+backward reuses the forward's values and IC(0) factors unless the same
+EdgeSystem was given another theta in between.
 All device work goes through libmtl4gpu; torch only holds the memory.
 """
 import numpy as np
@@ -34,6 +36,7 @@ class EdgeSystem:
         self.A = g.Matrix(self.S, self.I, self._assemble_host(e, theta0))
         self.P = g.PC(self.A, g.PC_IC0)
         self.e = e
+        self.epoch = 0              # counts set_theta calls: which theta the factors belong to
 
     @staticmethod
     def _csr(r, c, v, m, n):
@@ -57,6 +60,7 @@ class EdgeSystem:
         g.synchronize()
         self.A.set_values(vals)
         self.P.refactor(self.A)
+        self.epoch += 1
 
     def solve(self, rhs):
         """x = A^{-1} rhs with IC(0)-PCG to rtol (x0 = 0)."""
@@ -80,6 +84,7 @@ class SparseSolve(torch.autograd.Function):
         system.set_theta(theta)
         x = system.solve(b)
         ctx.system = system
+        ctx.epoch, ctx.theta_version = system.epoch, theta._version
         ctx.save_for_backward(theta, x)
         return x
 
@@ -88,7 +93,8 @@ class SparseSolve(torch.autograd.Function):
         system = ctx.system
         theta, x = ctx.saved_tensors
         g.set_stream(torch.cuda.current_stream().cuda_stream)
-        system.set_theta(theta)                      # the factorisation of this theta
+        if system.epoch != ctx.epoch or theta._version != ctx.theta_version:
+            system.set_theta(theta)                  # another theta was factorised since the forward
         lam = system.solve(gx.contiguous())          # A^T lambda = dL/dx (A symmetric)
         grad_vals = torch.empty(system.nnz, dtype=torch.float64, device=x.device)
         g.pattern_gradient(system.A, g.Vector(tensor=lam), g.Vector(tensor=x), g.Vector(tensor=grad_vals))

@@ -90,3 +90,25 @@ def test_cfg5_gradient(env, side):
     err_fd = np.max(np.abs(fd - grad[idx])) / scale
     print(f"  finite differences on 64 entries: max rel err {err_fd:.2e} (scale {scale:.3e})", flush=True)
     assert err_fd < 1e-4
+
+
+def test_backward_after_another_forward(env):
+    # the backward of the first solve must use the first theta's factors even
+    # though the same system was refactorised for a second theta in between
+    torch, g, gt = env
+    sysm = gt.EdgeSystem(24, 20, rtol=1e-12)
+    rng = np.random.default_rng(7)
+    th1 = np.exp(rng.standard_normal(sysm.n_edges))
+    th2 = np.exp(rng.standard_normal(sysm.n_edges))
+    b_h, xt_h = rng.standard_normal(sysm.n), rng.standard_normal(sysm.n)
+    b, xt = torch.tensor(b_h, device="cuda"), torch.tensor(xt_h, device="cuda")
+    theta1 = torch.tensor(th1, device="cuda", requires_grad=True)
+    theta2 = torch.tensor(th2, device="cuda", requires_grad=True)
+    loss1 = 0.5 * torch.sum((gt.solve(theta1, b, sysm) - xt) ** 2)
+    loss2 = 0.5 * torch.sum((gt.solve(theta2, b, sysm) - xt) ** 2)
+    loss1.backward()
+    loss2.backward()
+    for th_h, th in ((th1, theta1), (th2, theta2)):
+        _, ref = cpu_reference_gradient(sysm.e, th_h, b_h, xt_h)
+        got = th.grad.cpu().numpy()
+        assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-8
