@@ -416,6 +416,8 @@ def main():
     ap.add_argument("--no-assembly", action="store_true", help="skip the device assembly measurement")
     ap.add_argument("--no-krylov-family", action="store_true", help="skip the cgs ... idr_s timings")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation (adjoint) timings")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="profiling run: only the differentiation workload (one JSON line)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: a calibration axpy of known size, 5 SpMVs and 5 IC(0) applications")
     args = ap.parse_args()
@@ -425,6 +427,10 @@ def main():
         sys.exit(2)
 
     import mtl4gpu as g
+
+    if args.diff_only:
+        print(json.dumps({"differentiation": differentiation(g, cpu=not args.no_cpu)}), flush=True)
+        return
 
     N = args.grid
     t0 = time.perf_counter()
