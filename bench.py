@@ -611,6 +611,15 @@ def main():
                              f"{iters[-1]} iterations",
                    "ms_per_iteration": round(omp_iter_ms, 2)}
 
+    if cpu:
+        host = {"nproc": os.cpu_count(), "cpu_model": None}
+        try:
+            with open("/proc/cpuinfo") as f:
+                host["cpu_model"] = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+        except OSError:
+            pass
+        cpu["host"] = cpu_omp["host"] = host
+
     mixed = None if args.no_mixed else mixed_precision(g, args.mixed_grid)
     batch = None if args.no_batch else batched(g)
     diff = None if args.no_diff else differentiation(g, cpu=not args.no_cpu)
