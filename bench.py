@@ -286,6 +286,54 @@ def matrix_market(g, N=64):
             "write_mb_s": round(size / 1e6 / w, 1), "read_mb_s": round(size / 1e6 / r, 1), "round_trip_exact": ok}
 
 
+def cpu_runnable_case(g, side=64, steps=5, cpu=True):
+    """The CPU-runnable configuration: 2D 5-point Poisson on a side x side grid
+    (n = 4,096, nnz = 20,224 at 64), x_true ~ U[-1,1] (seed 1), b = A x_true,
+    IC(0)-PCG from x0 = 0 to 1e-10 relative residual.  GPU: factorisation and
+    solve timed separately (mean of `steps` solves); CPU: the oracle's
+    factorisation + solve on one host core; iteration counts side by side."""
+    S, I, V = g.laplacian2d(side, side)
+    n = len(S) - 1
+    xt = g.rng_uniform(1, 1, n)
+    A = g.Matrix(S, I, V)
+    b = g.Vector(n)
+    g.spmv(A, g.Vector(data=xt), b)
+    nb = g.two_norm(b)
+    g.synchronize()
+    t0 = time.perf_counter()
+    P = g.PC(A, g.PC_IC0)
+    g.synchronize()
+    factor_ms = (time.perf_counter() - t0) * 1e3
+    x = g.Vector(n)
+
+    def solve():
+        x.fill(0.0)
+        it, _ = g.make_iteration(2000, 1e-10, 0.0, nb, history=False)
+        assert g.cg(A, x, b, P, it) == 0
+        return it.iterations
+
+    solve()
+    g.synchronize()
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        its = solve()
+        g.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    out = {"workload": f"ic0_pcg_laplacian2d_{side}", "n": n, "nnz": int(S[-1]), "rtol": 1e-10,
+           "factor_ms": round(factor_ms, 2), "solve_ms": round(float(np.mean(ts)), 3), "iterations": its}
+    if cpu:
+        o = _oracle()
+        Ao = o.CSR(S, I, V)
+        bh = b.numpy()
+        t0 = time.perf_counter()
+        r = o.solve("cg", Ao, bh, pc=o.PC_IC0, max_iter=2000, rtol=1e-10, history=False)
+        out.update(cpu_ms=round((time.perf_counter() - t0) * 1e3, 2), cpu_iterations=r["iterations"],
+                   max_abs_diff_vs_cpu_x=float(np.max(np.abs(x.numpy() - r["x"]))),
+                   cpu_kind="oracle IC(0) factorisation + PCG, 1 thread")
+    return out
+
+
 def differentiation(g, side=256, steps=3, cpu=True):
     """x = A(theta)^{-1} b on the side x side edge-conductivity diffusion grid,
     theta ~ lognormal(0,1), b and x_target ~ N(0,1), L = 1/2 |x - x_target|^2.
@@ -623,6 +671,7 @@ def main():
     mixed = None if args.no_mixed else mixed_precision(g, args.mixed_grid)
     batch = None if args.no_batch else batched(g)
     diff = None if args.no_diff else differentiation(g, cpu=not args.no_cpu)
+    small = cpu_runnable_case(g, cpu=not args.no_cpu)
 
     out = {
         "metric": METRIC,
@@ -667,6 +716,7 @@ def main():
         "mixed": mixed,
         "batched": batch,
         "differentiation": diff,
+        "cpu_runnable_case": small,
     }
     print(json.dumps(out), flush=True)
 
