@@ -18,6 +18,8 @@
 // upper_trisolve.hpp:159-176).
 #include "schedule.hpp"
 
+#include <thread>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -698,24 +700,35 @@ bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<in
 bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64_t>& UI, SweepRows& lower,
               SweepRows& upper)
 {
-    // upper sweep on U: diagonal first, then entries in column order
-    upper.n = n;
-    upper.lower = false;
-    upper.ptr.assign(n + 1, 0);
-    upper.col.clear();
-    upper.val_pos.clear();
-    upper.col.reserve(UI.size());
-    upper.val_pos.reserve(UI.size());
-    upper.diag_pos.assign(n, -1);
-    for (int64_t r = 0; r < n; r++) {
-        if (US[r] == US[r + 1] || UI[US[r]] != r) return false;
-        upper.diag_pos[r] = US[r];
-        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
-            upper.col.push_back(UI[p]);
-            upper.val_pos.push_back(p);
+    // upper sweep on U: diagonal first, then entries in column order (on a
+    // second host thread; the lower rows below do not depend on it)
+    bool upper_ok = true;
+    std::thread upper_thread([&] {
+        upper.n = n;
+        upper.lower = false;
+        upper.ptr.assign(n + 1, 0);
+        upper.col.clear();
+        upper.val_pos.clear();
+        upper.col.reserve(UI.size());
+        upper.val_pos.reserve(UI.size());
+        upper.diag_pos.assign(n, -1);
+        for (int64_t r = 0; r < n; r++) {
+            if (US[r] == US[r + 1] || UI[US[r]] != r) {
+                upper_ok = false;
+                return;
+            }
+            upper.diag_pos[r] = US[r];
+            for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
+                upper.col.push_back(UI[p]);
+                upper.val_pos.push_back(p);
+            }
+            upper.ptr[r + 1] = int64_t(upper.col.size());
         }
-        upper.ptr[r + 1] = int64_t(upper.col.size());
-    }
+    });
+    struct Join {
+        std::thread& t;
+        ~Join() { if (t.joinable()) t.join(); }
+    } join{upper_thread};
     // lower sweep on L = trans(U): row i holds U's column i (rows k < i
     // ascending), inverted diagonal last
     lower.n = n;
@@ -736,7 +749,8 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
             lower.val_pos[d] = p;
         }
     }
-    return true;
+    upper_thread.join();
+    return upper_ok;
 }
 
 bool ilu0_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper,
