@@ -22,6 +22,6 @@ int main()
     int err = cg(A, x, b, P, iter);
     mtl::dense_vector<double> r(b - A * x);
     std::cout << iter.iterations() << " iterations, true relres " << two_norm(r) / two_norm(b) << "\n";
-    MTL_THROW_IF(err != 0 || two_norm(r) > 1e-9 * two_norm(b), mtl::runtime_error("cfg1 failed"));
+    MTL_THROW_IF(err != 0 || two_norm(r) > 1e-9 * two_norm(b), mtl::runtime_error("64x64 IC(0)-PCG failed"));
     return 0;
 }

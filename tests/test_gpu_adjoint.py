@@ -47,9 +47,9 @@ def test_gradcheck_small(env):
 
 
 @pytest.mark.parametrize("side", [64, 256])
-def test_cfg5_gradient(env, side):
+def test_edge_diffusion_gradient(env, side):
     torch, g, gt = env
-    print(f"cfg5 side={side}", flush=True)
+    print(f"edge diffusion side={side}", flush=True)
     sysm = gt.EdgeSystem(side, side, rtol=1e-10)
     ne, n = sysm.n_edges, sysm.n
     theta_h = g.rng_lognormal(5, 1, ne)

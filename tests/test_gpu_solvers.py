@@ -60,9 +60,9 @@ def compare(g, method, gen, pc, rtol, seed=1, max_iter=2000, strict=False):
     return got, ref
 
 
-def test_cfg1_ic0_pcg(g):
+def test_laplacian64_ic0_pcg(g):
     # 2D 5-point 64x64, IC(0)-PCG to 1e-10
-    say("cfg1 ic0 pcg")
+    say("64x64 laplacian ic0 pcg")
     compare(g, "cg", g.laplacian2d(64, 64), o.PC_IC0, 1e-10, strict=True)
 
 

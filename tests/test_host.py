@@ -49,7 +49,7 @@ def test_poisson3d_counts_and_stencil():
     np.testing.assert_array_equal(d.sum(1), 6 - nb)
 
 
-def test_poisson3d_cfg2_nnz():
+def test_poisson3d_128_nnz():
     # only the count (generation of 14.6M entries takes a moment)
     nnz = g._f["m4g_gen_poisson3d"](128, 128, 128, None, None, None)
     assert nnz == 14581760

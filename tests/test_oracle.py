@@ -47,7 +47,7 @@ def test_laplacian_setup_stencil():
             ii, jj = i + di, j + dj
             if 0 <= ii < 3 and 0 <= jj < 4:
                 assert A[r, ii * 4 + jj] == -1
-    assert o.laplacian_setup(64, 64).nnz == 20224   # cfg1
+    assert o.laplacian_setup(64, 64).nnz == 20224   # the 64x64 5-point system
 
 
 # ---------------------------------------------------------------- SpMV
