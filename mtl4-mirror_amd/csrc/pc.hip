@@ -17,6 +17,9 @@
 
 #include <algorithm>
 #include <future>
+#include <chrono>
+#include <cstdio>
+#include <string>
 #include <cmath>
 #include <cstdlib>
 
@@ -1413,6 +1416,17 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
         P->n = A->nrows;
         hipStream_t s = current_stream();
         const int64_t n = A->nrows;
+        // M4G_PC_TIMING: host time of each construction phase on stderr
+        const bool timing = std::getenv("M4G_PC_TIMING") != nullptr;
+        auto t_prev = std::chrono::steady_clock::now();
+        std::string tlog;
+        auto mark = [&](const char* what) {
+            if (!timing) return;
+            const auto t = std::chrono::steady_clock::now();
+            tlog += std::string(what) + "=" +
+                    std::to_string(std::chrono::duration<double, std::milli>(t - t_prev).count()) + "ms ";
+            t_prev = t;
+        };
         if (kind == M4G_PC_DIAGONAL) {
             P->inv_diag.reset(std::max<int64_t>(n, 1));
             if (n)
@@ -1453,17 +1467,23 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                                            hipMemcpyDeviceToDevice, s));
                 factorize_levels(*P, lo, *A);
             }
+            mark("rows+factor_launch");
             const GridInfo g = fg.get();
+            mark("grid_wait");
             P->grid = g;
             int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
             SweepPlan pl, pu;
             plan_sweep_pair(lo, up, g, cap, ftype, pl, pu);
+            mark("plan");
             P->lower.upload(std::move(pl), P->fstore.ptr, ftype);
             P->upper.upload(std::move(pu), P->fstore.ptr, ftype);
+            mark("upload");
             plan_line_pair(lo, up, g, P->lower, P->upper);
             P->tmp_bytes = 0;
         }
         M4G_HIP(hipStreamSynchronize(s));
+        mark("sync");
+        if (timing) std::fprintf(stderr, "m4g_pc_create: %s\n", tlog.c_str());
         *out = P.release();
         return M4G_OK;
     });
