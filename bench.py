@@ -11,7 +11,12 @@ Also measured in the same run: standalone SpMV (HBM GB/s, warm and with the
 Infinity Cache flushed between launches), the triangular-sweep kernel, both
 against the 8 TB/s HBM roofline using algorithmic bytes, and the CPU path
 (the oracle restatement of MTL4's algorithms, one host thread) on a bounded
-sample of the same solve.
+sample of the same solve; beside it the ELL and banded products, device
+assembly, the other Krylov solvers, Matrix Market I/O, the FP32-inner / FP64-
+refined solve on the 96^3 27-point convection-diffusion system, the batched
+512-system IC(0)-PCG, the differentiation of x = A(theta)^-1 b on the 256^2
+edge-diffusion grid and the 64^2 IC(0)-PCG case, each with its CPU
+counterpart.
 
 Prints one JSON line.
 """
