@@ -1348,6 +1348,22 @@ void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g
     pu = fu.get();
 }
 
+// Plans both directions and uploads them: the lower plan is uploaded (on this
+// thread, the library's stream) while the upper one is still being planned.
+void plan_upload_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
+                      SweepDev& dl, SweepDev& du, const double* fstore)
+{
+    const size_t budget = sweep_lds_budget();
+    auto fu = std::async(std::launch::async, [&] { return plan_sweep_budget(up, g, cap, ftype, budget); });
+    try {
+        dl.upload(plan_sweep_budget(lo, g, cap, ftype, budget), fstore, ftype);
+    } catch (...) {
+        fu.wait();
+        throw;
+    }
+    du.upload(fu.get(), fstore, ftype);
+}
+
 }  // namespace m4g
 
 using namespace m4g;
@@ -1472,12 +1488,8 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
             mark("grid_wait");
             P->grid = g;
             int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
-            SweepPlan pl, pu;
-            plan_sweep_pair(lo, up, g, cap, ftype, pl, pu);
-            mark("plan");
-            P->lower.upload(std::move(pl), P->fstore.ptr, ftype);
-            P->upper.upload(std::move(pu), P->fstore.ptr, ftype);
-            mark("upload");
+            plan_upload_pair(lo, up, g, cap, ftype, P->lower, P->upper, P->fstore.ptr);
+            mark("plan+upload");
             plan_line_pair(lo, up, g, P->lower, P->upper);
             P->tmp_bytes = 0;
         }
@@ -1580,10 +1592,7 @@ void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipS
             M4G_REQUIRE(ilu0_adjoint_rows(n, P.fS.data(), P.fI.data(), lo, up), M4G_ERR_MISSING_DIAGONAL,
                         "missing diagonal");
             int64_t cap = sweep_tile_cap(P.factor_type);
-            SweepPlan pl, pu;
-            plan_sweep_pair(lo, up, P.grid, cap, P.factor_type, pl, pu);
-            P.adj_lower.upload(std::move(pl), P.fstore.ptr, P.factor_type);
-            P.adj_upper.upload(std::move(pu), P.fstore.ptr, P.factor_type);
+            plan_upload_pair(lo, up, P.grid, cap, P.factor_type, P.adj_lower, P.adj_upper, P.fstore.ptr);
             plan_line_pair(lo, up, P.grid, P.adj_lower, P.adj_upper);
             P.adj_ready = true;
         }

@@ -57,6 +57,9 @@ SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int 
 // both directions of a preconditioner, planned on two host threads
 void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
                      SweepPlan& pl, SweepPlan& pu);
+// both directions planned, the lower uploaded while the upper is planned
+void plan_upload_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
+                      SweepDev& dl, SweepDev& du, const double* fstore);
 
 }  // namespace m4g
 
