@@ -156,8 +156,6 @@ typedef struct {
     int64_t steps_lower, steps_upper;     /* max in-tile levels over tiles */
     int64_t grid_nx, grid_ny, grid_nz;    /* inferred grid (0 if none) */
     int64_t tile_nx, tile_ny, tile_nz;    /* tile box (0 for row-block tiles) */
-    int64_t line_bands;                   /* bands of 64 grid lines of the line-pipelined sweeps (opt-in,
-                                             M4G_LINE_SWEEP=1) that run instead of the tiles on 2D 5-point grids (0: tiled) */
 } m4g_pc_stats;
 int m4g_pc_stats_get(m4g_pc P, m4g_pc_stats* stats);
 

@@ -1,9 +1,9 @@
 // Host-only analysis entry points: plan construction, validation and CPU
-// emulation of the tile-pipelined sweeps and (M4G_LINE_SWEEP=1) of the
-// line-pipelined ones (see mtl4gpu_analysis.h).
+// emulation of the tile-pipelined sweeps (see mtl4gpu_analysis.h).
 #include "../../include/mtl4gpu_analysis.h"
 #include "schedule.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -17,65 +17,6 @@ int64_t analysis_default_cap(int vtype) { return vtype == M4G_F64 ? 8192 : 16384
 
 using namespace m4g;
 
-namespace {
-
-// CPU emulation of k_line_sweep's schedule: bands in order, then steps, then
-// lanes, with the kernel's operations (selects for absent entries, the
-// previous lane's result of the previous step as the across-line value, band
-// k-1's line 64k-1 for lane 0).  Returns false on an index outside a buffer.
-template <typename FV, typename VT>
-bool emulate_line(const LinePlan& p, bool unit, const FV* f, const VT* v, VT* y)
-{
-    const int64_t plane = p.nbands * p.steps * 64;
-    std::vector<VT> last(p.nx), prev_last(p.nx);
-    for (int64_t k = 0; k < p.nbands; k++) {
-        std::vector<VT> yw(64, VT(0)), ynew(64);
-        for (int64_t st = 0; st < p.steps; st++) {
-            for (int64_t lane = 0; lane < 64; lane++) {
-                const int64_t j = 64 * k + lane, i = st - lane;
-                const bool act = j < p.ny && i >= 0 && i < p.nx;
-                const int64_t c = (k * p.steps + st) * 64 + lane;
-                auto coef = [&](int64_t plane_no, FV absent) {
-                    const int64_t q = p.pos[plane_no * plane + c];
-                    return q >= 0 ? f[q] : absent;
-                };
-                const FV w = coef(0, FV(0)), sv = coef(1, FV(0)), d = coef(2, FV(1));
-                const int64_t q = j * p.nx + i, r = p.rev ? p.n - 1 - q : q;
-                if (act && (r < 0 || r >= p.n)) return false;
-                const VT x0 = k > 0 && st < p.nx ? prev_last[st] : VT(0);
-                const VT ys = lane == 0 ? x0 : yw[lane - 1];
-                VT rr = act ? v[r] : VT(0);
-                const bool has_w = i > 0, has_s = j > 0;
-                if (p.s_first) {
-                    rr = has_s ? rr - VT(sv) * ys : rr;
-                    rr = has_w ? rr - VT(w) * yw[lane] : rr;
-                } else {
-                    rr = has_w ? rr - VT(w) * yw[lane] : rr;
-                    rr = has_s ? rr - VT(sv) * ys : rr;
-                }
-                if (!unit) rr = rr * VT(d);
-                if (act) {
-                    y[r] = rr;
-                    if (lane == 63) last[i] = rr;
-                }
-                ynew[lane] = rr;
-            }
-            yw.swap(ynew);
-        }
-        prev_last.swap(last);
-    }
-    return true;
-}
-
-template <typename FV, typename VT>
-bool emulate_line_pair(const LinePlan& pl, const LinePlan& pu, bool unit_l, bool unit_u, const FV* f, const VT* v,
-                       VT* y, int64_t n)
-{
-    std::vector<VT> tmp(n);
-    return emulate_line<FV, VT>(pl, unit_l, f, v, tmp.data()) && emulate_line<FV, VT>(pu, unit_u, f, tmp.data(), y);
-}
-
-}  // namespace
 
 extern "C" {
 
@@ -122,34 +63,12 @@ int m4g_analysis_pc_solve(int kind, int adjoint, int64_t n, const int64_t* S, co
             set_last_error("invalid plan: " + m1 + " / " + m2);
             return M4G_ERR_LOGIC;
         }
-        // M4G_LINE_SWEEP=1: the line-pipelined schedule where both directions have it
-        LinePlan ll, lu;
-        const char* le = std::getenv("M4G_LINE_SWEEP");
-        const bool line = le && le[0] == '1' && g.nx >= 32 && plan_line_sweep(lo, g, 8, ll) &&
-                          plan_line_sweep(up, g, 8, lu);
         size_t fn = 0;
         for (auto p : lo.val_pos) fn = std::max<size_t>(fn, size_t(p) + 1);
         for (auto p : up.val_pos) fn = std::max<size_t>(fn, size_t(p) + 1);
         for (auto p : lo.diag_pos) fn = std::max<size_t>(fn, size_t(p + 1));
         for (auto p : up.diag_pos) fn = std::max<size_t>(fn, size_t(p + 1));
-        if (line) {
-            bool ok;
-            if (ftype == M4G_F64 && vtype == M4G_F64) {
-                ok = emulate_line_pair<double, double>(ll, lu, pl.unit, pu.unit, factor, (const double*)rhs,
-                                                       (double*)out, n);
-            } else {
-                std::vector<float> f32(fn);
-                for (size_t i = 0; i < fn; i++) f32[i] = float(factor[i]);
-                ok = vtype == M4G_F64 ? emulate_line_pair<float, double>(ll, lu, pl.unit, pu.unit, f32.data(),
-                                                                         (const double*)rhs, (double*)out, n)
-                                      : emulate_line_pair<float, float>(ll, lu, pl.unit, pu.unit, f32.data(),
-                                                                        (const float*)rhs, (float*)out, n);
-            }
-            if (!ok) {
-                set_last_error("line plan indexes outside the vectors");
-                return M4G_ERR_LOGIC;
-            }
-        } else if (ftype == M4G_F64 && vtype == M4G_F64) {
+        if (ftype == M4G_F64 && vtype == M4G_F64) {
             std::vector<double> tmp(n);
             emulate_sweep<double, double>(pl, factor, (const double*)rhs, tmp.data());
             emulate_sweep<double, double>(pu, factor, tmp.data(), (double*)out);
@@ -182,7 +101,6 @@ int m4g_analysis_pc_solve(int kind, int adjoint, int64_t n, const int64_t* S, co
                 stats->tile_ny = sl.e2;
                 stats->tile_nz = sl.e1;
             }
-            if (line) stats->line_bands = ll.nbands;
         }
         return M4G_OK;
     } catch (const std::exception& e) {

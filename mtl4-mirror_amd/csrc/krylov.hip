@@ -693,7 +693,7 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
     run_batches(ws, it, body);
 #undef FIN
     int fault = 0;
-    if (P && P->kind >= M4G_PC_IC0) fault = check_sweep_fault(P->lower) | check_sweep_fault(P->upper);
+    if (P) fault = check_pc_faults(*P);
     return finish_state(ws, it, fault);
 }
 
@@ -770,7 +770,7 @@ static int bicgstab_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4
     run_batches(ws, it, body);
 #undef FIN
     int fault = 0;
-    if (P && P->kind >= M4G_PC_IC0) fault = check_sweep_fault(P->lower) | check_sweep_fault(P->upper);
+    if (P) fault = check_pc_faults(*P);
     return finish_state(ws, it, fault);
 }
 

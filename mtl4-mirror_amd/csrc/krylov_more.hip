@@ -208,7 +208,7 @@ void apply_pc(m4g_pc_s* P, const double* x, double* y, bool adjoint, const Ops& 
 
 int sweep_fault(m4g_pc_s* P)
 {
-    return P && P->kind >= M4G_PC_IC0 ? check_sweep_fault(P->lower) | check_sweep_fault(P->upper) : 0;
+    return P ? check_pc_faults(*P) : 0;
 }
 
 // status of a finished solve: the iteration's error, codes set by

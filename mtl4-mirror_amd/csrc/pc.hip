@@ -156,8 +156,6 @@ struct SweepArgs {
     int* fault;                       // set when a bounded wait runs out
     int64_t other_nexport;            // size of the other direction's exchange buffer
     int64_t spin_limit;
-    int64_t* trace;                   // optional: [tile][trace_words] wall-clock stamps (100 MHz)
-    int32_t trace_words;
 };
 
 // Workgroup roles: wavefront 0 imports, 1 computes, 2 and 3 load records,
@@ -167,10 +165,6 @@ constexpr int kSweepThreads = kSweepWaves * kWave;
 constexpr int kImportDepth = 4;       // imports each importer lane polls per window
 constexpr int kFillUnroll = 26;       // row values each thread moves at once between HBM and LDS
 constexpr int kFillLists = 8;         // import and export list entries per thread in the first fill pass
-// Trace words per tile: start, compute end, compute time, polls waiting for
-// records, polls waiting for imports, fill end, first group start, drain end,
-// first export stored, first import published.
-constexpr int kTraceWords = 10;
 
 // Byte offsets of the dynamic LDS regions of a sweep workgroup: header (the
 // workgroup's counters), values (rows, imports, one zero), group table, export
@@ -311,8 +305,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
     }
     __syncthreads();
     const int t = tile_sh;
-    int64_t* trace = a.trace ? a.trace + int64_t(t) * a.trace_words : nullptr;
-    if (trace && tid == 0) trace[0] = int64_t(wall_clock64());
     // the other direction's exchange buffer is free: reset it for its next use
     for (int64_t x = int64_t(t) * kSweepThreads + tid; x < a.other_nexport; x += a.ntiles * int64_t(kSweepThreads))
         xbuf_other[x] = Sentinel<VT>::value();
@@ -384,7 +376,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         tab[i] = e;
     }
     __syncthreads();
-    if (trace && tid == 0) trace[5] = int64_t(wall_clock64());
 
     if (wave == 0) {
         // ---- importer ----
@@ -396,9 +387,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         int head = 0;
         int64_t polls = 0;
         bool wide = true;
-#ifdef M4G_SWEEP_GROUP_TRACE
-        int igroup = 0;
-#endif
         while (head < ni) {
             int32_t xs[kImportDepth];
             VT val[kImportDepth];
@@ -426,17 +414,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
             }
             wide = first_missing > head;
             if (wide) {
-                if (trace && head == 0 && lane == 0) trace[9] = int64_t(wall_clock64());
-#ifdef M4G_SWEEP_GROUP_TRACE
-                // diagnostic build only: when the imports each group needs were published
-                if (trace && lane == 0) {
-                    const int64_t now = int64_t(wall_clock64());
-                    while (igroup < ng && tab[igroup].y <= first_missing) {
-                        trace[kTraceWords + 2 * a.max_steps + igroup] = now;
-                        igroup++;
-                    }
-                }
-#endif
                 head = first_missing;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 if (lane == 0) __hip_atomic_store(&progress, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -463,16 +440,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         // of its last record: they compute the same value and write it to the
         // same slot, so no lane is masked off.
         const i32x4* ringv = reinterpret_cast<const i32x4*>(ring);
-        // (No clock reads inside the loop: scalar memory operations there
-        // would make the compiler wait for every LDS read; waits are counted
-        // in polls instead.)
+        // (No clock reads or other scalar memory operations inside the loop:
+        // they would make the compiler wait for every LDS read.)
         int seen = 0;    // imports known to be in LDS
-        int64_t polls_ring = 0, polls_imp = 0, w_begin = trace ? int64_t(wall_clock64()) : 0;
         for (int b = 0; b < nb; b++) {
             __hip_atomic_store(&consumed, b * B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             while (__hip_atomic_load(half_batch + (b & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < b) {
                 __builtin_amdgcn_s_sleep(1);
-                polls_ring++;
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             const i32x4* rs = ringv + (b & 1) * (B * C * kWave) + lane;
@@ -494,14 +468,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
                     while ((seen = __hip_atomic_load(&progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) <
                            need) {
                         __builtin_amdgcn_s_sleep(1);
-                        polls_imp++;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
-#ifdef M4G_SWEEP_GROUP_TRACE
-                // diagnostic build only: start time of every group
-                if (trace && lane == 0) trace[kTraceWords + b * B + j] = int64_t(wall_clock64());
-#endif
                 VT yv[W];
 #pragma unroll
                 for (int k = 0; k < W; k++) yv[k] = *lds_at<VT>(cur.src[k]);
@@ -525,21 +494,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
                 __hip_atomic_store(&finished, b * B + j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        if (trace && lane == 0) {
-            trace[2] = int64_t(wall_clock64()) - w_begin;
-            trace[3] = polls_ring;
-            trace[4] = polls_imp;
-            trace[6] = w_begin;
-        }
     } else if (wave == 4) {
         // ---- exporter ----
         // Follows the compute wavefront and stores each finished exported row
         // (in finish order, so exchange slots xb, xb+1, ...) for later tiles.
         VT* xout = xbuf + xb;
         int e = 0;
-#ifdef M4G_SWEEP_GROUP_TRACE
-        int xgroup = 0;
-#endif
         while (e < nx) {
             const int fin = __hip_atomic_load(&finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const int avail = tab[fin].z;
@@ -548,19 +508,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
                 continue;
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (trace && e == 0 && lane == 0) trace[8] = int64_t(wall_clock64());
             for (int i = e + lane; i < avail; i += kWave)
                 __hip_atomic_store(xout + i, Sentinel<VT>::canon(lds[xl[i]]), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             e = avail;
-#ifdef M4G_SWEEP_GROUP_TRACE
-            // diagnostic build only: when each group's exports were issued
-            if (trace && lane == 0) {
-                const int64_t now = int64_t(wall_clock64());
-                for (int gg = xgroup; gg < fin; gg++) trace[kTraceWords + a.max_steps + gg] = now;
-            }
-            xgroup = fin;
-#endif
         }
     } else {
         // ---- loaders: wavefront 2 fills ring half 0, wavefront 3 half 1 ----
@@ -583,7 +534,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
-    if (trace && tid == 0) trace[1] = int64_t(wall_clock64());
     for (int32_t i0 = rb + tid; i0 < re; i0 += kFillUnroll * kSweepThreads) {
         int32_t r[kFillUnroll], sl[kFillUnroll];
 #pragma unroll
@@ -596,7 +546,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         for (int u = 0; u < kFillUnroll; u++)
             if (r[u] >= 0) y[r[u]] = lds[sl[u]];
     }
-    if (trace && tid == 0) trace[7] = int64_t(wall_clock64());
     if (tid == 0) {
         unsigned done = atomicAdd(&a.tickets[1], 1u);
         if (done == unsigned(a.ntiles) - 1) {   // every tile holds a ticket: reset for the next launch
@@ -671,219 +620,6 @@ __global__ void k_diag_setup(int64_t n, const int32_t* __restrict__ S, const int
         for (int32_t p = S[r]; p < S[r + 1]; p++)
             if (I[p] == r) d = V[p];
         inv[r] = 1.0 / d;   // reciprocal (diagonal.hpp:44)
-    }
-}
-
-// ---------------------------------------------------------------------------
-// line-pipelined sweep of a 2D 5-point grid (LinePlan)
-// ---------------------------------------------------------------------------
-constexpr int kLineGroup = 8;     // steps per group (one ring slot)
-constexpr int kLineLoaders = 6;   // loader wavefronts = ring slots
-constexpr int kLineThreads = 64 * (1 + kLineLoaders);
-
-struct LineArgs {
-    int64_t n, plane;            // rows; entries of one coefficient plane
-    int32_t nx, ny, nbands, steps, rev;
-    int64_t other_count;         // entries of the other direction's exchange buffer
-    unsigned* tickets;           // [0] next band, [1] finished bands
-    int* fault;
-    int64_t spin_limit;
-};
-
-// LDS ring of a line sweep workgroup: per slot and step, every lane's
-// right-hand side (then result), along-line, across-line and diagonal
-// value, and band k-1's cell for lane 0.
-template <typename FV, typename VT>
-struct LineRing {
-    VT v[kLineLoaders][kLineGroup][64];
-    FV w[kLineLoaders][kLineGroup][64], s[kLineLoaders][kLineGroup][64], d[kLineLoaders][kLineGroup][64];
-    VT x[kLineLoaders][kLineGroup];
-    int ready[kLineLoaders];     // group + 1 held by each slot
-    int done;                    // groups the compute wavefront has finished
-};
-
-// The value the previous lane holds (DPP wave_shr:1); lane 0 gets `first`.
-__device__ inline double prev_lane(double v, double first, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, int(b), 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x138, 0xF, 0xF, false);
-    const double sh = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-    return lane == 0 ? first : sh;
-}
-__device__ inline float prev_lane(float v, float first, int lane)
-{
-    const float sh = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
-    return lane == 0 ? first : sh;
-}
-
-// Bounded wait until an LDS counter reaches target; false when the spin
-// limit ran out or another wave flagged a fault (then the fault is set).
-__device__ inline bool line_wait_lds(int* p, int target, int64_t spin_limit, int* fault)
-{
-    int64_t polls = 0;
-    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++polls > spin_limit ||
-            ((polls & 63) == 0 && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return true;
-}
-
-// One band of 64 grid lines per workgroup (bands in ticket order).  Wavefront
-// 0 computes: lane l owns line j = 64k + l of the sweep's virtual numbering
-// and computes cell i = s - l at step s:  rr = v_r; rr -= L * y for the
-// across-line and the along-line entry in the rows' stored order; rr *= d_r
-// (lower_trisolve.hpp:191-209 / 172-189, upper_trisolve.hpp:159-176).  The
-// along-line value is the lane's own result of the previous step, the
-// across-line value the previous lane's (DPP) or, for lane 0, band k-1's
-// last line.  Absent entries are skipped, not multiplied by zero, so every
-// row's operations are the reference's.  The compute wavefront touches only
-// LDS and registers: loader wavefront L owns ring slot L and groups L, L+6,
-// ...; for each it writes the slot's previous results back (y, and line
-// 64k + 63 to the exchange buffer for band k+1), then loads the group's
-// right-hand sides, factor values and band k-1's cells (waiting, bounded,
-// for those to arrive) into the slot.
-template <typename FV, typename VT, bool UNIT, bool SFIRST>
-__global__ __launch_bounds__(kLineThreads) void k_line_sweep(LineArgs a, const FV* __restrict__ coef,
-                                                             const VT* __restrict__ v, VT* __restrict__ y,
-                                                             VT* xbuf, VT* xbuf_other, const int* skip)
-{
-    if (skip && *skip) return;   // the solve this sweep belongs to has finished
-    extern __shared__ unsigned char lds_raw[];
-    LineRing<FV, VT>& R = *reinterpret_cast<LineRing<FV, VT>*>(lds_raw);
-    __shared__ int band_sh;
-    const int tid = threadIdx.x, lane = tid % 64;
-    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
-    if (tid == 0) {
-        band_sh = int(atomicAdd(&a.tickets[0], 1u));
-        R.done = 0;
-    }
-    if (tid < kLineLoaders) R.ready[tid] = 0;
-    __syncthreads();
-    const int k = band_sh;
-    // the other direction's exchange buffer is free: reset it for its next use
-    for (int64_t x = int64_t(k) * kLineThreads + tid; x < a.other_count; x += int64_t(a.nbands) * kLineThreads)
-        xbuf_other[x] = Sentinel<VT>::value();
-    const int nx = a.nx, ngroups = a.steps / kLineGroup;
-    const int j = k * 64 + lane;
-    const bool line_ok = j < a.ny;
-    const int64_t base = int64_t(j) * nx;   // virtual row of cell (0, j)
-    if (wave == 0) {
-        const bool has_s = j > 0;
-        VT yw = VT(0);
-        for (int g = 0; g < ngroups; g++) {
-            const int sl = g % kLineLoaders;
-            if (!line_wait_lds(&R.ready[sl], g + 1, a.spin_limit, a.fault)) break;
-            VT vv[kLineGroup], xx[kLineGroup];
-            FV ww[kLineGroup], ss[kLineGroup], dd[kLineGroup];
-#pragma unroll
-            for (int u = 0; u < kLineGroup; u++) {
-                vv[u] = R.v[sl][u][lane];
-                ww[u] = R.w[sl][u][lane];
-                ss[u] = R.s[sl][u][lane];
-                dd[u] = R.d[sl][u][lane];
-                xx[u] = R.x[sl][u];
-            }
-#pragma unroll
-            for (int u = 0; u < kLineGroup; u++) {
-                const int i = g * kLineGroup + u - lane;
-                const VT ys = prev_lane(yw, xx[u], lane);
-                const bool has_w = i > 0;
-                VT rr = vv[u];
-                if (SFIRST) {
-                    rr = has_s ? rr - VT(ss[u]) * ys : rr;
-                    rr = has_w ? rr - VT(ww[u]) * yw : rr;
-                } else {
-                    rr = has_w ? rr - VT(ww[u]) * yw : rr;
-                    rr = has_s ? rr - VT(ss[u]) * ys : rr;
-                }
-                if (!UNIT) rr = rr * VT(dd[u]);
-                R.v[sl][u][lane] = rr;
-                yw = rr;
-            }
-            // LDS serves a wavefront's operations in order: the loader that
-            // sees this count also sees the results written above
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (lane == 0) __hip_atomic_store(&R.done, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    } else {
-        const int L = wave - 1;
-        const FV* cw = coef + int64_t(k) * a.steps * 64 + lane;
-        VT* xin = k > 0 ? xbuf + int64_t(k - 1) * nx : nullptr;
-        VT* xout = k + 1 < a.nbands ? xbuf + int64_t(k) * nx : nullptr;
-        for (int g = L; g < ngroups + kLineLoaders; g += kLineLoaders) {
-            const int gp = g - kLineLoaders;   // the group the slot held before
-            if (gp >= 0) {
-                if (!line_wait_lds(&R.done, gp + 1, a.spin_limit, a.fault)) break;
-#pragma unroll
-                for (int u = 0; u < kLineGroup; u++) {
-                    const int i = gp * kLineGroup + u - lane;
-                    if (line_ok && i >= 0 && i < nx) {
-                        const VT rr = R.v[L][u][lane];
-                        const int64_t q = base + i;
-                        y[a.rev ? a.n - 1 - q : q] = rr;
-                        if (xout && lane == 63)
-                            __hip_atomic_store(xout + i, Sentinel<VT>::canon(rr), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
-            }
-            if (g >= ngroups) break;
-            VT vv[kLineGroup];
-            FV ww[kLineGroup], ss[kLineGroup], dd[kLineGroup];
-#pragma unroll
-            for (int u = 0; u < kLineGroup; u++) {
-                const int st = g * kLineGroup + u, i = st - lane;
-                // every lane loads (an idle one row 0, in bounds), so the
-                // group's loads issue back to back and are waited for once
-                const bool act = line_ok && i >= 0 && i < nx;
-                const int64_t q = act ? base + i : 0;
-                const VT t = v[a.rev ? a.n - 1 - q : q];
-                vv[u] = act ? t : VT(0);
-                const int64_t c = int64_t(st) * 64;
-                ww[u] = cw[c];
-                ss[u] = cw[a.plane + c];
-                dd[u] = cw[2 * a.plane + c];
-            }
-            VT xv = VT(0);
-            const int xs = g * kLineGroup + lane;   // lanes 0..7: band k-1's cells of the group
-            if (xin && lane < kLineGroup && xs < nx) {
-                int64_t polls = 0;
-                while (Sentinel<VT>::is(xv = __hip_atomic_load(xin + xs, __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT))) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++polls > a.spin_limit ||
-                        ((polls & 63) == 0 && __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                        __hip_atomic_store(a.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        xv = VT(0);   // the results are meaningless; the host reports the fault
-                        break;
-                    }
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kLineGroup; u++) {
-                R.v[L][u][lane] = vv[u];
-                R.w[L][u][lane] = ww[u];
-                R.s[L][u][lane] = ss[u];
-                R.d[L][u][lane] = dd[u];
-            }
-            if (lane < kLineGroup) R.x[L][lane] = xv;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (lane == 0) __hip_atomic_store(&R.ready[L], g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned done = atomicAdd(&a.tickets[1], 1u);
-        if (done == unsigned(a.nbands) - 1) {   // every band holds a ticket: reset for the next launch
-            a.tickets[0] = 0;
-            a.tickets[1] = 0;
-        }
     }
 }
 
@@ -987,7 +723,6 @@ void SweepDev::refresh_values(const double* f, int ftype)
     factor_type = ftype;
     fsrc = f;
     recs_vbytes = 0;
-    line_coef_ok = false;
 }
 
 void SweepDev::ensure_recs(int vbytes, hipStream_t s)
@@ -1013,74 +748,13 @@ void SweepDev::ensure_exchange(int vtype)
     M4G_HIP(hipGetLastError());
 }
 
-void SweepDev::upload_line(LinePlan&& p)
-{
-    line_pos.reset(std::max<size_t>(p.pos.size(), 1));
-    line_pos.upload(p.pos.data(), p.pos.size());
-    p.pos.clear();
-    p.pos.shrink_to_fit();
-    line_meta = std::move(p);
-    line = true;
-    line_coef_ok = false;
-    line_xbuf_type = -1;
-    line_tickets.reset(2);
-    M4G_HIP(hipMemsetAsync(line_tickets.ptr, 0, 2 * sizeof(unsigned), current_stream()));
-}
 
-void SweepDev::ensure_line(int vtype, hipStream_t s)
-{
-    const LinePlan& p = line_meta;
-    const int64_t plane = p.nbands * p.steps * 64;
-    if (!line_coef_ok) {
-        const size_t fb = factor_type == M4G_F64 ? 8 : 4;
-        if (line_coef.count != size_t(3 * plane) * fb) line_coef.reset(size_t(3 * plane) * fb);
-        for (int c = 0; c < 3; c++) {
-            const double absent = c == 2 ? 1.0 : 0.0;   // unit diagonal; no entry
-            if (factor_type == M4G_F64)
-                hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(plane)), dim3(256), 0, s, plane,
-                                   line_pos.ptr + c * plane, fsrc, (double*)line_coef.ptr + c * plane, absent);
-            else
-                hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(plane)), dim3(256), 0, s, plane,
-                                   line_pos.ptr + c * plane, fsrc, (float*)line_coef.ptr + c * plane, float(absent));
-        }
-        M4G_HIP(hipGetLastError());
-        line_coef_ok = true;
-    }
-    const int64_t cnt = std::max<int64_t>((p.nbands - 1) * p.nx, 1);
-    const size_t bytes = size_t(cnt) * (vtype == M4G_F64 ? 8 : 4);
-    if (line_xbuf_type == vtype && line_xbuf.count == bytes) return;
-    line_xbuf.reset(bytes);
-    line_xbuf_type = vtype;
-    if (vtype == M4G_F64)
-        hipLaunchKernelGGL(k_fill_sentinel<double>, dim3(grid_for(cnt)), dim3(256), 0, s, cnt, (double*)line_xbuf.ptr);
-    else
-        hipLaunchKernelGGL(k_fill_sentinel<float>, dim3(grid_for(cnt)), dim3(256), 0, s, cnt, (float*)line_xbuf.ptr);
-    M4G_HIP(hipGetLastError());
-}
-
-void plan_line_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, SweepDev& dl, SweepDev& du)
-{
-    dl.line = du.line = false;
-    // opt-in: on MI355X the tiled sweeps are faster on every grid measured
-    // (a 256 x 256 sweep: 88 us tiled, 191 us line-pipelined), so the line
-    // form is kept as an exact alternative, selected by M4G_LINE_SWEEP=1
-    const char* e = std::getenv("M4G_LINE_SWEEP");
-    if (!e || e[0] != '1') return;
-    // short lines or many bands pipeline poorly: the tiled form is kept there
-    if (g.nx < 32 || g.ny > 64 * 64) return;
-    LinePlan pl, pu;
-    if (!plan_line_sweep(lo, g, kLineGroup, pl) || !plan_line_sweep(up, g, kLineGroup, pu)) return;
-    dl.upload_line(std::move(pl));
-    du.upload_line(std::move(pu));
-}
-
+// Polls a sweep's bounded waits make before giving up (M4G_SPIN_LIMIT; read
+// at every launch, so a process can change it between solves).
 static int64_t spin_limit()
 {
-    static int64_t lim = [] {
-        const char* e = std::getenv("M4G_SPIN_LIMIT");
-        return e ? std::max<int64_t>(1, std::atoll(e)) : int64_t(1) << 20;   // about one second
-    }();
-    return lim;
+    const char* e = std::getenv("M4G_SPIN_LIMIT");
+    return e ? std::max<int64_t>(1, std::atoll(e)) : int64_t(1) << 20;   // about one second
 }
 
 // Dynamic LDS of a sweep launch (layout in sweep_lds_layout).
@@ -1139,21 +813,6 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
     a.fault = d.fault.ptr;
     a.other_nexport = other.plan_meta.nexport;
     a.spin_limit = spin_limit();
-    a.trace = nullptr;
-    a.trace_words = kTraceWords;
-    static const char* trace_path = std::getenv("M4G_SWEEP_TRACE");
-    static int traced = 0;
-    DevBuf<int64_t> trace_buf;
-    if (trace_path && traced < 4) {
-#ifdef M4G_SWEEP_GROUP_TRACE
-        a.trace_words = int32_t(kTraceWords + 3 * p.max_steps);
-#else
-        a.trace_words = kTraceWords;
-#endif
-        trace_buf.reset(size_t(p.ntiles) * a.trace_words);
-        M4G_HIP(hipMemsetAsync(trace_buf.ptr, 0, trace_buf.count * sizeof(int64_t), s));
-        a.trace = trace_buf.ptr;
-    }
     VT* xo = (VT*)other.xbuf.ptr;
     auto go = [&](auto unit_tag) {
         constexpr bool U = decltype(unit_tag)::value;
@@ -1168,83 +827,13 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
     if (p.unit) go(std::true_type());
     else go(std::false_type());
     M4G_HIP(hipGetLastError());
-    if (a.trace) {
-        // trace file: header (ntiles, groups, words) then the stamps of every tile
-        std::vector<int64_t> h(trace_buf.count);
-        trace_buf.download(h.data(), h.size());
-        std::string path = std::string(trace_path) + "." + std::to_string(traced++);
-        if (FILE* f = std::fopen(path.c_str(), "wb")) {
-            int64_t hdr[3] = {p.ntiles, p.max_steps, a.trace_words};
-            std::fwrite(hdr, sizeof(hdr), 1, f);
-            std::fwrite(h.data(), sizeof(int64_t), h.size(), f);
-            std::fclose(f);
-        }
-    }
 }
 
-// Forward sweep on `first` then backward on `second`, v -> tmp -> y.
-template <typename FV, typename VT>
-static void launch_line_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, const int* skip, hipStream_t s)
-{
-    const LinePlan& p = d.line_meta;
-    if (p.n == 0) return;
-    LineArgs a;
-    a.n = p.n;
-    a.plane = p.nbands * p.steps * 64;
-    a.nx = int32_t(p.nx);
-    a.ny = int32_t(p.ny);
-    a.nbands = int32_t(p.nbands);
-    a.steps = int32_t(p.steps);
-    a.rev = p.rev ? 1 : 0;
-    a.other_count = int64_t(other.line_xbuf.count / sizeof(VT));
-    a.tickets = d.line_tickets.ptr;
-    a.fault = d.fault.ptr;
-    a.spin_limit = spin_limit();
-    const FV* coef = (const FV*)d.line_coef.ptr;
-    VT* xb = (VT*)d.line_xbuf.ptr;
-    VT* xo = (VT*)other.line_xbuf.ptr;
-    const size_t lds = sizeof(LineRing<FV, VT>);
-    // beyond 64 KB of dynamic LDS must be requested, once per kernel
-    static bool granted = false;
-    if (lds > 65536 && !granted) {
-        for (const void* kern : {reinterpret_cast<const void*>(k_line_sweep<FV, VT, true, true>),
-                                 reinterpret_cast<const void*>(k_line_sweep<FV, VT, true, false>),
-                                 reinterpret_cast<const void*>(k_line_sweep<FV, VT, false, true>),
-                                 reinterpret_cast<const void*>(k_line_sweep<FV, VT, false, false>)})
-            M4G_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-        granted = true;
-    }
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(unsigned(p.nbands)), dim3(kLineThreads), lds, s, a, coef, v, y, xb, xo, skip);
-    };
-    const bool unit = d.plan_meta.unit;
-    if (unit && p.s_first) go(k_line_sweep<FV, VT, true, true>);
-    else if (unit) go(k_line_sweep<FV, VT, true, false>);
-    else if (p.s_first) go(k_line_sweep<FV, VT, false, true>);
-    else go(k_line_sweep<FV, VT, false, false>);
-    M4G_HIP(hipGetLastError());
-}
 
 void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp, void* y, int vtype,
                     const int* skip, hipStream_t s)
 {
     M4G_REQUIRE(first.factor_type == second.factor_type, M4G_ERR_INVALID, "sweep factor types differ");
-    if (first.line && second.line) {
-        first.ensure_line(vtype, s);
-        second.ensure_line(vtype, s);
-        if (first.factor_type == M4G_F64) {
-            M4G_REQUIRE(vtype == M4G_F64, M4G_ERR_INVALID, "double factors need double vectors");
-            launch_line_t<double, double>(first, second, (const double*)v, (double*)tmp, skip, s);
-            launch_line_t<double, double>(second, first, (const double*)tmp, (double*)y, skip, s);
-        } else if (vtype == M4G_F64) {
-            launch_line_t<float, double>(first, second, (const double*)v, (double*)tmp, skip, s);
-            launch_line_t<float, double>(second, first, (const double*)tmp, (double*)y, skip, s);
-        } else {
-            launch_line_t<float, float>(first, second, (const float*)v, (float*)tmp, skip, s);
-            launch_line_t<float, float>(second, first, (const float*)tmp, (float*)y, skip, s);
-        }
-        return;
-    }
     first.ensure_exchange(vtype);
     second.ensure_exchange(vtype);
     if (first.factor_type == M4G_F64) {
@@ -1260,12 +849,30 @@ void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp,
     }
 }
 
+// Reads the sweep's fault flag and clears it: a wait that gave up spoils the
+// solve that launched it, and only that one.
 int check_sweep_fault(SweepDev& d)
 {
     int h = 0;
     M4G_HIP(hipMemcpyAsync(&h, d.fault.ptr, sizeof(int), hipMemcpyDeviceToHost, current_stream()));
     M4G_HIP(hipStreamSynchronize(current_stream()));
+    if (h) {
+        M4G_HIP(hipMemsetAsync(d.fault.ptr, 0, sizeof(int), current_stream()));
+        M4G_HIP(hipStreamSynchronize(current_stream()));
+    }
     return h;
+}
+
+int check_pc_faults(m4g_pc_s& P)
+{
+    if (P.kind < M4G_PC_IC0 || P.n == 0) return 0;
+    int f = check_sweep_fault(P.lower);
+    f |= check_sweep_fault(P.upper);
+    if (P.adj_ready) {
+        f |= check_sweep_fault(P.adj_lower);
+        f |= check_sweep_fault(P.adj_upper);
+    }
+    return f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1490,7 +1097,6 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
             int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
             plan_upload_pair(lo, up, g, cap, ftype, P->lower, P->upper, P->fstore.ptr);
             mark("plan+upload");
-            plan_line_pair(lo, up, g, P->lower, P->upper);
             P->tmp_bytes = 0;
         }
         M4G_HIP(hipStreamSynchronize(s));
@@ -1593,7 +1199,6 @@ void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipS
                         "missing diagonal");
             int64_t cap = sweep_tile_cap(P.factor_type);
             plan_upload_pair(lo, up, P.grid, cap, P.factor_type, P.adj_lower, P.adj_upper, P.fstore.ptr);
-            plan_line_pair(lo, up, P.grid, P.adj_lower, P.adj_upper);
             P.adj_ready = true;
         }
         run_sweep_pair(P.adj_lower, P.adj_upper, x, tmp, y, vtype, skip, s);
@@ -1621,11 +1226,7 @@ static int pc_solve_common(m4g_pc P, m4g_vector x, m4g_vector y, bool adjoint)
         M4G_REQUIRE(x->ptr != y->ptr || P->n == 0 || P->kind <= M4G_PC_DIAGONAL, M4G_ERR_INVALID,
                     "m4g_pc_solve: x and y alias");
         pc_apply(*P, x->ptr, y->ptr, x->type, adjoint, current_stream());
-        if (P->kind >= M4G_PC_IC0 && P->n) {
-            SweepDev& d = adjoint && P->kind == M4G_PC_ILU0 ? P->adj_lower : P->lower;
-            M4G_REQUIRE(check_sweep_fault(d) == 0 && check_sweep_fault(adjoint && P->kind == M4G_PC_ILU0 ? P->adj_upper : P->upper) == 0,
-                        M4G_ERR_TIMEOUT, "sweep wait timed out");
-        }
+        M4G_REQUIRE(check_pc_faults(*P) == 0, M4G_ERR_TIMEOUT, "sweep wait timed out");
         return M4G_OK;
     });
 }
@@ -1736,7 +1337,6 @@ int m4g_pc_stats_get(m4g_pc P, m4g_pc_stats* st)
             st->tile_ny = l.shape.e2;
             st->tile_nz = l.shape.e1;
         }
-        if (P->lower.line && P->upper.line) st->line_bands = P->lower.line_meta.nbands;
         return M4G_OK;
     });
 }

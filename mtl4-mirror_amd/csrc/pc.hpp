@@ -26,28 +26,11 @@ struct SweepDev {
     DevBuf<int> fault;
     int factor_type = M4G_F64;
 
-    // line-pipelined form of a 2D 5-point grid sweep (LinePlan); launched
-    // instead of the tiled one when both directions of a pair have it
-    bool line = false;
-    LinePlan line_meta;                   // scalars (positions dropped after the upload)
-    DevBuf<int64_t> line_pos;             // three planes of factor-storage indices
-    DevBuf<unsigned char> line_coef;      // the same planes packed as factor_type values
-    bool line_coef_ok = false;
-    DevBuf<unsigned char> line_xbuf;      // band k's last line for band k+1, sentinel when empty
-    int line_xbuf_type = -1;
-    DevBuf<unsigned> line_tickets;
-
     void upload(const SweepPlan& p, const double* fstore_dev, int ftype);
-    void upload_line(LinePlan&& p);
     void refresh_values(const double* fstore_dev, int ftype);
     void ensure_recs(int vbytes, hipStream_t s);
     void ensure_exchange(int vtype);
-    void ensure_line(int vtype, hipStream_t s);
 };
-
-// Plans the line-pipelined form of both directions of a pair and uploads it
-// to both, or to neither (a pair runs one form).  Only with M4G_LINE_SWEEP=1.
-void plan_line_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, SweepDev& dl, SweepDev& du);
 
 void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp, void* y, int vtype,
                     const int* skip, hipStream_t s);
@@ -87,4 +70,7 @@ namespace m4g {
 // y = P^{-1} x (or the adjoint) on stream s; vtype is the vectors' value type.
 void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipStream_t s,
               const int* skip = nullptr);
+// Reads and clears the fault flags of every sweep of P (nonzero: a bounded
+// wait of a sweep launched since the last check gave up).
+int check_pc_faults(m4g_pc_s& P);
 }  // namespace m4g

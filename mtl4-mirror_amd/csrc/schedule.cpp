@@ -827,58 +827,4 @@ bool ilu0_adjoint_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows&
     return true;
 }
 
-bool plan_line_sweep(const SweepRows& rows, const GridInfo& g, int unroll, LinePlan& out)
-{
-    const int64_t n = rows.n, nx = g.nx, ny = g.ny;
-    if (nx < 2 || ny < 1 || g.nz > 1 || nx * ny != n || unroll < 1) return false;
-    const bool lower = rows.lower;
-    int order = -1;   // 1: the across-line entry first, 0: the along-line entry first
-    std::vector<int64_t> wpos(n, -1), spos(n, -1);   // by virtual row
-    for (int64_t r = 0; r < n; r++) {
-        const int64_t q = lower ? r : n - 1 - r, i = q % nx, j = q / nx;
-        int64_t w = -1, sp = -1;
-        int first = -1;
-        for (int64_t e = rows.ptr[r]; e < rows.ptr[r + 1]; e++) {
-            const int64_t c = rows.col[e], qc = lower ? c : n - 1 - c;
-            if (q - qc == 1 && i > 0 && w < 0) {
-                w = rows.val_pos[e];
-                if (first < 0) first = 0;
-            } else if (q - qc == nx && j > 0 && sp < 0) {
-                sp = rows.val_pos[e];
-                if (first < 0) first = 1;
-            } else {
-                return false;
-            }
-        }
-        if ((w >= 0) != (i > 0) || (sp >= 0) != (j > 0)) return false;
-        if (w >= 0 && sp >= 0) {
-            if (order >= 0 && order != first) return false;
-            order = first;
-        }
-        wpos[q] = w;
-        spos[q] = sp;
-    }
-    out.n = n;
-    out.nx = nx;
-    out.ny = ny;
-    out.nbands = (ny + 63) / 64;
-    out.steps = (nx + 63 + unroll - 1) / unroll * unroll;
-    out.rev = !lower;
-    out.s_first = order == 1;
-    const int64_t plane = out.nbands * out.steps * 64;
-    out.pos.assign(size_t(3 * plane), -1);
-    for (int64_t k = 0; k < out.nbands; k++)
-        for (int64_t st = 0; st < out.steps; st++)
-            for (int64_t l = 0; l < 64; l++) {
-                const int64_t j = 64 * k + l, i = st - l;
-                if (j >= ny || i < 0 || i >= nx) continue;
-                const int64_t q = i + nx * j, r = lower ? q : n - 1 - q;
-                const int64_t x = (k * out.steps + st) * 64 + l;
-                out.pos[x] = wpos[q];
-                out.pos[plane + x] = spos[q];
-                out.pos[2 * plane + x] = rows.diag_pos[r];
-            }
-    return true;
-}
-
 }  // namespace m4g

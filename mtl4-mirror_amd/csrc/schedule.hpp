@@ -73,24 +73,6 @@ struct SweepPlan {
     int64_t max_tile_imports = 0, nimports = 0;
 };
 
-// Line-pipelined sweep of a 2D 5-point grid (nz == 1).  In the sweep's own
-// direction (virtual row q = r for a lower sweep, q = n-1-r for an upper one;
-// q = i + nx*j), every row depends on exactly q-1 when i > 0 and q-nx when
-// j > 0, subtracted in the same order in every row.  Lane l of band k owns
-// grid line j = 64k + l and computes cell i = s - l at step s, so each step
-// is one anti-diagonal of the band; the cell below comes from the previous
-// lane (or, for lane 0, from band k-1's last line through an exchange buffer).
-// pos holds three planes [w | s | d][band][step][lane] of factor-storage
-// indices (-1: none) of the along-line entry, the across-line entry and the
-// diagonal.
-struct LinePlan {
-    int64_t n = 0, nx = 0, ny = 0, nbands = 0, steps = 0;   // steps per band, padded to `unroll`
-    bool rev = false, s_first = false;
-    std::vector<int64_t> pos;
-};
-// Fills `out` and returns true when the rows have the structure above.
-bool plan_line_sweep(const SweepRows& rows, const GridInfo& g, int unroll, LinePlan& out);
-
 // Entries per piece for rows of at most max_len entries.
 int sweep_piece_width(int64_t max_len);
 

@@ -55,7 +55,7 @@ _vp = ctypes.c_void_p
 class PcStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "levels_lower", "levels_upper", "tiles_lower", "tiles_upper", "steps_lower", "steps_upper",
-        "grid_nx", "grid_ny", "grid_nz", "tile_nx", "tile_ny", "tile_nz", "line_bands")]
+        "grid_nx", "grid_ny", "grid_nz", "tile_nx", "tile_ny", "tile_nz")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -415,7 +415,12 @@ class Vector:
         if tensor is not None:
             import torch
             assert tensor.is_cuda and tensor.is_contiguous(), "tensor must be contiguous on the GPU"
-            code = F64 if tensor.dtype == torch.float64 else F32
+            if tensor.dtype == torch.float64:
+                code = F64
+            elif tensor.dtype == torch.float32:
+                code = F32
+            else:
+                raise TypeError(f"Vector(tensor=...) takes float64 or float32 tensors, not {tensor.dtype}")
             self.n, self.dtype = tensor.numel(), _np_dtype(code)
             _check(_f["m4g_vector_wrap"](ctypes.byref(h), ctypes.c_void_p(tensor.data_ptr()), self.n, code),
                    "m4g_vector_wrap")
