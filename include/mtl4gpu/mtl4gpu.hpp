@@ -244,6 +244,8 @@ class dense_vector {
     dense_vector(const mat_cvec_times_expr<Matrix, dense_vector>& e) { *this = e; }
     template <typename Matrix>
     dense_vector(const residual_expr<dense_vector, Matrix>& e) { *this = e; }
+    template <typename Matrix>
+    dense_vector(const product_minus_expr<dense_vector, Matrix>& e) { *this = e; }
     dense_vector(const vec_sum_expr<dense_vector>& e) { *this = e; }
     ~dense_vector()
     {
