@@ -110,6 +110,19 @@ int m4g_vector_copy(m4g_vector dst, m4g_vector src);      /* dst = src, converti
 /* ---- level-1 kernels ------------------------------------------------------ */
 /* dot(x, y) = sum x_i * y_i (mtl/operation/dot.hpp:62-256) */
 int m4g_dot(m4g_vector x, m4g_vector y, double* result);
+/* dot(x, y) as m4g_dot, for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It calls no hipMalloc,
+ * hipFree or other allocating routine: the reduction partials and the result
+ * live in a file-scope device array of 1056 doubles (8448 bytes) per GPU,
+ * held until the process ends; its address is looked up when the device is
+ * set up (m4g_set_device or the first handle made on it), and a device that
+ * is not set up is an error.  Any n fits (the reduction uses at most 1024
+ * partials); a size beyond the scratch is rejected with M4G_ERR_SIZE.  The
+ * result is bit-identical to m4g_dot's.  Every call on a GPU shares that
+ * array, so calls that overlap on one GPU (several host threads or streams)
+ * race. */
+int m4g_dot_noalloc(m4g_vector x, m4g_vector y, double* result);
 /* two_norm(x) (mtl/operation/two_norm.hpp:33-61) */
 int m4g_two_norm(m4g_vector x, double* result);
 /* unary_dot(x) = sum x_i^2 (mtl/operation/unary_dot.hpp:29-65) */

@@ -105,12 +105,18 @@ static int stream_grid(int64_t n)
     return int(g);
 }
 
+int reduce_partials(int64_t n) { return reduce_grid(n); }
+
+void launch_dot_with(int64_t n, const double* x, const double* y, double* partials, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, y, partials, nullptr, out);
+    launch_sum_partials(partials, reduce_grid(n), out, s);
+    M4G_HIP(hipGetLastError());
+}
+
 void launch_dot(int64_t n, const double* x, const double* y, double* out, hipStream_t s)
 {
-    ReduceWs& ws = reduce_ws();
-    hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, y, ws.partials, ws.counters, out);
-    launch_sum_partials(ws.partials, reduce_grid(n), out, s);
-    M4G_HIP(hipGetLastError());
+    launch_dot_with(n, x, y, reduce_ws().partials, out, s);
 }
 
 void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s)

@@ -69,9 +69,11 @@ struct DeviceState {
     std::mutex pool_mutex;
     std::multimap<size_t, void*> pool; // freed buffers by size, reused exactly
     double* pinned = nullptr;          // small pinned host staging buffer (scalars)
+    double* noalloc_scratch = nullptr; // this device's copy of m4g_dot_noalloc's scratch array
 };
 
-DeviceState& device_state();          // state of the calling thread's device
+DeviceState& device_state();          // state of the calling thread's device (created on first use)
+DeviceState* existing_device_state(); // the same, or nullptr when it has not been set up
 hipStream_t current_stream();          // stream for this thread's work
 
 void* pool_alloc(size_t bytes);
@@ -187,6 +189,9 @@ void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* 
 
 // Reductions into a device scalar (deterministic two-stage).
 void launch_dot(int64_t n, const double* x, const double* y, double* out, hipStream_t s);
+// dot with caller-provided partials (room for reduce_partials(n) doubles)
+int reduce_partials(int64_t n);
+void launch_dot_with(int64_t n, const double* x, const double* y, double* partials, double* out, hipStream_t s);
 void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s);
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s);
 void launch_xpay(int64_t n, const double* x, double beta, double* y, hipStream_t s);

@@ -14,6 +14,17 @@ void set_last_error(const std::string& msg) { tl_last_error = msg; }
 static std::mutex g_states_mutex;
 static std::map<int, std::unique_ptr<DeviceState>> g_states;
 
+// Scratch of m4g_dot_noalloc, the entry point that allocates nothing: the
+// reduction partials (at most kReduceBlocks, one per workgroup of the first
+// stage) and, after them, the result slot.  One copy per GPU, held until the
+// process ends; every m4g_dot_noalloc call on a GPU shares it, so calls that
+// overlap on one GPU race.  In host code `scratch` names a host-side stand-in:
+// only the address hipGetSymbolAddress returns for it (kept per device in
+// DeviceState::noalloc_scratch) may reach a kernel or a copy.
+constexpr int kNoAllocResult = kReduceBlocks;          // index of the result slot
+constexpr int kNoAllocSlots = kReduceBlocks + 32;      // 8448 bytes
+static __device__ __align__(256) double scratch[kNoAllocSlots];
+
 DeviceState& device_state()
 {
     std::lock_guard<std::mutex> lock(g_states_mutex);
@@ -34,9 +45,24 @@ DeviceState& device_state()
     M4G_HIP(hipGetDeviceProperties(&prop, tl_device));
     st->num_cus = prop.multiProcessorCount;
     M4G_HIP(hipHostMalloc(reinterpret_cast<void**>(&st->pinned), 64 * sizeof(double), hipHostMallocDefault));
+    // this device's copy of the no-allocation scratch (the device is current)
+    void* sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sp)
+        throw Error(M4G_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+    st->noalloc_scratch = static_cast<double*>(sp);
     DeviceState& ref = *st;
     g_states[tl_device] = std::move(st);
     return ref;
+}
+
+DeviceState* existing_device_state()
+{
+    std::lock_guard<std::mutex> lock(g_states_mutex);
+    auto it = g_states.find(tl_device);
+    if (it == g_states.end()) return nullptr;
+    M4G_HIP(hipSetDevice(tl_device));
+    return it->second.get();
 }
 
 hipStream_t current_stream()
@@ -436,6 +462,33 @@ int m4g_dot(m4g_vector x, m4g_vector y, double* result)
         double* out = ws.partials + kReduceBlocks * 16;
         launch_dot(x->n, (const double*)x->ptr, (const double*)y->ptr, out, current_stream());
         *result = read_scalar(out);
+        return M4G_OK;
+    });
+}
+
+int m4g_dot_noalloc(m4g_vector x, m4g_vector y, double* result)
+{
+    return guard([&] {
+        check_f64(x, "m4g_dot_noalloc");
+        check_f64(y, "m4g_dot_noalloc");
+        M4G_REQUIRE(result, M4G_ERR_INVALID, "m4g_dot_noalloc: null result");
+        M4G_REQUIRE(x->n == y->n, M4G_ERR_SIZE, "m4g_dot_noalloc: incompatible size");
+        // set-up (m4g_set_device, or the first handle made on this device)
+        // created the state and looked up the scratch; nothing is created here
+        DeviceState* st = existing_device_state();
+        M4G_REQUIRE(st && st->noalloc_scratch, M4G_ERR_INVALID,
+                    "m4g_dot_noalloc: the device is not set up (call m4g_set_device first)");
+        const int parts = reduce_partials(x->n);
+        M4G_REQUIRE(parts <= kNoAllocResult, M4G_ERR_SIZE,
+                    "m4g_dot_noalloc: more reduction partials than the scratch holds");
+        hipStream_t s = tl_stream ? tl_stream : st->stream;
+        double* out = st->noalloc_scratch + kNoAllocResult;
+        // every partial the second stage reads is written by the first, and the
+        // result slot by the second: nothing left from an earlier call is read
+        launch_dot_with(x->n, (const double*)x->ptr, (const double*)y->ptr, st->noalloc_scratch, out, s);
+        M4G_HIP(hipMemcpyAsync(st->pinned + 1, out, sizeof(double), hipMemcpyDeviceToHost, s));
+        M4G_HIP(hipStreamSynchronize(s));
+        *result = st->pinned[1];
         return M4G_OK;
     });
 }

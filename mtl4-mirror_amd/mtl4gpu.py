@@ -99,6 +99,7 @@ _SIGS = {
     "m4g_vector_fill": (ctypes.c_int, [_vp, ctypes.c_double]),
     "m4g_vector_copy": (ctypes.c_int, [_vp, _vp]),
     "m4g_dot": (ctypes.c_int, [_vp, _vp, _c_dp]),
+    "m4g_dot_noalloc": (ctypes.c_int, [_vp, _vp, _c_dp]),
     "m4g_two_norm": (ctypes.c_int, [_vp, _c_dp]),
     "m4g_unary_dot": (ctypes.c_int, [_vp, _c_dp]),
     "m4g_axpy": (ctypes.c_int, [ctypes.c_double, _vp, _vp]),
@@ -525,6 +526,11 @@ def _scalar(fn, where, *args):
 
 def dot(x, y):
     return _scalar(_f["m4g_dot"], "m4g_dot", x.h, y.h)
+
+
+def dot_noalloc(x, y):
+    """dot(x, y) through the entry point that allocates no device memory."""
+    return _scalar(_f["m4g_dot_noalloc"], "m4g_dot_noalloc", x.h, y.h)
 
 
 def two_norm(x):
