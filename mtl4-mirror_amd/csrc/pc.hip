@@ -561,8 +561,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
 template <typename FV, int W>
 __global__ __launch_bounds__(64) void k_build_recs(const int4* __restrict__ group_tab, const int2* __restrict__ meta,
                                                    const int32_t* __restrict__ src,
-                                                   const int64_t* __restrict__ val_pos,
-                                                   const int64_t* __restrict__ diag_pos, const double* __restrict__ f,
+                                                   const int32_t* __restrict__ val_pos,
+                                                   const int32_t* __restrict__ diag_pos, const double* __restrict__ f,
                                                    int vbytes, int4* __restrict__ recs)
 {
     constexpr int C = kRecChunks<FV, W>;
@@ -579,10 +579,10 @@ __global__ __launch_bounds__(64) void k_build_recs(const int4* __restrict__ grou
 #pragma unroll
     for (int k = 0; k < W; k++) {
         u.r.src[k] = uint32_t(kSweepHeader) + uint32_t(src[pc * W + k]) * uint32_t(vbytes);
-        const int64_t vp = val_pos[pc * W + k];
+        const int32_t vp = val_pos[pc * W + k];
         u.r.val[k] = vp >= 0 ? FV(f[vp]) : FV(0);
     }
-    const int64_t dp = diag_pos[pc];
+    const int32_t dp = diag_pos[pc];
     u.r.diag = dp >= 0 ? FV(f[dp]) : FV(1);
 #pragma unroll
     for (int c = 0; c < C; c++) recs[int64_t(first) * C + int64_t(c) * m + q] = u.c[c];
@@ -635,11 +635,7 @@ void SweepDev::upload(const SweepPlan& p, const double* fstore_dev, int ftype)
 {
     M4G_REQUIRE(p.npieces * int64_t(p.width) < (int64_t(1) << 31) && p.n < (int64_t(1) << 31), M4G_ERR_INVALID,
                 "sweep too large");
-    auto up32 = [](DevBuf<int32_t>& d, const std::vector<int32_t>& h) {
-        d.reset(std::max<size_t>(h.size(), 1));
-        d.upload(h.data(), h.size());
-    };
-    auto up64 = [](DevBuf<int64_t>& d, const std::vector<int64_t>& h) {
+    auto up32 = [](DevBuf<int32_t>& d, const auto& h) {
         d.reset(std::max<size_t>(h.size(), 1));
         d.upload(h.data(), h.size());
     };
@@ -652,8 +648,8 @@ void SweepDev::upload(const SweepPlan& p, const double* fstore_dev, int ftype)
     up32(tile_export_begin, p.tile_export_begin);
     up32(xlist, p.xlist);
     up32(pc_src, p.pc_src);
-    up64(pc_val_pos, p.pc_val_pos);
-    up64(pc_diag_pos, p.pc_diag_pos);
+    up32(pc_val_pos, p.pc_val_pos);
+    up32(pc_diag_pos, p.pc_diag_pos);
     std::vector<int4> tab(p.ngroups + 1);
     for (int64_t g = 0; g <= p.ngroups; g++) {
         tab[g].x = p.group_piece_begin[g];
@@ -916,14 +912,23 @@ static size_t sweep_lds_budget()
 }
 
 // host only (no HIP call): safe on any thread
-static SweepPlan plan_sweep_budget(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype, size_t budget)
+static SweepPlan plan_sweep_budget(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype, size_t budget,
+                                   const std::vector<int32_t>* levels = nullptr)
 {
     // sized for double vectors, the widest the sweeps use
+    std::vector<int32_t> own;
+    int64_t nlev = 0;
+    if (!levels) {
+        own = natural_levels(rows, &nlev);
+        levels = &own;
+    } else {
+        for (int32_t l : *levels) nlev = std::max<int64_t>(nlev, int64_t(l) + 1);
+    }
     SweepPlan p;
     for (;;) {
         TileShape shape = choose_tiles(rows, g, cap, env_double("M4G_SWEEP_HOP_US", kSweepHopUs),
-                                       env_double("M4G_SWEEP_STEP_US", kSweepStepUs));
-        p = build_sweep_plan(rows, g, shape, cap);
+                                       env_double("M4G_SWEEP_STEP_US", kSweepStepUs), levels);
+        p = build_sweep_plan(rows, g, shape, cap, 0, nlev);
         if (sweep_lds_bytes(p, 8, ftype) <= budget || cap <= 64) break;
         cap = cap * 3 / 4;
     }
@@ -958,12 +963,12 @@ void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g
 // Plans both directions and uploads them: the lower plan is uploaded (on this
 // thread, the library's stream) while the upper one is still being planned.
 void plan_upload_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
-                      SweepDev& dl, SweepDev& du, const double* fstore)
+                      SweepDev& dl, SweepDev& du, const double* fstore, const std::vector<int32_t>* lo_levels)
 {
     const size_t budget = sweep_lds_budget();
     auto fu = std::async(std::launch::async, [&] { return plan_sweep_budget(up, g, cap, ftype, budget); });
     try {
-        dl.upload(plan_sweep_budget(lo, g, cap, ftype, budget), fstore, ftype);
+        dl.upload(plan_sweep_budget(lo, g, cap, ftype, budget, lo_levels), fstore, ftype);
     } catch (...) {
         fu.wait();
         throw;
@@ -977,11 +982,12 @@ using namespace m4g;
 
 m4g_pc_s::~m4g_pc_s() = default;
 
-static void factorize_levels(m4g_pc_s& P, const SweepRows& lower_dag, const m4g_matrix_s& A)
+static void factorize_levels(m4g_pc_s& P, const SweepRows& lower_dag, const m4g_matrix_s& A,
+                             const std::vector<int32_t>& lvl)
 {
     // natural-order level sets of the lower DAG
     int64_t nlev = 0;
-    std::vector<int32_t> lvl = natural_levels(lower_dag, &nlev);
+    for (int32_t l : lvl) nlev = std::max<int64_t>(nlev, int64_t(l) + 1);
     std::vector<int64_t> lptr(nlev + 1, 0);
     for (int64_t r = 0; r < P.n; r++) lptr[lvl[r] + 1]++;
     for (int64_t l = 0; l < nlev; l++) lptr[l + 1] += lptr[l];
@@ -1065,6 +1071,7 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 ~Join() { if (f.valid()) f.wait(); }
             } join_g{fg};
             SweepRows lo, up;
+            std::vector<int32_t> lo_levels;   // natural levels of the lower DAG (factorisation and sweep)
             if (kind == M4G_PC_IC0) {
                 std::vector<int64_t> a_pos;
                 M4G_REQUIRE(upper_pattern(n, A->h_starts.data(), A->h_indices.data(), P->fS, P->fI, a_pos),
@@ -1078,7 +1085,8 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(a_pos.size())), dim3(256), 0, s,
                                        int64_t(a_pos.size()), P->a_pos.ptr, A->values.ptr, P->fstore.ptr, 0.0);
                 M4G_HIP(hipGetLastError());
-                factorize_levels(*P, lo, *A);
+                lo_levels = natural_levels(lo, nullptr);
+                factorize_levels(*P, lo, *A, lo_levels);
             } else {
                 P->fS = A->h_starts;
                 P->fI = A->h_indices;
@@ -1088,14 +1096,15 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 if (A->nnz)
                     M4G_HIP(hipMemcpyAsync(P->fstore.ptr, A->values.ptr, A->nnz * sizeof(double),
                                            hipMemcpyDeviceToDevice, s));
-                factorize_levels(*P, lo, *A);
+                lo_levels = natural_levels(lo, nullptr);
+                factorize_levels(*P, lo, *A, lo_levels);
             }
             mark("rows+factor_launch");
             const GridInfo g = fg.get();
             mark("grid_wait");
             P->grid = g;
             int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
-            plan_upload_pair(lo, up, g, cap, ftype, P->lower, P->upper, P->fstore.ptr);
+            plan_upload_pair(lo, up, g, cap, ftype, P->lower, P->upper, P->fstore.ptr, &lo_levels);
             mark("plan+upload");
             P->tmp_bytes = 0;
         }
@@ -1141,7 +1150,7 @@ int m4g_pc_refactor(m4g_pc P, m4g_matrix A)
                     M4G_HIP(hipMemcpyAsync(P->fstore.ptr, A->values.ptr, A->nnz * sizeof(double),
                                            hipMemcpyDeviceToDevice, s));
             }
-            factorize_levels(*P, lo, *A);
+            factorize_levels(*P, lo, *A, natural_levels(lo, nullptr));
             P->lower.refresh_values(P->fstore.ptr, P->factor_type);
             P->upper.refresh_values(P->fstore.ptr, P->factor_type);
             if (P->adj_ready) {

@@ -16,7 +16,7 @@ struct SweepDev {
     DevBuf<int4> group_tab;               // per group (+1): first piece, imports needed, exports before
     DevBuf<int2> pc_meta;                 // per piece: slot | len << 16, export slot
     DevBuf<int32_t> pc_src;               // per piece and entry: LDS index
-    DevBuf<int64_t> pc_val_pos, pc_diag_pos;
+    DevBuf<int32_t> pc_val_pos, pc_diag_pos; // per piece and entry / per piece: factor storage index, -1 none
     DevBuf<int4> recs;                    // packed records for factor_type and recs_vbytes
     const double* fsrc = nullptr;         // factor storage the records are packed from
     int recs_vbytes = 0;                  // vector entry size the records address (0: stale)
@@ -42,7 +42,8 @@ void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g
                      SweepPlan& pl, SweepPlan& pu);
 // both directions planned, the lower uploaded while the upper is planned
 void plan_upload_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
-                      SweepDev& dl, SweepDev& du, const double* fstore);
+                      SweepDev& dl, SweepDev& du, const double* fstore,
+                      const std::vector<int32_t>* lo_levels = nullptr);
 
 }  // namespace m4g
 

@@ -31,12 +31,47 @@
 
 namespace m4g {
 
+// Host threads a sweep plan may use: M4G_PLAN_THREADS, else half the
+// hardware threads (the two directions of a preconditioner are planned at
+// once), at most 16.
+int plan_threads()
+{
+    static const int nt = [] {
+        if (const char* e = std::getenv("M4G_PLAN_THREADS"); e && std::atoi(e) > 0)
+            return std::min(64, std::atoi(e));
+        const int hw = int(std::thread::hardware_concurrency());
+        return std::max(1, std::min(16, hw / 2));
+    }();
+    return nt;
+}
+
+// Runs fn(begin, end, thread) over [0, count) in contiguous chunks of at
+// least min_chunk items on up to nt host threads (the caller takes the
+// first chunk).  fn must not throw.
+template <typename F>
+static void parallel_chunks(int64_t count, int nt, F&& fn, int64_t min_chunk = 16384)
+{
+    const int64_t k = std::max<int64_t>(1, std::min<int64_t>(nt, count / std::max<int64_t>(1, min_chunk)));
+    if (k <= 1) {
+        if (count > 0) fn(int64_t(0), count, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(size_t(k - 1));
+    for (int64_t t = 1; t < k; t++)
+        th.emplace_back([&fn, count, k, t] { fn(count * t / k, count * (t + 1) / k, int(t)); });
+    fn(int64_t(0), count / k, 0);
+    for (auto& x : th) x.join();
+}
+
 // ---------------------------------------------------------------------------
 // grid inference
 // ---------------------------------------------------------------------------
-// Number of distinct neighbour offsets (di, dj, dk) of the pattern on the
-// grid, or -1 when some entry couples cells further apart than one step.
-static int64_t grid_score(int64_t n, const int64_t* S, const int64_t* I, int64_t nx, int64_t ny, int64_t stride)
+// Bit set of the neighbour offsets (di, dj, dk) that rows r_begin, r_begin +
+// stride, ... < r_end use on the grid, or -1 when some entry couples cells
+// further apart than one step.
+static int64_t grid_score_rows(int64_t n, const int64_t* S, const int64_t* I, int64_t nx, int64_t ny, int64_t stride,
+                               int64_t r_begin, int64_t r_end)
 {
     const int64_t nxy = nx * ny, nz = n / nxy;
     uint32_t seen = 0;   // bit (di+1) + 3(dj+1) + 9(dk+1)
@@ -45,7 +80,7 @@ static int64_t grid_score(int64_t n, const int64_t* S, const int64_t* I, int64_t
     int64_t off[27];
     for (int code = 0; code < 27; code++)
         off[code] = (code % 3 - 1) + nx * ((code / 3) % 3 - 1) + nxy * (code / 9 - 1);
-    for (int64_t r = 0; r < n; r += stride) {
+    for (int64_t r = r_begin; r < r_end; r += stride) {
         const int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
         int guess = 0;
         for (int64_t p = S[r]; p < S[r + 1]; p++) {
@@ -62,6 +97,25 @@ static int64_t grid_score(int64_t n, const int64_t* S, const int64_t* I, int64_t
             guess = found + 1;   // columns ascend: the next entry's offset is larger
         }
     }
+    return int64_t(seen);
+}
+
+// Number of distinct neighbour offsets (di, dj, dk) of the pattern on the
+// grid, or -1 when some entry couples cells further apart than one step;
+// every stride-th row is looked at (rows split over the planning threads).
+static int64_t grid_score(int64_t n, const int64_t* S, const int64_t* I, int64_t nx, int64_t ny, int64_t stride)
+{
+    const int64_t rows = (n + stride - 1) / stride;
+    const int nt = rows >= 65536 ? plan_threads() : 1;
+    std::vector<int64_t> part(size_t(nt), 0);
+    parallel_chunks(rows, nt, [&](int64_t a, int64_t b, int th) {
+        part[size_t(th)] = grid_score_rows(n, S, I, nx, ny, stride, a * stride, std::min(n, b * stride));
+    });
+    uint32_t seen = 0;
+    for (int64_t v : part) {
+        if (v < 0) return -1;
+        seen |= uint32_t(v);
+    }
     return int64_t(__builtin_popcount(seen));
 }
 
@@ -71,18 +125,23 @@ GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
     if (n < 4) return best;
     std::set<int64_t> offs;
     {
-        // distinct lower offsets r - c; a short list scanned linearly
-        std::vector<int64_t> seen;
-        for (int64_t r = 0; r < n; r++)
-            for (int64_t p = S[r]; p < S[r + 1]; p++)
-                if (I[p] < r) {
-                    const int64_t d = r - I[p];
-                    if (std::find(seen.begin(), seen.end(), d) == seen.end()) {
-                        seen.push_back(d);
-                        if (seen.size() > 64) return best;    // not a stencil pattern
+        // distinct lower offsets r - c, per chunk of rows a short list scanned
+        // linearly; more than 64 means no stencil pattern
+        const int nt = n >= 65536 ? plan_threads() : 1;
+        std::vector<std::vector<int64_t>> seen{size_t(nt)};
+        parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int th) {
+            std::vector<int64_t>& sv = seen[size_t(th)];
+            for (int64_t r = r0; r < r1 && sv.size() <= 64; r++)
+                for (int64_t p = S[r]; p < S[r + 1]; p++)
+                    if (I[p] < r) {
+                        const int64_t d = r - I[p];
+                        if (std::find(sv.begin(), sv.end(), d) == sv.end()) sv.push_back(d);
                     }
-                }
-        offs.insert(seen.begin(), seen.end());
+        });
+        for (const auto& sv : seen) {
+            offs.insert(sv.begin(), sv.end());
+            if (offs.size() > 64) return best;
+        }
     }
     if (offs.empty()) return best;
     std::set<int64_t> nxs;
@@ -152,17 +211,26 @@ static Coef skew_coefficients(const SweepRows& s, const GridInfo& g)
     if (!g.nx) return k;
     const int64_t nx = g.nx, ny = g.ny, nxy = g.nx * g.ny;
     int64_t a = 0, b = 0;
-    std::vector<std::array<int, 3>> deltas;   // distinct, in order of first appearance
-    for (int64_t r = 0; r < s.n; r++) {
-        int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
-        for (int64_t p = s.ptr[r]; p < s.ptr[r + 1]; p++) {
-            int64_t c = s.col[p];
-            int di = int(c % nx - ri), dj = int((c / nx) % ny - rj), dk = int(c / nxy - rk);
-            if (!s.lower) { di = -di; dj = -dj; dk = -dk; }
-            std::array<int, 3> d = {di, dj, dk};
-            if (std::find(deltas.begin(), deltas.end(), d) == deltas.end()) deltas.push_back(d);
+    // distinct (di, dj, dk) of the dependencies, collected per chunk of rows
+    const int nt = s.n >= 65536 ? plan_threads() : 1;
+    std::vector<std::vector<std::array<int, 3>>> part{size_t(nt)};
+    parallel_chunks(s.n, nt, [&](int64_t r0, int64_t r1, int th) {
+        std::vector<std::array<int, 3>>& dl = part[size_t(th)];
+        for (int64_t r = r0; r < r1; r++) {
+            const int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
+            for (int64_t p = s.ptr[r]; p < s.ptr[r + 1]; p++) {
+                const int64_t c = s.col[p];
+                int di = int(c % nx - ri), dj = int((c / nx) % ny - rj), dk = int(c / nxy - rk);
+                if (!s.lower) { di = -di; dj = -dj; dk = -dk; }
+                const std::array<int, 3> d = {di, dj, dk};
+                if (std::find(dl.begin(), dl.end(), d) == dl.end()) dl.push_back(d);
+            }
         }
-    }
+    });
+    std::vector<std::array<int, 3>> deltas;
+    for (const auto& dl : part)
+        for (const auto& d : dl)
+            if (std::find(deltas.begin(), deltas.end(), d) == deltas.end()) deltas.push_back(d);
     for (auto& d : deltas) {
         if (d[2] > 0) return k;
         if (d[2] == 0 && d[1] > 0) return k;
@@ -211,7 +279,8 @@ static double passes_per_level(int64_t x1, int64_t x2, int64_t x3, int64_t w1, i
     return levels ? double(groups) / double(levels) : 1.0;
 }
 
-TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, double hop_us, double step_us)
+TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, double hop_us, double step_us,
+                       const std::vector<int32_t>* levels)
 {
     TileShape best;
     Coef cf = skew_coefficients(s, g);
@@ -222,7 +291,11 @@ TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t cap, doubl
     }
     // level function of the natural DAG is linear on a grid: measure it
     int64_t nlev = 0;
-    std::vector<int32_t> lvl = natural_levels(s, &nlev);
+    std::vector<int32_t> own;
+    if (!levels) own = natural_levels(s, &nlev);
+    const std::vector<int32_t>& lvl = levels ? *levels : own;
+    if (levels)
+        for (int32_t l : lvl) nlev = std::max<int64_t>(nlev, int64_t(l) + 1);
     auto at = [&](int64_t i, int64_t j, int64_t k) -> double {
         int64_t r = i + g.nx * (j + g.ny * k);
         return r < s.n ? double(lvl[r]) : 0.0;
@@ -290,7 +363,8 @@ int sweep_piece_width(int64_t max_len)
     return 16;
 }
 
-SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t cap, int width)
+SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t cap, int width,
+                           int64_t levels)
 {
     SweepPlan p;
     const int64_t n = s.n;
@@ -307,244 +381,318 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     for (int64_t r = 0; r < n; r++)
         if (s.diag_pos[r] >= 0) { p.unit = false; break; }
     p.shape = shape;
-    natural_levels(s, &p.levels);
+    if (levels >= 0) p.levels = levels;
+    else natural_levels(s, &p.levels);
+    const int nt = plan_threads();
 
     // 1. tile of every row: linear box index, then tickets = the occupied
     //    boxes in topological order (ascending box index for the lower
     //    sweep, descending for the upper)
-    std::vector<int64_t> lin(n);
+    std::vector<int32_t> tile(n);
     int64_t nbox = 1;
+    std::vector<int64_t> lin(n);
     if (shape.grid) {
         const int64_t N1 = (g.nz - 1) / shape.e1 + 1;
         const int64_t N2 = ((g.ny - 1) + shape.a * (g.nz - 1)) / shape.e2 + 1;
         const int64_t N3 = ((g.nx - 1) + shape.b * (g.ny - 1) + shape.c * (g.nz - 1)) / shape.e3 + 1;
         nbox = N1 * N2 * N3;
         const int64_t nxy = g.nx * g.ny;
-        for (int64_t r = 0; r < n; r++) {
-            const int64_t k = r / nxy, j = (r - k * nxy) / g.nx, i = r - k * nxy - j * g.nx;
-            int64_t t1 = k, t2 = j + shape.a * k, t3 = i + shape.b * j + shape.c * k;
-            lin[r] = ((t1 / shape.e1) * N2 + t2 / shape.e2) * N3 + t3 / shape.e3;
-        }
+        parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+            for (int64_t r = r0; r < r1; r++) {
+                const int64_t k = r / nxy, j = (r - k * nxy) / g.nx, i = r - k * nxy - j * g.nx;
+                const int64_t t1 = k, t2 = j + shape.a * k, t3 = i + shape.b * j + shape.c * k;
+                lin[r] = ((t1 / shape.e1) * N2 + t2 / shape.e2) * N3 + t3 / shape.e3;
+            }
+        });
     } else {
-        int64_t rb = std::max<int64_t>(1, shape.rows_per_block);
+        const int64_t rb = std::max<int64_t>(1, shape.rows_per_block);
         nbox = n ? (n - 1) / rb + 1 : 1;
-        for (int64_t r = 0; r < n; r++) lin[r] = r / rb;
+        parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+            for (int64_t r = r0; r < r1; r++) lin[r] = r / rb;
+        });
     }
-    std::vector<int32_t> rank(size_t(nbox), -1);
-    for (int64_t r = 0; r < n; r++) rank[size_t(lin[r])] = 0;
-    p.ntiles = 0;
-    for (int64_t b = 0; b < nbox; b++) {
-        int32_t& rk = rank[size_t(s.lower ? b : nbox - 1 - b)];
-        if (rk == 0) rk = int32_t(p.ntiles++);
-    }
-    std::vector<int32_t> tile(n);
-    for (int64_t r = 0; r < n; r++) tile[r] = rank[size_t(lin[r])];
-    std::vector<int64_t>().swap(lin);
-
-    // 2. rows of each tile in natural order -> LDS slots
-    p.tile_row_begin.assign(p.ntiles + 1, 0);
-    for (int64_t r = 0; r < n; r++) p.tile_row_begin[tile[r] + 1]++;
-    for (int64_t t = 0; t < p.ntiles; t++) p.tile_row_begin[t + 1] += p.tile_row_begin[t];
-    p.rows.assign(n, 0);
-    std::vector<int32_t> slot(n);
     {
-        std::vector<int32_t> fill(p.tile_row_begin.begin(), p.tile_row_begin.end() - 1);
-        for (int64_t r = 0; r < n; r++) {
-            int32_t t = tile[r];
-            slot[r] = fill[t] - p.tile_row_begin[t];
-            p.rows[fill[t]++] = int32_t(r);
+        std::vector<int32_t> rank(size_t(nbox), -1);
+        parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+            for (int64_t r = r0; r < r1; r++) __atomic_store_n(&rank[size_t(lin[r])], 0, __ATOMIC_RELAXED);
+        });
+        p.ntiles = 0;
+        for (int64_t b = 0; b < nbox; b++) {
+            int32_t& rk = rank[size_t(s.lower ? b : nbox - 1 - b)];
+            if (rk == 0) rk = int32_t(p.ntiles++);
         }
+        parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+            for (int64_t r = r0; r < r1; r++) tile[r] = rank[size_t(lin[r])];
+        });
     }
-    for (int64_t t = 0; t < p.ntiles; t++)
+    std::vector<int64_t>().swap(lin);
+    const int64_t T = p.ntiles;
+
+    // 2. rows of each tile in natural order: per-chunk counts per tile, so
+    //    every chunk writes its rows at fixed positions
+    p.tile_row_begin.assign(T + 1, 0);
+    p.rows.assign(n, 0);
+    {
+        const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(nt, n / 4096));
+        std::vector<std::vector<int32_t>> cnt(size_t(nc), std::vector<int32_t>(size_t(T), 0));
+        auto chunk = [&](int64_t c, int64_t& r0, int64_t& r1) { r0 = n * c / nc; r1 = n * (c + 1) / nc; };
+        parallel_chunks(nc, nt, [&](int64_t c0, int64_t c1, int) {
+            for (int64_t c = c0; c < c1; c++) {
+                int64_t r0, r1;
+                chunk(c, r0, r1);
+                for (int64_t r = r0; r < r1; r++) cnt[c][tile[r]]++;
+            }
+        }, 1);
+        for (int64_t t = 0; t < T; t++) {
+            int32_t tot = 0;
+            for (int64_t c = 0; c < nc; c++) tot += cnt[c][t];
+            p.tile_row_begin[t + 1] = p.tile_row_begin[t] + tot;
+        }
+        // cnt becomes each chunk's first position per tile
+        for (int64_t t = 0; t < T; t++) {
+            int32_t pos = p.tile_row_begin[t];
+            for (int64_t c = 0; c < nc; c++) {
+                const int32_t k = cnt[c][t];
+                cnt[c][t] = pos;
+                pos += k;
+            }
+        }
+        parallel_chunks(nc, nt, [&](int64_t c0, int64_t c1, int) {
+            for (int64_t c = c0; c < c1; c++) {
+                int64_t r0, r1;
+                chunk(c, r0, r1);
+                for (int64_t r = r0; r < r1; r++) p.rows[cnt[c][tile[r]]++] = int32_t(r);
+            }
+        }, 1);
+    }
+    for (int64_t t = 0; t < T; t++)
         p.max_tile_rows = std::max<int64_t>(p.max_tile_rows, p.tile_row_begin[t + 1] - p.tile_row_begin[t]);
     if (p.max_tile_rows > cap) throw std::runtime_error("sweep tile exceeds LDS capacity");
 
-    // 3. pieces and their local levels; rows read by other tiles are exported
+    // 3. per tile: pieces in processing order (ascending rows for the lower
+    //    sweep, descending for the upper) with their local levels, then
+    //    ordered stably by level and cut into groups of at most 64 pieces of
+    //    one level.  Rows read by other tiles are marked exported.
     const int W = p.width;
     struct Piece {
         int32_t row, k0, len, level;
     };
-    std::vector<Piece> pieces;
-    pieces.reserve(size_t(n) + size_t(n) / 4);
+    std::vector<std::vector<Piece>> tpieces{size_t(T)};
+    std::vector<std::vector<int32_t>> tgroups{size_t(T)};   // tile-local first piece of each group
     std::vector<int32_t> llev(n, 0);   // local level of the row's last piece
     std::vector<char> exported(n, 0);
-    for (int64_t q = 0; q < n; q++) {
-        int64_t r = s.lower ? q : n - 1 - q;
-        const int64_t len = s.ptr[r + 1] - s.ptr[r];
-        int32_t prev = -1;
-        for (int64_t k0 = 0; k0 == 0 || k0 < len; k0 += W) {
-            const int64_t k1 = std::min<int64_t>(len, k0 + W);
-            int32_t l = prev + 1;
-            for (int64_t e = s.ptr[r] + k0; e < s.ptr[r] + k1; e++) {
-                int64_t c = s.col[e];
-                if (tile[c] == tile[r]) l = std::max(l, llev[c] + 1);
-                else exported[c] = 1;
+    parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
+        std::vector<Piece> raw;
+        std::vector<int32_t> lc;
+        for (int64_t t = ta; t < tb; t++) {
+            const int32_t q0 = p.tile_row_begin[t], q1 = p.tile_row_begin[t + 1];
+            raw.clear();
+            int32_t maxlev = 0;
+            for (int32_t qq = 0; qq < q1 - q0; qq++) {
+                const int32_t r = p.rows[s.lower ? q0 + qq : q1 - 1 - qq];
+                const int64_t len = s.ptr[r + 1] - s.ptr[r];
+                int32_t prev = -1;
+                for (int64_t k0 = 0; k0 == 0 || k0 < len; k0 += W) {
+                    const int64_t k1 = std::min<int64_t>(len, k0 + W);
+                    int32_t l = prev + 1;
+                    for (int64_t e = s.ptr[r] + k0; e < s.ptr[r] + k1; e++) {
+                        const int64_t c = s.col[e];
+                        if (tile[c] == t) l = std::max(l, llev[c] + 1);
+                        else __atomic_store_n(&exported[c], char(1), __ATOMIC_RELAXED);
+                    }
+                    raw.push_back({r, int32_t(k0), int32_t(k1 - k0), l});
+                    prev = l;
+                }
+                llev[r] = prev;
+                maxlev = std::max(maxlev, prev);
             }
-            pieces.push_back({int32_t(r), int32_t(k0), int32_t(k1 - k0), l});
-            prev = l;
+            // stable counting sort by level
+            lc.assign(size_t(maxlev) + 2, 0);
+            for (const Piece& pc : raw) lc[size_t(pc.level) + 1]++;
+            for (size_t l = 1; l < lc.size(); l++) lc[l] += lc[l - 1];
+            std::vector<Piece>& out = tpieces[size_t(t)];
+            out.resize(raw.size());
+            for (const Piece& pc : raw) out[size_t(lc[size_t(pc.level)]++)] = pc;
+            std::vector<int32_t>& grp = tgroups[size_t(t)];
+            grp.clear();
+            for (int32_t q = 0; q < int32_t(out.size());) {
+                int32_t q1e = q;
+                while (q1e < int32_t(out.size()) && q1e - q < 64 && out[q1e].level == out[q].level) q1e++;
+                grp.push_back(q);
+                q = q1e;
+            }
         }
-        llev[r] = prev;
+    }, 1);
+    // global numbering of pieces and groups: tiles in ticket order
+    std::vector<int64_t> tpiece0(size_t(T) + 1, 0), tgroup0(size_t(T) + 1, 0);
+    for (int64_t t = 0; t < T; t++) {
+        tpiece0[t + 1] = tpiece0[t] + int64_t(tpieces[t].size());
+        tgroup0[t + 1] = tgroup0[t] + int64_t(tgroups[t].size());
     }
-    p.npieces = int64_t(pieces.size());
+    p.npieces = tpiece0[T];
+    p.ngroups = tgroup0[T];
     if (p.npieces >= (int64_t(1) << 31) / std::max(W, 1)) throw std::runtime_error("sweep too large");
-    // 4. groups: pieces ordered by (tile, local level, processing order),
-    //    each run of one (tile, level) cut into groups of at most 64
-    // (pieces were made in processing order, so a stable counting sort on
-    // the key (tile, level) gives the (tile, level, processing) order; the
-    // records are moved once, so later passes read them sequentially)
-    {
-        int32_t maxlev = 0;
-        for (const Piece& pc : pieces) maxlev = std::max(maxlev, pc.level);
-        const int64_t nl = int64_t(maxlev) + 1;
-        std::vector<Piece> sorted(size_t(p.npieces));
-        if (p.ntiles * nl <= 4 * p.npieces + 4096) {
-            std::vector<int64_t> cnt(size_t(p.ntiles * nl) + 1, 0);
-            for (const Piece& pc : pieces) cnt[size_t(int64_t(tile[pc.row]) * nl + pc.level) + 1]++;
-            for (size_t k = 1; k < cnt.size(); k++) cnt[k] += cnt[k - 1];
-            for (const Piece& pc : pieces) sorted[size_t(cnt[size_t(int64_t(tile[pc.row]) * nl + pc.level)]++)] = pc;
-        } else {
-            // many (tile, level) pairs: by level, then stably by tile
-            std::vector<int64_t> cnt(size_t(std::max<int64_t>(nl, p.ntiles)) + 1, 0);
-            std::vector<Piece> by_level(size_t(p.npieces));
-            for (const Piece& pc : pieces) cnt[size_t(pc.level) + 1]++;
-            for (int64_t l = 0; l < nl; l++) cnt[size_t(l) + 1] += cnt[size_t(l)];
-            for (const Piece& pc : pieces) by_level[size_t(cnt[size_t(pc.level)]++)] = pc;
-            std::fill(cnt.begin(), cnt.end(), 0);
-            for (const Piece& pc : by_level) cnt[size_t(tile[pc.row]) + 1]++;
-            for (int64_t t = 0; t < p.ntiles; t++) cnt[size_t(t) + 1] += cnt[size_t(t)];
-            for (const Piece& pc : by_level) sorted[size_t(cnt[size_t(tile[pc.row])]++)] = pc;
-        }
-        pieces.swap(sorted);
+
+    p.group_piece_begin.assign(size_t(p.ngroups) + 1, int32_t(p.npieces));
+    p.tile_group_begin.assign(T + 1, int32_t(p.ngroups));
+    for (int64_t t = 0; t < T; t++) {
+        p.tile_group_begin[t] = int32_t(tgroup0[t]);
+        p.max_steps = std::max<int64_t>(p.max_steps, tgroup0[t + 1] - tgroup0[t]);
     }
-    std::vector<int32_t> ptile(size_t(p.npieces));
-    for (int64_t q = 0; q < p.npieces; q++) ptile[size_t(q)] = tile[pieces[size_t(q)].row];
-    p.group_piece_begin.clear();
-    for (int64_t q = 0; q < p.npieces;) {
-        const Piece& h = pieces[q];
-        int64_t q1 = q;
-        while (q1 < p.npieces && q1 - q < 64 && ptile[q1] == ptile[q] && pieces[q1].level == h.level) q1++;
-        p.group_piece_begin.push_back(int32_t(q));
-        q = q1;
-    }
-    p.ngroups = int64_t(p.group_piece_begin.size());
-    p.group_piece_begin.push_back(int32_t(p.npieces));
-    p.tile_group_begin.assign(p.ntiles + 1, int32_t(p.ngroups));
-    for (int64_t gi = p.ngroups - 1; gi >= 0; gi--)
-        p.tile_group_begin[ptile[p.group_piece_begin[gi]]] = int32_t(gi);
-    for (int64_t t = p.ntiles - 1; t >= 0; t--)   // tiles without rows: empty group range
-        if (p.tile_row_begin[t + 1] == p.tile_row_begin[t]) p.tile_group_begin[t] = p.tile_group_begin[t + 1];
-    for (int64_t t = 0; t < p.ntiles; t++)
-        p.max_steps = std::max<int64_t>(p.max_steps, p.tile_group_begin[t + 1] - p.tile_group_begin[t]);
-    // LDS slots in (group, lane) order: the rows of a group, and on a grid
-    // the rows they read from the group before, sit in consecutive slots, so
-    // a wavefront's LDS accesses fall in distinct banks
-    {
-        std::vector<int32_t> next(p.ntiles, 0);
-        std::vector<char> placed(n, 0);
-        for (int64_t q = 0; q < p.npieces; q++) {
-            const int32_t r = pieces[q].row;
-            if (!placed[r]) {
-                placed[r] = 1;
-                slot[r] = next[tile[r]]++;
+
+    // 4. per tile: LDS slots in (group, lane) order (the rows of a group, and
+    //    on a grid the rows they read from the group before, sit in
+    //    consecutive slots, so a wavefront's LDS accesses fall in distinct
+    //    banks); the tile's export count, in the order its rows finish
+    std::vector<int32_t> slot(n, -1);
+    std::vector<int32_t> texports(size_t(T), 0);
+    parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
+        for (int64_t t = ta; t < tb; t++) {
+            int32_t next = 0, nx = 0;
+            for (const Piece& pc : tpieces[t]) {
+                if (slot[pc.row] < 0) slot[pc.row] = next++;
+                if (exported[pc.row] && pc.k0 + pc.len == s.ptr[pc.row + 1] - s.ptr[pc.row]) nx++;
             }
+            texports[t] = nx;
         }
-        p.row_slot.resize(n);
-        for (int64_t q = 0; q < n; q++) p.row_slot[q] = slot[p.rows[q]];
-    }
+    }, 1);
+    p.row_slot.resize(n);
+    parallel_chunks(n, nt, [&](int64_t q0, int64_t q1, int) {
+        for (int64_t q = q0; q < q1; q++) p.row_slot[q] = slot[p.rows[q]];
+    });
     // export slots: contiguous per tile, in the order the rows finish (group
     // order of their last pieces); each tile's list of exported LDS slots, and
     // per group the tile's exports finished before it
+    p.tile_export_begin.assign(T + 1, 0);
+    for (int64_t t = 0; t < T; t++) {
+        p.tile_export_begin[t + 1] = p.tile_export_begin[t] + texports[t];
+        p.max_tile_exports = std::max<int64_t>(p.max_tile_exports, texports[t]);
+    }
+    p.nexport = p.tile_export_begin[T];
+    p.xlist.assign(size_t(p.nexport), 0);
+    p.group_xbefore.assign(size_t(p.ngroups), 0);
     std::vector<int32_t> xslot(n, -1);
-    p.tile_export_begin.assign(p.ntiles + 1, 0);
-    p.xlist.clear();
-    p.group_xbefore.assign(p.ngroups, 0);
-    for (int64_t t = 0; t < p.ntiles; t++) {
-        const int32_t xb = int32_t(p.xlist.size());
-        p.tile_export_begin[t] = xb;
-        for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
-            p.group_xbefore[gi] = int32_t(p.xlist.size()) - xb;
-            for (int32_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++) {
-                const Piece& pc = pieces[q];
-                const int32_t r = pc.row;
-                if (exported[r] && pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r]) {
-                    xslot[r] = int32_t(p.xlist.size());
-                    p.xlist.push_back(slot[r]);
+    parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
+        for (int64_t t = ta; t < tb; t++) {
+            const int32_t xb = p.tile_export_begin[t];
+            int32_t x = xb;
+            const std::vector<Piece>& tp = tpieces[t];
+            const std::vector<int32_t>& grp = tgroups[t];
+            for (size_t gl = 0; gl < grp.size(); gl++) {
+                const int64_t gi = tgroup0[t] + int64_t(gl);
+                p.group_piece_begin[gi] = int32_t(tpiece0[t] + grp[gl]);
+                p.group_xbefore[gi] = x - xb;
+                const size_t qe = gl + 1 < grp.size() ? size_t(grp[gl + 1]) : tp.size();
+                for (size_t q = size_t(grp[gl]); q < qe; q++) {
+                    const Piece& pc = tp[q];
+                    if (exported[pc.row] && pc.k0 + pc.len == s.ptr[pc.row + 1] - s.ptr[pc.row]) {
+                        xslot[pc.row] = x;
+                        p.xlist[x++] = slot[pc.row];
+                    }
                 }
             }
         }
-        p.max_tile_exports = std::max<int64_t>(p.max_tile_exports, int64_t(p.xlist.size()) - xb);
-    }
-    p.tile_export_begin[p.ntiles] = int32_t(p.xlist.size());
-    p.nexport = int64_t(p.xlist.size());
+    }, 1);
+    // tiles without rows: empty group range
+    for (int64_t t = T - 1; t >= 0; t--)
+        if (p.tile_row_begin[t + 1] == p.tile_row_begin[t]) p.tile_group_begin[t] = p.tile_group_begin[t + 1];
 
     // 5. piece records in group order; sources still as slot / -(export + 1)
     p.pc_slot.resize(p.npieces);
     p.pc_len.resize(p.npieces);
     p.pc_export.resize(p.npieces);
     p.pc_diag_pos.resize(p.npieces);
-    p.pc_src.assign(size_t(p.npieces) * W, 0);
-    p.pc_val_pos.assign(size_t(p.npieces) * W, -1);
-    for (int64_t q = 0; q < p.npieces; q++) {
-        if (q + 16 < p.npieces) {   // rows of a group are far apart: fetch ahead
-            const int32_t rf = pieces[q + 16].row;
-            __builtin_prefetch(&s.ptr[rf]);
-            __builtin_prefetch(&s.col[s.ptr[rf] + pieces[q + 16].k0]);
-            __builtin_prefetch(&s.val_pos[s.ptr[rf] + pieces[q + 16].k0]);
-            __builtin_prefetch(&slot[rf]);
-        }
-        const Piece& pc = pieces[q];
-        const int32_t r = pc.row;
-        const bool last = pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r];
-        p.pc_slot[q] = slot[r];
-        p.pc_len[q] = pc.len;
-        p.pc_export[q] = last ? xslot[r] : -1;
-        p.pc_diag_pos[q] = last ? s.diag_pos[r] : -1;
-        for (int32_t k = 0; k < W; k++) {
-            if (k >= pc.len) {
-                p.pc_src[q * W + k] = -1;   // padding: the zero slot, set below
-                continue;
+    p.pc_src.resize(size_t(p.npieces) * W);
+    p.pc_val_pos.resize(size_t(p.npieces) * W);
+    int too_far = 0;   // a factor position needs more than 32 bits
+    parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
+        int64_t far = 0;
+        for (int64_t t = ta; t < tb; t++) {
+            const std::vector<Piece>& tp = tpieces[t];
+            const int64_t base = tpiece0[t];
+            for (size_t ql = 0; ql < tp.size(); ql++) {
+                if (ql + 16 < tp.size()) {   // rows of a group are far apart: fetch ahead
+                    const Piece& f = tp[ql + 16];
+                    __builtin_prefetch(&s.ptr[f.row]);
+                    __builtin_prefetch(&s.col[s.ptr[f.row] + f.k0]);
+                    __builtin_prefetch(&s.val_pos[s.ptr[f.row] + f.k0]);
+                }
+                const int64_t q = base + int64_t(ql);
+                const Piece& pc = tp[ql];
+                const int32_t r = pc.row;
+                const bool last = pc.k0 + pc.len == s.ptr[r + 1] - s.ptr[r];
+                p.pc_slot[q] = slot[r];
+                p.pc_len[q] = pc.len;
+                p.pc_export[q] = last ? xslot[r] : -1;
+                p.pc_diag_pos[q] = last ? int32_t(s.diag_pos[r]) : -1;
+                for (int32_t k = 0; k < W; k++) {
+                    if (k >= pc.len) {
+                        p.pc_src[q * W + k] = -1;   // padding: the zero slot, set below
+                        p.pc_val_pos[q * W + k] = -1;
+                        continue;
+                    }
+                    const int64_t e = s.ptr[r] + pc.k0 + k, c = s.col[e];
+                    p.pc_src[q * W + k] = tile[c] == t ? slot[c] : -(xslot[c] + 1);
+                    p.pc_val_pos[q * W + k] = int32_t(s.val_pos[e]);
+                    far |= s.val_pos[e];
+                }
+                if (last && s.diag_pos[r] > 0) far |= s.diag_pos[r];   // -1: unit diagonal
             }
-            const int64_t e = s.ptr[r] + pc.k0 + k, c = s.col[e];
-            p.pc_src[q * W + k] = tile[c] == tile[r] ? slot[c] : -(xslot[c] + 1);
-            p.pc_val_pos[q * W + k] = s.val_pos[e];
         }
-    }
+        if (far >> 31) __atomic_store_n(&too_far, 1, __ATOMIC_RELAXED);
+    }, 1);
+    if (too_far) throw std::runtime_error("factor storage too large for a sweep plan (2^31 entries)");
+    std::vector<std::vector<Piece>>().swap(tpieces);
+
     // 6. imports: the values each tile reads from other tiles, numbered per
     //    tile in order of first use; they follow the rows in LDS
-    p.tile_import_begin.assign(p.ntiles + 1, 0);
-    p.group_need.assign(p.ngroups, 0);
-    p.imports.clear();
+    std::vector<std::vector<int32_t>> timports{size_t(T)};
+    p.group_need.assign(size_t(p.ngroups), 0);
     {
-        std::vector<int32_t> local(p.nexport, -1);   // exchange slot -> import of the current tile
-        for (int64_t t = 0; t < p.ntiles; t++) {
-            const int32_t ib = int32_t(p.imports.size());
-            p.tile_import_begin[t] = ib;
-            for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
-                for (int64_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++)
-                    for (int32_t k = 0; k < p.pc_len[q]; k++) {
-                        int32_t& src = p.pc_src[q * W + k];
-                        if (src >= 0) continue;
-                        const int32_t xs = -(src + 1);
-                        if (local[xs] < 0) {
-                            local[xs] = int32_t(p.imports.size()) - ib;
-                            p.imports.push_back(xs);
+        const int32_t mtr = int32_t(p.max_tile_rows);
+        std::vector<std::vector<int32_t>> local{size_t(nt)};
+        parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int th) {
+            std::vector<int32_t>& loc = local[size_t(th)];   // exchange slot -> import of the current tile
+            loc.assign(size_t(p.nexport), -1);
+            for (int64_t t = ta; t < tb; t++) {
+                std::vector<int32_t>& imp = timports[t];
+                for (int32_t gi = p.tile_group_begin[t]; gi < p.tile_group_begin[t + 1]; gi++) {
+                    for (int64_t q = p.group_piece_begin[gi]; q < p.group_piece_begin[gi + 1]; q++)
+                        for (int32_t k = 0; k < p.pc_len[q]; k++) {
+                            int32_t& src = p.pc_src[q * W + k];
+                            if (src >= 0) continue;
+                            const int32_t xs = -(src + 1);
+                            if (loc[xs] < 0) {
+                                loc[xs] = int32_t(imp.size());
+                                imp.push_back(xs);
+                            }
+                            src = mtr + loc[xs];
                         }
-                        src = int32_t(p.max_tile_rows) + local[xs];
-                    }
-                p.group_need[gi] = int32_t(p.imports.size()) - ib;
+                    p.group_need[gi] = int32_t(imp.size());
+                }
+                for (int32_t xs : imp) loc[xs] = -1;
             }
-            for (int64_t i = ib; i < int64_t(p.imports.size()); i++) local[p.imports[i]] = -1;
-            p.max_tile_imports = std::max<int64_t>(p.max_tile_imports, int64_t(p.imports.size()) - ib);
-        }
-        p.tile_import_begin[p.ntiles] = int32_t(p.imports.size());
-        p.nimports = int64_t(p.imports.size());
+        }, 1);
     }
+    p.tile_import_begin.assign(T + 1, 0);
+    for (int64_t t = 0; t < T; t++) {
+        p.tile_import_begin[t + 1] = p.tile_import_begin[t] + int32_t(timports[t].size());
+        p.max_tile_imports = std::max<int64_t>(p.max_tile_imports, int64_t(timports[t].size()));
+    }
+    p.nimports = p.tile_import_begin[T];
+    p.imports.resize(size_t(p.nimports));
+    parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
+        for (int64_t t = ta; t < tb; t++)
+            std::copy(timports[t].begin(), timports[t].end(), p.imports.begin() + p.tile_import_begin[t]);
+    }, 1);
     // padding entries read the zero slot after the imports with value 0:
     // rr - 0 * (+0) == rr exactly for every rr, so padding changes nothing
     if (p.max_tile_rows + p.max_tile_imports + 1 > 65535) throw std::runtime_error("sweep tile exceeds LDS index range");
     const int32_t zero_slot = int32_t(p.max_tile_rows + p.max_tile_imports);
-    for (int64_t q = 0; q < p.npieces; q++)
-        for (int32_t k = p.pc_len[q]; k < W; k++) p.pc_src[q * W + k] = zero_slot;
+    parallel_chunks(p.npieces, nt, [&](int64_t q0, int64_t q1, int) {
+        for (int64_t q = q0; q < q1; q++)
+            for (int32_t k = p.pc_len[q]; k < W; k++) p.pc_src[q * W + k] = zero_slot;
+    });
     return p;
 }
 
@@ -677,23 +825,36 @@ namespace m4g {
 bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<int64_t>& US, std::vector<int64_t>& UI,
                    std::vector<int64_t>& a_pos)
 {
+    // two passes over the rows (count, then fill), each split over the
+    // planning threads; a U row must start with its diagonal
+    const int nt = n >= 65536 ? plan_threads() : 1;
     US.assign(n + 1, 0);
-    UI.clear();
-    a_pos.clear();
-    UI.reserve(size_t((S[n] + n) / 2 + 1));
-    a_pos.reserve(UI.capacity());
-    for (int64_t r = 0; r < n; r++) {
-        bool first = true;
-        for (int64_t p = S[r]; p < S[r + 1]; p++) {
-            if (I[p] < r) continue;
-            if (first && I[p] != r) return false;   // U row must start with its diagonal
-            first = false;
-            UI.push_back(I[p]);
-            a_pos.push_back(p);
+    int bad = 0;
+    parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+        for (int64_t r = r0; r < r1; r++) {
+            const int64_t* b = I + S[r];
+            const int64_t* e = I + S[r + 1];
+            const int64_t* f = std::lower_bound(b, e, r);   // columns ascend
+            if (f == e || *f != r) {
+                __atomic_store_n(&bad, 1, __ATOMIC_RELAXED);
+                return;
+            }
+            US[r + 1] = e - f;
         }
-        if (first) return false;
-        US[r + 1] = int64_t(UI.size());
-    }
+    });
+    if (bad) return false;
+    for (int64_t r = 0; r < n; r++) US[r + 1] += US[r];
+    UI.resize(size_t(US[n]));
+    a_pos.resize(size_t(US[n]));
+    parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+        for (int64_t r = r0; r < r1; r++) {
+            const int64_t first = S[r + 1] - (US[r + 1] - US[r]);
+            for (int64_t k = 0; k < US[r + 1] - US[r]; k++) {
+                UI[US[r] + k] = I[first + k];
+                a_pos[US[r] + k] = first + k;
+            }
+        }
+    });
     return true;
 }
 
@@ -704,26 +865,30 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
     // second host thread; the lower rows below do not depend on it)
     bool upper_ok = true;
     std::thread upper_thread([&] {
-        upper.n = n;
-        upper.lower = false;
-        upper.ptr.assign(n + 1, 0);
-        upper.col.clear();
-        upper.val_pos.clear();
-        upper.col.reserve(UI.size());
-        upper.val_pos.reserve(UI.size());
-        upper.diag_pos.assign(n, -1);
-        for (int64_t r = 0; r < n; r++) {
+        for (int64_t r = 0; r < n; r++)
             if (US[r] == US[r + 1] || UI[US[r]] != r) {
                 upper_ok = false;
                 return;
             }
-            upper.diag_pos[r] = US[r];
-            for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
-                upper.col.push_back(UI[p]);
-                upper.val_pos.push_back(p);
+        // row r keeps U's row without its diagonal: entries US[r] + 1 ..
+        upper.n = n;
+        upper.lower = false;
+        upper.ptr.resize(n + 1);
+        upper.diag_pos.resize(n);
+        upper.col.resize(UI.size() - size_t(n));
+        upper.val_pos.resize(UI.size() - size_t(n));
+        const int nt = n >= 65536 ? std::max(1, plan_threads() - 1) : 1;
+        parallel_chunks(n + 1, nt, [&](int64_t r0, int64_t r1, int) {
+            for (int64_t r = r0; r < r1; r++) {
+                upper.ptr[r] = US[r] - r;
+                if (r == n) continue;
+                upper.diag_pos[r] = US[r];
+                for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
+                    upper.col[p - r - 1] = UI[p];
+                    upper.val_pos[p - r - 1] = p;
+                }
             }
-            upper.ptr[r + 1] = int64_t(upper.col.size());
-        }
+        });
     });
     struct Join {
         std::thread& t;

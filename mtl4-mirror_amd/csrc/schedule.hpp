@@ -2,10 +2,37 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace m4g {
+
+// Allocator whose value-initialisation leaves trivial elements uninitialised:
+// the plan's large arrays are written in parallel, and each page is first
+// touched by the thread that fills it.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <typename U>
+    void construct(U* p) noexcept
+    {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <typename U, typename... A>
+    void construct(U* p, A&&... a)
+    {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+using PlanVec32 = std::vector<int32_t, NoInitAlloc<int32_t>>;
 
 // Natural (lexicographic) grid inferred from a matrix pattern: row
 // r = i + nx*(j + ny*k) and every stored entry couples cells at Chebyshev
@@ -64,10 +91,11 @@ struct SweepPlan {
     std::vector<int32_t> pc_slot;         // [npieces] LDS slot of the piece's row in its tile
     std::vector<int32_t> pc_len;          // [npieces] entries of the piece (<= width)
     std::vector<int32_t> pc_export;       // [npieces] export slot (last piece of an exported row) or -1
-    std::vector<int64_t> pc_diag_pos;     // [npieces] factor index of the diagonal; -1: none
-    std::vector<int32_t> pc_src;          // [npieces*width] LDS index of each entry's value:
+    PlanVec32 pc_diag_pos;                // [npieces] factor index of the diagonal; -1: none
+    PlanVec32 pc_src;                     // [npieces*width] LDS index of each entry's value:
                                           //   slot (< max_tile_rows) or max_tile_rows + tile-local import
-    std::vector<int64_t> pc_val_pos;      // [npieces*width] factor index; -1: padding
+    PlanVec32 pc_val_pos;                 // [npieces*width] factor index; -1: padding
+                                          // (factor storage is limited to 2^31 - 1 entries)
     std::vector<int32_t> tile_import_begin;  // [ntiles+1] into imports
     std::vector<int32_t> imports;         // exchange slot of each import, per tile in first-use order
     int64_t max_tile_imports = 0, nimports = 0;
@@ -85,11 +113,17 @@ constexpr double kSweepHopUs = 3.0;
 constexpr double kSweepStepUs = 0.2;
 
 // Chooses a tile shape for the sweep (cost model: pipeline hops * hop_us +
-// in-tile levels * step_us) subject to max_tile_rows.
-TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t max_tile_rows, double hop_us, double step_us);
+// in-tile levels * step_us) subject to max_tile_rows.  levels: natural_levels
+// of s when the caller has them (else computed here).
+TileShape choose_tiles(const SweepRows& s, const GridInfo& g, int64_t max_tile_rows, double hop_us, double step_us,
+                       const std::vector<int32_t>* levels = nullptr);
 
+// levels: the natural level count of s when known (else computed here).
+// Construction runs on plan_threads() host threads; the plan does not depend
+// on their number.
 SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShape& shape, int64_t max_tile_rows,
-                           int width = 0);
+                           int width = 0, int64_t levels = -1);
+int plan_threads();
 
 // Checks every invariant the sweep kernel relies on (bounds, ordering of
 // dependencies before their use, tile order acyclic).  Returns "" when valid.
