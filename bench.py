@@ -93,34 +93,55 @@ def measured_traffic(kernel):
     return None
 
 
-def batched(g, nsys=512, nrhs=8, sample=16):
+def batched(g, nsys=512, nrhs=8, per_class=6):
     """512 SPD 2D 5-point variable-coefficient systems (side ~ U{32..256},
     kappa ~ lognormal(0,1) per cell), 8 right-hand sides ~ N(0,1) each, IC(0)-PCG
-    to 1e-8: one factorisation launch + one persistent solve launch (autotuned
-    grouping and block size) vs a loop of single solves (factorisation + 8 CG
-    solves per system) timed on `sample` systems spread over the sizes and
-    scaled to all systems."""
+    to 1e-8: one persistent solve launch (autotuned grouping and block size)
+    vs a loop of single GPU solves (8 CG solves per system).
+
+    Like for like: on both sides the set-up is timed apart and left out of the
+    compared times (batched: Batch construction, the factorisation launch and
+    the autotuner; loop: Matrix and PC construction per system).  The loop is
+    timed on `per_class` systems spread over each third of the batch by size
+    (the autotuner's size classes) and scaled class by class."""
     sides = g.rng_int(7, 1, nsys, 32, 256)
     systems = [g.varcoef5(int(sd), int(sd), seed=1000 + k) for k, sd in enumerate(sides)]
     sizes = np.array([len(S) - 1 for S, _, _ in systems])
     n_total, nnz_total = int(sizes.sum()), int(sum(int(S[-1]) for S, _, _ in systems))
     bs = [[g.rng_normal(2000 + k, j, int(sizes[k])) for j in range(nrhs)] for k in range(nsys)]
     bh = np.concatenate([np.stack(v, axis=1).reshape(-1) for v in bs])
+    g.synchronize()
+    t0 = time.perf_counter()
     B = g.Batch(systems)
+    create_ms = (time.perf_counter() - t0) * 1e3
     b = g.Vector(data=bh)
     x = g.Vector(len(bh))
     g.synchronize()
     t0 = time.perf_counter()
     B.factor_ic0()
     factor_ms = (time.perf_counter() - t0) * 1e3
+    # the first call tunes (a cache file of its own, empty), the second is
+    # answered from the cache the first one filled
+    import tempfile
+    tune_dir = tempfile.mkdtemp(prefix="m4g_tune_")
+    os.environ["M4G_AUTOTUNE_CACHE"] = os.path.join(tune_dir, "batch_autotune.jsonl")
     t0 = time.perf_counter()
-    rpi, block, _ = B.autotune(nrhs, b)
+    rpi, block, trials = B.autotune(nrhs, b)
     tune_ms = (time.perf_counter() - t0) * 1e3
-    x.fill(0.0)
-    g.synchronize()
+    tuned = bool(np.isfinite(trials).any())
     t0 = time.perf_counter()
-    its, res, err = B.cg_ic0(nrhs, x, b, 10000, 1e-8, rhs_per_item=rpi, block=block)
-    solve_ms = (time.perf_counter() - t0) * 1e3
+    rpi2, block2, _ = B.autotune(nrhs, b)          # same size classes and nrhs: answered from the cache
+    tune_again_ms = (time.perf_counter() - t0) * 1e3
+    assert block2 == block and np.array_equal(rpi2, rpi)
+    del os.environ["M4G_AUTOTUNE_CACHE"]
+    solve_runs = []
+    for _ in range(2):
+        x.fill(0.0)
+        g.synchronize()
+        t0 = time.perf_counter()
+        its, res, err = B.cg_ic0(nrhs, x, b, 10000, 1e-8, rhs_per_item=rpi, block=block)
+        solve_runs.append((time.perf_counter() - t0) * 1e3)
+    solve_ms = min(solve_runs)
     nb = np.array([[np.linalg.norm(bs[k][j]) for j in range(nrhs)] for k in range(nsys)])
     relres = float(np.max(res / nb))
     # HBM bytes the solve moves, per (system row, rhs, iteration): 17 vector
@@ -129,38 +150,54 @@ def batched(g, nsys=512, nrhs=8, sample=16):
     # sweeps (2 x 36 B) shared by the rhs of a work item (8 here)
     row_its = float(np.sum(its * sizes[:, None]))
     solve_bytes = row_its * (17 * 8 + (5 * 12 + 2 * 36) / 8.0)
-    # loop of single solves on a sample spread over the sizes
+    # loop of single solves: per size class, `per_class` systems spread over it
     order = np.argsort(sizes, kind="stable")
-    pick = [int(order[(2 * k + 1) * nsys // (2 * sample)]) for k in range(sample)]
-    loop_ms, loop_its = 0.0, []
-    for k in pick:
-        S, I, V = systems[k]
-        g.synchronize()
-        t0 = time.perf_counter()
-        A = g.Matrix(S, I, V)
-        P = g.PC(A, g.PC_IC0)
-        xs, bv = g.Vector(int(sizes[k])), g.Vector(int(sizes[k]))
-        for j in range(nrhs):
-            bv.upload(bs[k][j])
-            xs.fill(0.0)
-            it, _ = g.make_iteration(10000, 1e-8, 0.0, float(nb[k, j]), history=False)
-            assert g.cg(A, xs, bv, P, it) == 0
-            loop_its.append(it.iterations)
-        g.synchronize()
-        loop_ms += (time.perf_counter() - t0) * 1e3
-    loop_est_ms = loop_ms * nsys / sample
-    batched_ms = factor_ms + solve_ms
+    classes = [order[c * nsys // 3:(c + 1) * nsys // 3] for c in range(3)]
+    loop_est_ms, loop_setup_ms, loop_sample_ms, loop_its, sampled = 0.0, 0.0, 0.0, [], 0
+    for cls in classes:
+        pick = [int(cls[(2 * k + 1) * len(cls) // (2 * per_class)]) for k in range(per_class)]
+        cls_ms = 0.0
+        for k in pick:
+            S, I, V = systems[k]
+            g.synchronize()
+            t0 = time.perf_counter()
+            A = g.Matrix(S, I, V)
+            P = g.PC(A, g.PC_IC0)
+            xs, bv = g.Vector(int(sizes[k])), g.Vector(int(sizes[k]))
+            g.synchronize()
+            loop_setup_ms += (time.perf_counter() - t0) * 1e3
+            for j in range(nrhs):
+                bv.upload(bs[k][j])
+                xs.fill(0.0)
+                it, _ = g.make_iteration(10000, 1e-8, 0.0, float(nb[k, j]), history=False)
+                g.synchronize()
+                t0 = time.perf_counter()
+                assert g.cg(A, xs, bv, P, it) == 0
+                cls_ms += (time.perf_counter() - t0) * 1e3
+                loop_its.append(abs(int(it.iterations) - int(its[k, j])))
+            sampled += 1
+            P.close()
+            A.close()
+        loop_sample_ms += cls_ms
+        loop_est_ms += cls_ms * len(cls) / per_class
     return {"workload": f"batched_varcoef5_{nsys}x{nrhs}", "nsys": nsys, "nrhs": nrhs, "n_total": n_total,
             "nnz_total": nnz_total, "side_range": [int(sides.min()), int(sides.max())],
-            "factor_ms": round(factor_ms, 2), "solve_ms": round(solve_ms, 2), "autotune_ms": round(tune_ms, 1),
+            "create_ms": round(create_ms, 1), "factor_ms": round(factor_ms, 2),
+            "autotune_ms": round(tune_ms, 1), "autotune_ran_trials": tuned,
+            "autotune_cached_ms": round(tune_again_ms, 2),
+            "solve_ms": round(solve_ms, 2), "solve_runs_ms": [round(t, 2) for t in solve_runs],
             "block": block, "rhs_per_item_by_class": sorted(set(int(r) for r in rpi)),
             "all_converged": bool((err == 0).all()), "max_relres": relres,
             "iterations_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
-            "loop_sample_systems": sample, "loop_sample_ms": round(loop_ms, 1),
+            "loop_sample_systems": sampled, "loop_sample_solve_ms": round(loop_sample_ms, 1),
+            "loop_sample_setup_ms": round(loop_setup_ms, 1),
             "loop_estimate_ms": round(loop_est_ms, 1),
+            "loop_vs_batched_max_iteration_diff": int(max(loop_its)),
+            "comparison": "solve time only on both sides (set-up reported apart); the loop is scaled per size "
+                          "class from its sampled systems",
             "solve_hbm_bytes_est": int(solve_bytes),
             "solve_gbs_est": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 1),
-            "speedup_vs_loop": round(loop_est_ms / batched_ms, 2)}
+            "speedup_vs_loop": round(loop_est_ms / solve_ms, 2)}
 
 
 def laplacian_edge_triplets(nx, ny, nz):
@@ -461,7 +498,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--cpu-iters", type=int, default=12, help="PCG iterations of the bounded CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--mixed-grid", type=int, default=96)
@@ -631,6 +667,9 @@ def main():
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
 
     # ---- CPU baseline: the oracle (MTL4's algorithms restated), one thread ----
+    # The whole itl::cg call to the same 1e-8 tolerance is timed, as the
+    # reference's own harness times cg() (timing/ic_0_cg_timing.cpp:46-49);
+    # the IC(0) factorisation is timed on its own and left out, as on the GPU.
     cpu = cpu_omp = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -638,31 +677,34 @@ def main():
         Ao = o.CSR(S, I, V)
         bh = b.numpy()
         t0 = time.perf_counter()
-        st, U, L = o.ic0_factor(Ao)
+        pc_o = o.make_pc(Ao, o.PC_IC0)
         cpu_factor_ms = (time.perf_counter() - t0) * 1e3
         t0 = time.perf_counter()
-        res = o.solve("cg", Ao, bh, pc=o.PC_IC0, max_iter=args.cpu_iters, rtol=args.rtol, history=False)
-        cpu_sample_s = time.perf_counter() - t0 - cpu_factor_ms * 1e-3   # solve() factorises again
-        per_iter_ms = cpu_sample_s * 1e3 / max(1, res["iterations"])
-        cpu_ms = per_iter_ms * iters[-1]
+        res = o.solve("cg", Ao, bh, pc=o.PC_IC0, max_iter=10000, rtol=args.rtol, history=False, prebuilt=pc_o)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        assert res["status"] == 0, res["status"]
+        cpu_relres = o.two_norm(o.residual(Ao, res["x"], bh)) / o.two_norm(bh)
         cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"oracle IC(0) factorisation ({cpu_factor_ms:.0f} ms) and {res['iterations']} PCG "
-                         f"iterations on the same 128^3 system; value = per-iteration time x the GPU's "
-                         f"{iters[-1]} iterations (solve only, factorisation excluded as on the GPU side)",
-               "factor_ms": round(cpu_factor_ms, 1), "ms_per_iteration": round(per_iter_ms, 2)}
+               "sample": f"the whole oracle IC(0)-PCG solve of the same 128^3 system from x0 = 0 to "
+                         f"{args.rtol:g} ({res['iterations']} iterations, true relres {cpu_relres:.2e}); the "
+                         f"IC(0) factorisation ({cpu_factor_ms:.0f} ms) is timed apart and excluded, as on the "
+                         f"GPU side",
+               "iterations": res["iterations"], "factor_ms": round(cpu_factor_ms, 1),
+               "ms_per_iteration": round(cpu_ms / max(1, res["iterations"]), 2)}
+        U, L = pc_o[1][1], pc_o[1][0]
         # MTL_WITH_OPENMP semantics: parallel SpMV, dots and vector loops,
         # sequential factorisation and sweeps, on every host core available
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
         t0 = time.perf_counter()
-        _, omp_its = o.omp_ic0_pcg(Ao, U, L, bh, args.rtol, args.cpu_iters, threads)
-        omp_iter_ms = (time.perf_counter() - t0) * 1e3 / max(1, omp_its)
-        cpu_omp = {"value": round(omp_iter_ms * iters[-1], 1), "unit": "ms", "cores": threads, "kind": "port",
-                   "sample": f"oracle IC(0) factor, then {omp_its} PCG iterations of the same 128^3 system with "
-                             f"{threads} OpenMP threads (row loops of SpMV, dots and updates parallel, sweeps "
-                             f"sequential, as MTL_WITH_OPENMP); value = per-iteration time x the GPU's "
-                             f"{iters[-1]} iterations",
-                   "ms_per_iteration": round(omp_iter_ms, 2)}
+        _, omp_its = o.omp_ic0_pcg(Ao, U, L, bh, args.rtol, 10000, threads)
+        omp_ms = (time.perf_counter() - t0) * 1e3
+        cpu_omp = {"value": round(omp_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+                   "sample": f"the whole IC(0)-PCG solve of the same 128^3 system to {args.rtol:g} "
+                             f"({omp_its} iterations) with {threads} OpenMP threads (row loops of SpMV, dots "
+                             f"and updates parallel, sweeps sequential, as MTL_WITH_OPENMP); factorisation "
+                             f"excluded",
+                   "iterations": omp_its, "ms_per_iteration": round(omp_ms / max(1, omp_its), 2)}
 
     if cpu:
         host = {"nproc": os.cpu_count(), "cpu_model": None}

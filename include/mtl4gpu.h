@@ -320,7 +320,12 @@ int m4g_batch_cg_ic0(m4g_batch B, int nrhs, m4g_vector x, m4g_vector b, int max_
  * third of the systems by size try the other groupings.  Returns the block
  * size and each system's grouping.  trial_ms (optional, up to 24 entries)
  * receives the trial times: 3 * groupings of the first stage, then 3 *
- * groupings of the second. */
+ * groupings of the second.  Answers are cached, keyed by the size classes
+ * (count, smallest and largest n of each third), nrhs, trial_iters, the packed
+ * sweeps' width and the device: in the process and in a JSON Lines file
+ * ($M4G_AUTOTUNE_CACHE, else $XDG_CACHE_HOME or $HOME/.cache, under
+ * mtl4gpu/batch_autotune.jsonl; M4G_AUTOTUNE_CACHE=off: process only).  A
+ * cached answer runs no trials and leaves trial_ms untouched. */
 int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int* rhs_per_item, int* block,
                        double* trial_ms);
 

@@ -19,7 +19,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <string>
+#include <sys/stat.h>
 #include <vector>
 
 namespace m4g {
@@ -964,6 +970,92 @@ int m4g_batch_cg_ic0(m4g_batch B, int nrhs, m4g_vector x, m4g_vector b, int max_
     });
 }
 
+// ---------------------------------------------------------------------------
+// Autotune cache.  The tuner's answer depends on the batch only through its
+// size classes (thirds of the systems by size: count, smallest and largest n
+// of each), nrhs, the trial length, the packed sweeps' width and the device,
+// so that tuple is the key.  Answers are kept for the process and in a JSON
+// Lines file, one object per line:
+//   {"key": "...", "block": 1024, "rhs_per_class": [8, 8, 4]}
+// at $M4G_AUTOTUNE_CACHE, else $XDG_CACHE_HOME/mtl4gpu/batch_autotune.jsonl,
+// else $HOME/.cache/mtl4gpu/batch_autotune.jsonl; M4G_AUTOTUNE_CACHE=off keeps
+// the answers in the process only.  A file that cannot be read or written is
+// ignored (the tuner then runs as without a cache).
+// ---------------------------------------------------------------------------
+struct TuneAnswer {
+    int block;
+    std::vector<int> rpc;   // rhs per item of each size class
+};
+
+static std::mutex g_tune_mutex;
+static std::map<std::string, TuneAnswer> g_tune_mem;
+
+static std::string tune_cache_path()
+{
+    const char* e = std::getenv("M4G_AUTOTUNE_CACHE");
+    if (e && *e) return std::string(e) == "off" ? std::string() : std::string(e);
+    std::string dir;
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) dir = x;
+    else if (const char* h = std::getenv("HOME"); h && *h) dir = std::string(h) + "/.cache";
+    else return std::string();
+    return dir + "/mtl4gpu/batch_autotune.jsonl";
+}
+
+static void tune_cache_load(const std::string& path)
+{
+    FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[1024];
+    while (std::fgets(line, sizeof line, f)) {
+        char key[512];
+        int block = 0, r[3] = {0, 0, 0};
+        const int got = std::sscanf(line, " {\"key\": \"%511[^\"]\", \"block\": %d, \"rhs_per_class\": [%d, %d, %d]}",
+                                    key, &block, &r[0], &r[1], &r[2]);
+        if (got < 3 || (block != 256 && block != 512 && block != 1024)) continue;   // not an entry of ours
+        TuneAnswer a{block, {}};
+        for (int c = 0; c < got - 2; c++) {
+            if (r[c] < 1 || r[c] > kBatchMaxR) { a.rpc.clear(); break; }
+            a.rpc.push_back(r[c]);
+        }
+        if (!a.rpc.empty()) g_tune_mem[key] = a;   // later lines win
+    }
+    std::fclose(f);
+}
+
+static void tune_cache_append(const std::string& path, const std::string& key, const TuneAnswer& a)
+{
+    if (path.empty()) return;
+    // create the two directory levels under the cache root if needed
+    for (size_t pos = path.find('/', 1); pos != std::string::npos; pos = path.find('/', pos + 1))
+        (void)::mkdir(path.substr(0, pos).c_str(), 0755);
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return;
+    std::string r;
+    for (size_t c = 0; c < a.rpc.size(); c++) r += (c ? ", " : "") + std::to_string(a.rpc[c]);
+    std::fprintf(f, "{\"key\": \"%s\", \"block\": %d, \"rhs_per_class\": [%s]}\n", key.c_str(), a.block, r.c_str());
+    std::fclose(f);
+}
+
+static std::string tune_key(const m4g_batch_s& B, const std::vector<std::vector<int>>& cls, int nrhs,
+                            int trial_iters)
+{
+    hipDeviceProp_t prop;
+    M4G_HIP(hipGetDeviceProperties(&prop, device_state().device));
+    std::string k = std::string(prop.gcnArchName) + "/cu" + std::to_string(prop.multiProcessorCount) + "/nrhs" +
+                    std::to_string(nrhs) + "/trial" + std::to_string(trial_iters) + "/w" + std::to_string(B.W);
+    for (const auto& c : cls) {
+        int64_t lo = INT64_MAX, hi = 0;
+        for (int s : c) {
+            lo = std::min(lo, B.h_sys[s].n);
+            hi = std::max(hi, B.h_sys[s].n);
+        }
+        k += "/" + std::to_string(c.size()) + ":" + std::to_string(lo) + "-" + std::to_string(hi);
+    }
+    for (char& ch : k)
+        if (ch == '"' || ch == '\\' || ch == ' ') ch = '_';
+    return k;
+}
+
 int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int* rhs_per_item, int* block,
                        double* trial_ms)
 {
@@ -980,6 +1072,24 @@ int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int
         const int nclass = int(std::min<int64_t>(3, B->nsys));
         std::vector<std::vector<int>> cls(nclass);
         for (int64_t k = 0; k < B->nsys; k++) cls[size_t(k * nclass / B->nsys)].push_back(order[k]);
+        const std::string key = tune_key(*B, cls, nrhs, trial_iters);
+        const std::string path = tune_cache_path();
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            auto hit = g_tune_mem.find(key);
+            if (hit == g_tune_mem.end()) {
+                tune_cache_load(path);
+                hit = g_tune_mem.find(key);
+            }
+            if (hit != g_tune_mem.end() && int(hit->second.rpc.size()) == nclass) {
+                // a cached answer runs no trials and leaves trial_ms untouched
+                *block = hit->second.block;
+                for (int c = 0; c < nclass; c++)
+                    for (int sidx : cls[c]) rhs_per_item[sidx] = hit->second.rpc[c];
+                B->last_block = *block;
+                return M4G_OK;
+            }
+        }
         const int blocks[3] = {256, 512, 1024};
         std::vector<int> Rs;
         for (int R = kBatchMaxR; R >= 1; R /= 2)
@@ -1054,6 +1164,13 @@ int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int
         M4G_HIP(hipEventDestroy(e1));
         *block = blocks[best_b];
         for (int64_t k = 0; k < B->nsys; k++) rhs_per_item[k] = cur[k];
+        {
+            TuneAnswer ans{blocks[best_b], {}};
+            for (int c = 0; c < nclass; c++) ans.rpc.push_back(cur[cls[c].front()]);
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            g_tune_mem[key] = ans;
+            tune_cache_append(path, key, ans);
+        }
         if (trial_ms)
             for (size_t k = 0; k < t.size(); k++) trial_ms[k] = t[k];
         B->last_block = *block;

@@ -267,14 +267,16 @@ def make_pc(A, kind):
 
 
 def solve(method, A, b, x0=None, pc=PC_IC0, max_iter=1000, rtol=1e-10, atol=0.0, norm_r0=None,
-          history=True, right=None, param=None):
+          history=True, right=None, param=None, prebuilt=None):
     """Runs itl::cg / bicgstab / cgs / bicg / tfqmr / qmr / gmres / gmres_full /
     bicgstab_2 / bicgstab_ell / idr_s on the oracle; `pc` is the (left)
     preconditioner kind, `right` the right one (None: identity), `param` the
-    restart / l / s argument.  Returns dict(x, status, iterations, resid, history)."""
+    restart / l / s argument; `prebuilt` a make_pc() result to use instead of
+    building the left preconditioner here (so a timing can leave the
+    factorisation out).  Returns dict(x, status, iterations, resid, history)."""
     b = np.ascontiguousarray(b, np.float64)
     x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
-    P, keep = make_pc(A, pc)
+    P, keep = make_pc(A, pc) if prebuilt is None else prebuilt
     it = Iteration()
     it.max_iter = max_iter
     it.rtol, it.atol = rtol, atol
