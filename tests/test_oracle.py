@@ -457,6 +457,100 @@ def test_gmres_known_answers():
     assert abs(o.two_norm(o.residual(N, r["x"], b)) - r["resid"]) <= 1e-12 * o.two_norm(b)
 
 
+def _csr_from_dense(M):
+    S, I, V = [0], [], []
+    for row in M:
+        for j, v in enumerate(row):
+            if v != 0:
+                I.append(j)
+                V.append(float(v))
+        S.append(len(I))
+    return o.CSR(np.array(S), np.array(I), np.array(V))
+
+
+@pytest.mark.parametrize("method", ["gmres", "gmres_full"])
+def test_gmres_worked_swap_case(method):
+    # Worked by hand through gmres.hpp:59-116 for A = [[0, 1], [1, 0]],
+    # b = e1, x0 = 0 (identity preconditioners, atol 0), restart 2:
+    #   k = 0: v0 = e1, A v0 = e2, H00 = 0, H10 = 1, v1 = e2; the rotation
+    #          c0 = 0, s0 = -1 makes H00 = 1, H10 = 0 and g = [0, -1].
+    #   k = 1: A v1 = e1, H01 = 1, H11 = 0, H21 = 0 (breakdown: v2 = 0).
+    #          The loop over i < k builds its rotation from column k - 1,
+    #          (H00, H10) = (1, 0): c = 1, s = 0 (givens.hpp:43-44), so column
+    #          1 is NOT rotated by (c0, s0) (gmres.hpp:84-85); nu = 0.
+    #   lu_solve(H[0:2][0:2] = [[1, 1], [0, 0]], [0, -1]) does not throw: lu()
+    #   tests the pivots of rows 0 .. n-2 only (lu.hpp:76-84), and the
+    #   backward substitution divides by the zero pivot: y = [+inf, -inf].
+    #   x += V y multiplies 0 * inf in both entries: x = [NaN, NaN], and
+    #   terminate(b - A x) records a NaN residual.
+    # gmres_full's terminate() returns the error code, 0 while i < max_iter
+    # (basic_iteration.hpp:43-47, 80); the restarted driver keeps cycling on
+    # NaN until max_iter (status 1).
+    # Textbook GMRES (stored rotations applied to column 1) gives
+    # H = [[1, 0], [0, 1]], g = [0, 1] and the exact x = e2 in two steps; a
+    # pivot test on the last row would fall back to one vector and x = 0.
+    A = _csr_from_dense([[0, 1], [1, 0]])
+    b = np.array([1.0, 0.0])
+    for max_iter in (2, 6):
+        r = o.solve(method, A, b, pc=0, max_iter=max_iter, rtol=1e-10, param=2)
+        assert np.isnan(r["x"]).all() and np.isnan(r["resid"])
+        if method == "gmres_full":
+            assert r["iterations"] == 2 and r["status"] == (o.MAX_ITER if max_iter == 2 else o.OK)
+        else:
+            assert r["iterations"] == max_iter and r["status"] == o.MAX_ITER
+    # restart 1: g = [0, -1] after the first rotation, so the loop that drops
+    # trailing directions with |g[k-1]| <= atol (gmres.hpp:95) leaves none and
+    # the cycle fails with code 2 ("did not find any direction", :107-108),
+    # x untouched
+    r = o.solve("gmres", A, b, pc=0, max_iter=3, rtol=1e-10, param=1)
+    np.testing.assert_array_equal(r["x"], [0.0, 0.0])
+    assert r["status"] == 2 and r["iterations"] == 1 and r["resid"] == 1.0
+
+
+@pytest.mark.parametrize("restart", [2, 3, 5])
+def test_gmres_restart_cycle_invariants(restart):
+    # Each cycle of itl::gmres ends with iter.terminate(b - A x)
+    # (gmres.hpp:115-116): the residual it reports after c cycles is the true
+    # residual of the x it returns (identity preconditioners), bit for bit.
+    # And a cycle only adds a combination of its Arnoldi vectors:
+    # x_c - x_{c-1} lies in span{r, A r, ..., A^(m-1) r} with r = b - A x_{c-1}.
+    A = o.CSR(*_varcoef_like(9))
+    rng = np.random.default_rng(11)
+    b = rng.uniform(-1, 1, A.n)
+    x_prev = np.zeros(A.n)
+    for cycle in range(1, 5):
+        r = o.solve("gmres", A, b, x0=x_prev, pc=0, max_iter=restart, rtol=1e-300, param=restart)
+        assert r["iterations"] == restart and r["status"] == o.MAX_ITER
+        assert r["resid"] == o.two_norm(o.residual(A, r["x"], b))
+        rr = o.residual(A, x_prev, b)
+        K = [rr / np.linalg.norm(rr)]
+        for _ in range(restart - 1):
+            v = o.spmv(A, K[-1])
+            K.append(v / np.linalg.norm(v))
+        K = np.stack(K, axis=1)
+        d = r["x"] - x_prev
+        coef, *_ = np.linalg.lstsq(K, d, rcond=None)
+        assert np.linalg.norm(K @ coef - d) <= 1e-10 * max(np.linalg.norm(d), 1e-300), cycle
+        x_prev = r["x"]
+
+
+def _varcoef_like(m):
+    # nonsymmetric 5-point operator on an m x m grid (convection-like
+    # off-diagonals), diagonal dominant
+    S, I, V = [0], [], []
+    for r in range(m * m):
+        i, j = r % m, r // m
+        for c, v in ((r - m, -1.3), (r - 1, -0.7), (r, 4.5), (r + 1, -1.1), (r + m, -0.9)):
+            if c < 0 or c >= m * m:
+                continue
+            if (c == r - 1 and i == 0) or (c == r + 1 and i == m - 1):
+                continue
+            I.append(c)
+            V.append(v)
+        S.append(len(I))
+    return np.array(S), np.array(I), np.array(V)
+
+
 @pytest.mark.parametrize("method,param", [("bicgstab_2", None), ("bicgstab_ell", 1), ("bicgstab_ell", 3),
                                           ("idr_s", 2), ("idr_s", 4)])
 def test_ell_idr_residual_consistency(method, param):
