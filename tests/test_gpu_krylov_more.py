@@ -227,3 +227,34 @@ def test_interrupt_flag_stops_host_driven_solvers(g):
             fn(x, it)
         assert e.value.status == g.ERR_INTERRUPTED
         assert it.iterations <= 2
+
+
+@pytest.mark.parametrize("method,restart", [("gmres", 2), ("gmres_full", None), ("gmres", 1)])
+def test_gmres_worked_swap_case(g, method, restart):
+    # The case worked by hand in tests/test_oracle.py (A = [[0, 1], [1, 0]],
+    # b = e1, x0 = 0): the reference's rotation sequence leaves H singular,
+    # lu_solve divides by the unchecked last pivot and x becomes NaN
+    # (restart 2, gmres_full); restart 1 fails with code 2, x untouched.  The
+    # GPU path must give the oracle's outcome, not textbook GMRES's exact e2.
+    S, I, V = np.array([0, 1, 2]), np.array([1, 0]), np.array([1.0, 1.0])
+    A = o.CSR(S, I, V)
+    b = np.array([1.0, 0.0])
+    ref = o.solve(method, A, b, pc=0, max_iter=6, rtol=1e-10, history=False, param=restart)
+    Ag = g.Matrix(S, I, V)
+    P = g.PC(Ag, g.PC_IDENTITY)
+    x = g.Vector(2)
+    x.fill(0.0)
+    it, _ = g.make_iteration(6, 1e-10, 0.0, 1.0, history=False)
+    try:
+        st = _run_gpu(g, method, restart, Ag, x, g.Vector(data=b), P, None, it)
+    except g.M4GError as e:
+        st = e.status
+    xs = x.numpy()
+    print(f"{method}({restart}): gpu status {st} it {it.iterations} x {xs}; oracle status {ref['status']} "
+          f"it {ref['iterations']} x {ref['x']}", flush=True)
+    assert it.iterations == ref["iterations"]
+    if np.isnan(ref["x"]).any():
+        assert np.isnan(xs).all() and st == ref["status"]
+    else:
+        np.testing.assert_array_equal(xs, ref["x"])
+        assert (st == g.BREAKDOWN) if ref["status"] == 2 else (st == ref["status"])
