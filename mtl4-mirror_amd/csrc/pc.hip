@@ -1076,7 +1076,9 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 std::vector<int64_t> a_pos;
                 M4G_REQUIRE(upper_pattern(n, A->h_starts.data(), A->h_indices.data(), P->fS, P->fI, a_pos),
                             M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
+                mark("pattern");
                 M4G_REQUIRE(ic0_rows(n, P->fS, P->fI, lo, up), M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
+                mark("rows");
                 // U = upper(A)
                 P->fstore.reset(std::max<size_t>(a_pos.size(), 1));
                 P->a_pos.reset(std::max<size_t>(a_pos.size(), 1));
@@ -1085,21 +1087,25 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(a_pos.size())), dim3(256), 0, s,
                                        int64_t(a_pos.size()), P->a_pos.ptr, A->values.ptr, P->fstore.ptr, 0.0);
                 M4G_HIP(hipGetLastError());
+                mark("gather");
                 lo_levels = natural_levels(lo, nullptr);
+                mark("levels");
                 factorize_levels(*P, lo, *A, lo_levels);
             } else {
                 P->fS = A->h_starts;
                 P->fI = A->h_indices;
                 M4G_REQUIRE(ilu0_rows(n, A->h_starts.data(), A->h_indices.data(), lo, up, P->diag_pos),
                             M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
+                mark("rows");
                 P->fstore.reset(std::max<int64_t>(A->nnz, 1));
                 if (A->nnz)
                     M4G_HIP(hipMemcpyAsync(P->fstore.ptr, A->values.ptr, A->nnz * sizeof(double),
                                            hipMemcpyDeviceToDevice, s));
                 lo_levels = natural_levels(lo, nullptr);
+                mark("levels");
                 factorize_levels(*P, lo, *A, lo_levels);
             }
-            mark("rows+factor_launch");
+            mark("factor");
             const GridInfo g = fg.get();
             mark("grid_wait");
             P->grid = g;

@@ -895,25 +895,38 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
         ~Join() { if (t.joinable()) t.join(); }
     } join{upper_thread};
     // lower sweep on L = trans(U): row i holds U's column i (rows k < i
-    // ascending), inverted diagonal last
+    // ascending), inverted diagonal last.  Split over threads by ranges of
+    // L rows when U is banded enough: the thread of L rows [c0, c1) scans the
+    // U rows [c0 - band, c1), in ascending order, so each L row is filled in
+    // the same order as by one pass over all U rows.
     lower.n = n;
     lower.lower = true;
     lower.ptr.assign(n + 1, 0);
-    lower.diag_pos.assign(n, -1);
+    lower.diag_pos.resize(n);
+    int64_t band = 0;   // largest column - row of U
     for (int64_t r = 0; r < n; r++)
-        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) lower.ptr[UI[p] + 1]++;
-    for (int64_t r = 0; r < n; r++) lower.ptr[r + 1] += lower.ptr[r];
-    lower.col.assign(lower.ptr[n], 0);
-    lower.val_pos.assign(lower.ptr[n], 0);
-    std::vector<int64_t> fill(lower.ptr.begin(), lower.ptr.end() - 1);
-    for (int64_t r = 0; r < n; r++) {
-        lower.diag_pos[r] = US[r];
-        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
-            int64_t c = UI[p], d = fill[c]++;
-            lower.col[d] = r;
-            lower.val_pos[d] = p;
+        if (US[r + 1] - US[r] > 1) band = std::max(band, UI[US[r + 1] - 1] - r);
+    const int nt = n >= 65536 && band * plan_threads() <= n ? std::max(1, plan_threads() - 1) : 1;
+    std::vector<int64_t> counts(n, 0);
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) counts[UI[p]]++;
+    for (int64_t r = 0; r < n; r++) lower.ptr[r + 1] = lower.ptr[r] + counts[r];
+    lower.col.resize(size_t(lower.ptr[n]));
+    lower.val_pos.resize(size_t(lower.ptr[n]));
+    parallel_chunks(n, nt, [&](int64_t c0, int64_t c1, int) {
+        std::vector<int64_t> fill(lower.ptr.begin() + c0, lower.ptr.begin() + c1);
+        for (int64_t r = std::max<int64_t>(0, c0 - band); r < c1; r++) {
+            if (r >= c0) lower.diag_pos[r] = US[r];
+            // entries of U row r with a column in [c0, c1) (columns ascend)
+            const int64_t* b = UI.data() + US[r] + 1;
+            const int64_t* e = UI.data() + US[r + 1];
+            for (const int64_t* q = std::lower_bound(b, e, c0); q < e && *q < c1; q++) {
+                const int64_t d = fill[*q - c0]++;
+                lower.col[d] = r;
+                lower.val_pos[d] = q - UI.data();
+            }
         }
-    }
+    });
     upper_thread.join();
     return upper_ok;
 }
