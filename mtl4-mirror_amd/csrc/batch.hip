@@ -916,6 +916,7 @@ int m4g_batch_factor_ic0(m4g_batch B)
 {
     return guard([&] {
         M4G_REQUIRE(B, M4G_ERR_INVALID, "null batch");
+        TraceRange tr(kTraceBatchIc0Factorize);
         hipStream_t s = current_stream();
         const int64_t cnt = B->u_total;
         if (cnt)
@@ -955,6 +956,7 @@ int m4g_batch_cg_ic0(m4g_batch B, int nrhs, m4g_vector x, m4g_vector b, int max_
         M4G_REQUIRE(x->type == M4G_F64 && b->type == M4G_F64 && x->n == B->n_total * nrhs && b->n == x->n,
                     M4G_ERR_SIZE, "m4g_batch_cg_ic0: x and b need n_total * nrhs doubles");
         if (block == 0) block = 512;
+        TraceRange tr(kTraceBatchCgIc0);
         std::vector<int> all(B->nsys);
         std::iota(all.begin(), all.end(), 0);
         DevBuf<int> d_it(size_t(B->nsys) * nrhs), d_err(size_t(B->nsys) * nrhs);
@@ -1064,6 +1066,7 @@ int m4g_batch_autotune(m4g_batch B, int nrhs, m4g_vector b, int trial_iters, int
         M4G_REQUIRE(B->factored, M4G_ERR_INVALID, "m4g_batch_autotune: call m4g_batch_factor_ic0 first");
         M4G_REQUIRE(b->type == M4G_F64 && b->n == B->n_total * nrhs, M4G_ERR_SIZE, "m4g_batch_autotune: b size");
         if (trial_iters <= 0) trial_iters = 40;
+        TraceRange tr(kTraceBatchAutotune);
         hipStream_t s = current_stream();
         // size classes: thirds of the systems by size
         std::vector<int> order(B->nsys);

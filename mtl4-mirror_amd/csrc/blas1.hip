@@ -109,6 +109,7 @@ int reduce_partials(int64_t n) { return reduce_grid(n); }
 
 void launch_dot_with(int64_t n, const double* x, const double* y, double* partials, double* out, hipStream_t s)
 {
+    TraceRange tr(kTraceDot);
     hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, y, partials, nullptr, out);
     launch_sum_partials(partials, reduce_grid(n), out, s);
     M4G_HIP(hipGetLastError());
@@ -121,6 +122,7 @@ void launch_dot(int64_t n, const double* x, const double* y, double* out, hipStr
 
 void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s)
 {
+    TraceRange tr(kTraceUnaryDot);
     ReduceWs& ws = reduce_ws();
     hipLaunchKernelGGL(k_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, ws.partials, ws.counters, out);
     launch_sum_partials(ws.partials, reduce_grid(n), out, s);
@@ -129,6 +131,7 @@ void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s)
 
 void launch_axpy_sumsq(int64_t n, double alpha, const double* q, double* r, double* out, hipStream_t s)
 {
+    TraceRange tr(kTraceVecExpr);
     ReduceWs& ws = reduce_ws();
     hipLaunchKernelGGL(k_axpy_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, alpha, q, r, ws.partials,
                        ws.counters, out);
@@ -138,6 +141,7 @@ void launch_axpy_sumsq(int64_t n, double alpha, const double* q, double* r, doub
 
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t s)
 {
+    TraceRange tr(kTraceVecExpr);
     if (n == 0) return;
     hipLaunchKernelGGL(k_axpy, dim3(stream_grid(n)), dim3(kNT), 0, s, n, alpha, x, y);
     M4G_HIP(hipGetLastError());
@@ -145,6 +149,7 @@ void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_
 
 void launch_xpay(int64_t n, const double* x, double beta, double* y, hipStream_t s)
 {
+    TraceRange tr(kTraceVecExpr);
     if (n == 0) return;
     hipLaunchKernelGGL(k_xpay, dim3(stream_grid(n)), dim3(kNT), 0, s, n, x, beta, y);
     M4G_HIP(hipGetLastError());

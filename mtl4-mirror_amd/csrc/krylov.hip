@@ -810,6 +810,8 @@ static int solve_common(int method, m4g_matrix A, m4g_vector x, m4g_vector b, m4
         M4G_REQUIRE(!P || P->n == A->nrows, M4G_ERR_SIZE, "solver: preconditioner size differs");
         M4G_REQUIRE(it->max_iter >= 0, M4G_ERR_INVALID, "solver: negative max_iter");
         if (it->history) M4G_REQUIRE(it->history_capacity > 0, M4G_ERR_INVALID, "solver: history capacity");
+        // cg.hpp:44 (no preconditioner) and :84, bicgstab.hpp:34
+        TraceRange tr(method == M4G_METHOD_CG ? (P ? kTraceCg : kTraceCgNoPc) : kTraceBicgstab);
         return krylov_solve(method, *A, x->ptr, b->ptr, x->type, P, it, default_ws());
     });
 }

@@ -880,6 +880,7 @@ int m4g_cgs(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* i
 {
     return guard([&] {
         check_args("m4g_cgs", A, x, b, it);
+        TraceRange tr(kTraceCgs);
         return cgs_t(*A, (double*)x->ptr, (const double*)b->ptr, P, it);
     });
 }
@@ -888,6 +889,7 @@ int m4g_bicg(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc P, m4g_iteration* 
 {
     return guard([&] {
         check_args("m4g_bicg", A, x, b, it);
+        TraceRange tr(kTraceBicg);
         return bicg_t(*A, (double*)x->ptr, (const double*)b->ptr, P, it);
     });
 }
@@ -896,6 +898,7 @@ int m4g_tfqmr(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_
 {
     return guard([&] {
         check_args("m4g_tfqmr", A, x, b, it);
+        TraceRange tr(kTraceTfqmr);
         return tfqmr_t(*A, (double*)x->ptr, (const double*)b->ptr, L, R, it);
     });
 }
@@ -904,6 +907,7 @@ int m4g_qmr(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_it
 {
     return guard([&] {
         check_args("m4g_qmr", A, x, b, it);
+        TraceRange tr(kTraceQmr);
         return qmr_t(*A, (double*)x->ptr, (const double*)b->ptr, L, R, it);
     });
 }
@@ -912,6 +916,7 @@ int m4g_gmres_full(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R,
 {
     return guard([&] {
         check_args("m4g_gmres_full", A, x, b, it);
+        TraceRange tr(kTraceGmresFull);
         return gmres_full_t(*A, (double*)x->ptr, (const double*)b->ptr, L, R, it);
     });
 }
@@ -920,6 +925,7 @@ int m4g_gmres(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_
 {
     return guard([&] {
         check_args("m4g_gmres", A, x, b, it);
+        TraceRange tr(kTraceGmres);
         return gmres_t(*A, (double*)x->ptr, (const double*)b->ptr, L, R, it, restart);
     });
 }
@@ -928,6 +934,7 @@ int m4g_bicgstab_2(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_itera
 {
     return guard([&] {
         check_args("m4g_bicgstab_2", A, x, b, it);
+        TraceRange tr(kTraceBicgstab2);
         return bicgstab_ell_t(*A, (double*)x->ptr, (const double*)b->ptr, L, nullptr, it, 2);
     });
 }
@@ -936,6 +943,7 @@ int m4g_bicgstab_ell(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc 
 {
     return guard([&] {
         check_args("m4g_bicgstab_ell", A, x, b, it);
+        TraceRange tr(kTraceBicgstabEll);
         return bicgstab_ell_t(*A, (double*)x->ptr, (const double*)b->ptr, L, R, it, l);
     });
 }
@@ -946,6 +954,7 @@ int m4g_idr_s(m4g_matrix A, m4g_vector x, m4g_vector b, m4g_pc L, m4g_pc R, m4g_
     (void)R;
     return guard([&] {
         check_args("m4g_idr_s", A, x, b, it);
+        TraceRange tr(kTraceIdrS);
         return idr_s_t(*A, (double*)x->ptr, (const double*)b->ptr, it, s);
     });
 }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/mtl4gpu.h"
+#include "trace.hpp"
 
 namespace m4g {
 

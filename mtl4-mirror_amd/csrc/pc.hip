@@ -788,6 +788,7 @@ template <typename FV, typename VT>
 static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, const int* skip, hipStream_t s)
 {
     const SweepPlan& p = d.plan_meta;
+    TraceRange tr(!p.lower ? kTraceUpperTrisolve : p.unit ? kTraceLowerUnitTrisolve : kTraceLowerTrisolve);
     if (p.ntiles == 0) return;
     d.ensure_recs(int(sizeof(VT)), s);
     SweepArgs a;
@@ -1030,6 +1031,30 @@ static void factorize_levels(m4g_pc_s& P, const SweepRows& lower_dag, const m4g_
     P.factor_levels = nlev;
 }
 
+// Trace event of a preconditioner application (identity.hpp:37-58,
+// diagonal.hpp:59, ic_0.hpp:64-101, ilu.hpp:86-108).
+static int pc_event(int kind, bool adjoint)
+{
+    switch (kind) {
+    case M4G_PC_IDENTITY: return adjoint ? kTraceIdAdjoint : kTraceIdSolve;
+    case M4G_PC_DIAGONAL: return kTraceDiagSolve;
+    case M4G_PC_IC0: return adjoint ? kTraceIc0Adjoint : kTraceIc0Solve;
+    default: return adjoint ? kTraceIlu0Adjoint : kTraceIlu0Solve;
+    }
+}
+
+// Trace event of a preconditioner's construction (diagonal.hpp:39,
+// ic_0.hpp:140, ilu_0.hpp:50); 0 for the identity, which has none.
+static int pc_setup_event(int kind)
+{
+    switch (kind) {
+    case M4G_PC_DIAGONAL: return kTraceDiagSetup;
+    case M4G_PC_IC0: return kTraceIc0Factorize;
+    case M4G_PC_ILU0: return kTraceIlu0Factorize;
+    default: return 0;
+    }
+}
+
 extern "C" {
 
 int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
@@ -1039,6 +1064,7 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
         M4G_REQUIRE(kind >= M4G_PC_IDENTITY && kind <= M4G_PC_ILU0, M4G_ERR_INVALID, "m4g_pc_create: unknown kind");
         M4G_REQUIRE(ftype == M4G_F64 || ftype == M4G_F32, M4G_ERR_INVALID, "m4g_pc_create: unknown value type");
         M4G_REQUIRE(A->nrows == A->ncols, M4G_ERR_NOT_SQUARE, "Matrix is not square");
+        TraceRange tr(pc_setup_event(kind));
         auto P = std::make_unique<m4g_pc_s>();
         P->kind = kind;
         P->factor_type = ftype;
@@ -1127,6 +1153,7 @@ int m4g_pc_refactor(m4g_pc P, m4g_matrix A)
     return guard([&] {
         M4G_REQUIRE(P && A, M4G_ERR_INVALID, "m4g_pc_refactor: null argument");
         M4G_REQUIRE(A->nrows == P->n && A->ncols == P->n, M4G_ERR_SIZE, "m4g_pc_refactor: incompatible size");
+        TraceRange tr(pc_setup_event(P->kind));
         hipStream_t s = current_stream();
         const int64_t n = P->n;
         if (P->kind == M4G_PC_DIAGONAL) {
@@ -1192,6 +1219,7 @@ void* pc_tmp(m4g_pc_s& P, int vtype)
 void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipStream_t s, const int* skip)
 {
     const int64_t n = P.n;
+    TraceRange tr(pc_event(P.kind, adjoint));
     switch (P.kind) {
     case M4G_PC_IDENTITY:
         if (n && x != y) M4G_HIP(hipMemcpyAsync(y, x, n * (vtype == M4G_F64 ? 8 : 4), hipMemcpyDeviceToDevice, s));

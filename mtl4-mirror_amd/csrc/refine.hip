@@ -46,6 +46,7 @@ extern "C" int m4g_refined_solve(int method, m4g_matrix A, m4g_vector x, m4g_vec
         M4G_REQUIRE(x->n == A->nrows && b->n == A->nrows, M4G_ERR_SIZE, "m4g_refined_solve: incompatible size");
         M4G_REQUIRE(!P || P->kind <= M4G_PC_DIAGONAL || P->factor_type == M4G_F32, M4G_ERR_INVALID,
                     "m4g_refined_solve: the preconditioner's factors must be float");
+        TraceRange tr(kTraceRefinedSolve);
         const int64_t n = A->nrows;
         hipStream_t s = current_stream();
         if (!A->sell.vals32.ptr) sell_build_f32(*A);

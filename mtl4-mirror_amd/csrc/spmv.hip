@@ -297,6 +297,7 @@ static void spmv_geometry(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
 
 void launch_spmv(const m4g_matrix_s& A, const double* x, double* w, int mode, const double* b, hipStream_t st)
 {
+    TraceRange tr(kTraceCrsMult);
     if (A.nrows == 0) return;
     if (const int mw = sell_pipe_width(A)) {
         const double* v = A.sell.vals.ptr;
@@ -322,6 +323,7 @@ void launch_spmv(const m4g_matrix_s& A, const double* x, double* w, int mode, co
 
 void launch_spmv_dot(const m4g_matrix_s& A, const double* p, double* q, double* out, hipStream_t st)
 {
+    TraceRange tr(kTraceCrsMult);
     if (A.nrows == 0) {
         M4G_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
         return;
