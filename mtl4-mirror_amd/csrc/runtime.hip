@@ -5,11 +5,8 @@
 
 namespace m4g {
 
-static thread_local std::string tl_last_error;
 static thread_local int tl_device = 0;
 static thread_local hipStream_t tl_stream = nullptr;   // user stream (nullptr = library stream)
-
-void set_last_error(const std::string& msg) { tl_last_error = msg; }
 
 static std::mutex g_states_mutex;
 static std::map<int, std::unique_ptr<DeviceState>> g_states;
@@ -158,8 +155,6 @@ using namespace m4g;
 
 extern "C" {
 
-const char* m4g_last_error(void) { return tl_last_error.c_str(); }
-
 const char* m4g_version(void)
 {
     return "mtl4gpu 0.1 gfx950 (fp-contract=off, packed-fp32-ops disabled)";
@@ -280,6 +275,7 @@ int m4g_vector_fill(m4g_vector v, double value)
 {
     return guard([&] {
         M4G_REQUIRE(v, M4G_ERR_INVALID, "m4g_vector_fill: null vector");
+        TraceRange tr(value == 0.0 ? kTraceSetZero : kTraceVecExpr);
         launch_fill(v->n, value, v->ptr, v->type, current_stream());
         return M4G_OK;
     });
@@ -287,6 +283,7 @@ int m4g_vector_fill(m4g_vector v, double value)
 
 int m4g_vector_copy(m4g_vector dst, m4g_vector src)
 {
+    TraceRange tr(kTraceVectorCopy);
     return guard([&] {
         M4G_REQUIRE(dst && src, M4G_ERR_INVALID, "m4g_vector_copy: null vector");
         M4G_REQUIRE(dst->n == src->n, M4G_ERR_SIZE, "m4g_vector_copy: incompatible size");
@@ -507,6 +504,7 @@ int m4g_unary_dot(m4g_vector x, double* result)
 
 int m4g_two_norm(m4g_vector x, double* result)
 {
+    TraceRange tr(kTraceTwoNorm);
     int st = m4g_unary_dot(x, result);
     if (st == M4G_OK) *result = std::sqrt(*result);
     return st;

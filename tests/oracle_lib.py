@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+# M4G_ORACLE_LIBRARY selects another build of the oracle (e.g. a sanitized one)
+LIB_PATH = os.environ.get("M4G_ORACLE_LIBRARY") or os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
