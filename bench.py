@@ -93,6 +93,20 @@ def measured_traffic(kernel):
     return None
 
 
+def calibration_axpy(g):
+    """axpy y += a x over 2^25 doubles, 3 launches: 512 MiB read and 256 MiB
+    written per launch -- the yardstick for the HBM byte counters."""
+    cal_x, cal_y = g.Vector(1 << 25), g.Vector(1 << 25)
+    cal_x.fill(1.0)
+    cal_y.fill(0.0)
+    g.synchronize()
+    for _ in range(3):
+        g.axpy(0.5, cal_x, cal_y)
+    g.synchronize()
+    cal_x.close()
+    cal_y.close()
+
+
 def batched(g, nsys=512, nrhs=8, per_class=6):
     """512 SPD 2D 5-point variable-coefficient systems (side ~ U{32..256},
     kappa ~ lognormal(0,1) per cell), 8 right-hand sides ~ N(0,1) each, IC(0)-PCG
@@ -507,6 +521,10 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation (adjoint) timings")
     ap.add_argument("--diff-only", action="store_true",
                     help="profiling run: only the differentiation workload (one JSON line)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="profiling run: only the mixed-precision comparison (one JSON line)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="profiling run: only the batched 512-system workload (one JSON line)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: a calibration axpy of known size, 5 SpMVs and 5 IC(0) applications")
     args = ap.parse_args()
@@ -519,6 +537,13 @@ def main():
 
     if args.diff_only:
         print(json.dumps({"differentiation": differentiation(g, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.mixed_only:
+        print(json.dumps({"mixed_precision": mixed_precision(g, args.mixed_grid)}), flush=True)
+        return
+    if args.batch_only:
+        calibration_axpy(g)
+        print(json.dumps({"batched": batched(g)}), flush=True)
         return
 
     N = args.grid
@@ -534,14 +559,7 @@ def main():
     setup_s = time.perf_counter() - t0
 
     if args.kernels_only:
-        # axpy y += a x over 2^25 doubles: 512 MiB read, 256 MiB written -- the
-        # yardstick for the HBM byte counters
-        cal_x, cal_y = g.Vector(1 << 25), g.Vector(1 << 25)
-        cal_x.fill(1.0)
-        cal_y.fill(0.0)
-        g.synchronize()
-        for _ in range(3):
-            g.axpy(0.5, cal_x, cal_y)
+        calibration_axpy(g)
         P = g.PC(A, g.PC_IC0)
         y = g.Vector(n)
         for _ in range(5):
