@@ -41,13 +41,14 @@ int m4g_analysis_pc_solve(int kind, int adjoint, int64_t n, const int64_t* S, co
         if (cap <= 0) cap = analysis_default_cap(vtype);
         SweepRows lo, up;
         if (kind == M4G_PC_IC0) {
-            std::vector<int64_t> US, UI, apos;
+            PlanVec64 US, UI;
+            PlanVec32 apos;
             if (!upper_pattern(n, S, I, US, UI, apos) || !ic0_rows(n, US, UI, lo, up)) {
                 set_last_error("missing diagonal");
                 return M4G_ERR_MISSING_DIAGONAL;
             }
         } else {
-            std::vector<int64_t> dpos;
+            PlanVec64 dpos;
             bool ok = adjoint ? ilu0_adjoint_rows(n, S, I, lo, up) : ilu0_rows(n, S, I, lo, up, dpos);
             if (!ok) {
                 set_last_error("missing diagonal");

@@ -49,8 +49,8 @@ template <> struct Sentinel<float> {
 __global__ __launch_bounds__(256) void k_ic0_level(int64_t count, const int32_t* __restrict__ level_rows,
                                                    const int64_t* __restrict__ lptr,
                                                    const int32_t* __restrict__ lcol,
-                                                   const int64_t* __restrict__ lpos,
-                                                   const int64_t* __restrict__ diag_pos, double* __restrict__ U,
+                                                   const int32_t* __restrict__ lpos,
+                                                   const int32_t* __restrict__ diag_pos, double* __restrict__ U,
                                                    double* __restrict__ invdiag)
 {
     int64_t t = blockIdx.x * int64_t(256) + threadIdx.x;
@@ -588,12 +588,12 @@ __global__ __launch_bounds__(64) void k_build_recs(const int4* __restrict__ grou
     for (int c = 0; c < C; c++) recs[int64_t(first) * C + int64_t(c) * m + q] = u.c[c];
 }
 
-template <typename T>
-__global__ void k_gather(int64_t count, const int64_t* __restrict__ pos, const double* __restrict__ f,
+template <typename T, typename IT>
+__global__ void k_gather(int64_t count, const IT* __restrict__ pos, const double* __restrict__ f,
                          T* __restrict__ out, T absent)
 {
     for (int64_t e = blockIdx.x * int64_t(256) + threadIdx.x; e < count; e += int64_t(gridDim.x) * 256) {
-        int64_t p = pos[e];
+        const int64_t p = pos[e];
         out[e] = p >= 0 ? T(f[p]) : absent;
     }
 }
@@ -961,20 +961,47 @@ void plan_sweep_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g
     pu = fu.get();
 }
 
-// Plans both directions and uploads them: the lower plan is uploaded (on this
-// thread, the library's stream) while the upper one is still being planned.
+// Plans of both sweep directions, built on two host threads from the moment
+// the job is made, so the caller's thread can meanwhile run the
+// factorisation; finish() uploads them (the lower one while the upper one may
+// still be in the making).  lo, up and lo_levels must outlive the job.
+class PlanPairJob {
+  public:
+    PlanPairJob(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
+                const std::vector<int32_t>* lo_levels)
+        : ftype_(ftype)
+    {
+        const size_t budget = sweep_lds_budget();   // device query: on the caller's thread
+        lo_ = std::async(std::launch::async,
+                         [&lo, g, cap, ftype, budget, lo_levels] {
+                             return plan_sweep_budget(lo, g, cap, ftype, budget, lo_levels);
+                         });
+        up_ = std::async(std::launch::async,
+                         [&up, g, cap, ftype, budget] { return plan_sweep_budget(up, g, cap, ftype, budget); });
+    }
+    PlanPairJob(const PlanPairJob&) = delete;
+    PlanPairJob& operator=(const PlanPairJob&) = delete;
+    ~PlanPairJob()
+    {
+        if (lo_.valid()) lo_.wait();
+        if (up_.valid()) up_.wait();
+    }
+    void finish(SweepDev& dl, SweepDev& du, const double* fstore)
+    {
+        dl.upload(lo_.get(), fstore, ftype_);
+        du.upload(up_.get(), fstore, ftype_);
+    }
+
+  private:
+    int ftype_;
+    std::future<SweepPlan> lo_, up_;
+};
+
 void plan_upload_pair(const SweepRows& lo, const SweepRows& up, const GridInfo& g, int64_t cap, int ftype,
                       SweepDev& dl, SweepDev& du, const double* fstore, const std::vector<int32_t>* lo_levels)
 {
-    const size_t budget = sweep_lds_budget();
-    auto fu = std::async(std::launch::async, [&] { return plan_sweep_budget(up, g, cap, ftype, budget); });
-    try {
-        dl.upload(plan_sweep_budget(lo, g, cap, ftype, budget, lo_levels), fstore, ftype);
-    } catch (...) {
-        fu.wait();
-        throw;
-    }
-    du.upload(fu.get(), fstore, ftype);
+    PlanPairJob job(lo, up, g, cap, ftype, lo_levels);
+    job.finish(dl, du, fstore);
 }
 
 }  // namespace m4g
@@ -1002,13 +1029,12 @@ static void factorize_levels(m4g_pc_s& P, const SweepRows& lower_dag, const m4g_
     DevBuf<double> invdiag(std::max<int64_t>(P.n, 1));
     hipStream_t s = current_stream();
     if (P.kind == M4G_PC_IC0) {
-        std::vector<int32_t> lcol(lower_dag.col.begin(), lower_dag.col.end());
-        DevBuf<int64_t> d_lptr(P.n + 1), d_lpos(std::max<size_t>(lower_dag.val_pos.size(), 1)),
-            d_diag(std::max<int64_t>(P.n, 1));
-        DevBuf<int32_t> d_lcol(std::max<size_t>(lcol.size(), 1));
+        DevBuf<int64_t> d_lptr(P.n + 1);
+        DevBuf<int32_t> d_lpos(std::max<size_t>(lower_dag.val_pos.size(), 1)), d_diag(std::max<int64_t>(P.n, 1)),
+            d_lcol(std::max<size_t>(lower_dag.col.size(), 1));
         d_lptr.upload(lower_dag.ptr.data(), P.n + 1);
         d_lpos.upload(lower_dag.val_pos.data(), lower_dag.val_pos.size());
-        d_lcol.upload(lcol.data(), lcol.size());
+        d_lcol.upload(lower_dag.col.data(), lower_dag.col.size());
         d_diag.upload(lower_dag.diag_pos.data(), P.n);
         for (int64_t l = 0; l < nlev; l++) {
             int64_t cnt = lptr[l + 1] - lptr[l];
@@ -1099,7 +1125,7 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
             SweepRows lo, up;
             std::vector<int32_t> lo_levels;   // natural levels of the lower DAG (factorisation and sweep)
             if (kind == M4G_PC_IC0) {
-                std::vector<int64_t> a_pos;
+                PlanVec32 a_pos;
                 M4G_REQUIRE(upper_pattern(n, A->h_starts.data(), A->h_indices.data(), P->fS, P->fI, a_pos),
                             M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
                 mark("pattern");
@@ -1110,16 +1136,13 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 P->a_pos.reset(std::max<size_t>(a_pos.size(), 1));
                 P->a_pos.upload(a_pos.data(), a_pos.size());
                 if (!a_pos.empty())
-                    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(a_pos.size())), dim3(256), 0, s,
+                    hipLaunchKernelGGL((k_gather<double, int32_t>), dim3(grid_for(a_pos.size())), dim3(256), 0, s,
                                        int64_t(a_pos.size()), P->a_pos.ptr, A->values.ptr, P->fstore.ptr, 0.0);
                 M4G_HIP(hipGetLastError());
                 mark("gather");
-                lo_levels = natural_levels(lo, nullptr);
-                mark("levels");
-                factorize_levels(*P, lo, *A, lo_levels);
             } else {
-                P->fS = A->h_starts;
-                P->fI = A->h_indices;
+                P->fS.assign(A->h_starts.begin(), A->h_starts.end());
+                P->fI.assign(A->h_indices.begin(), A->h_indices.end());
                 M4G_REQUIRE(ilu0_rows(n, A->h_starts.data(), A->h_indices.data(), lo, up, P->diag_pos),
                             M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
                 mark("rows");
@@ -1127,16 +1150,19 @@ int m4g_pc_create(m4g_pc* out, m4g_matrix A, int kind, int ftype)
                 if (A->nnz)
                     M4G_HIP(hipMemcpyAsync(P->fstore.ptr, A->values.ptr, A->nnz * sizeof(double),
                                            hipMemcpyDeviceToDevice, s));
-                lo_levels = natural_levels(lo, nullptr);
-                mark("levels");
-                factorize_levels(*P, lo, *A, lo_levels);
             }
-            mark("factor");
+            lo_levels = natural_levels(lo, nullptr);
+            mark("levels");
             const GridInfo g = fg.get();
             mark("grid_wait");
             P->grid = g;
-            int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
-            plan_upload_pair(lo, up, g, cap, ftype, P->lower, P->upper, P->fstore.ptr, &lo_levels);
+            // the sweep plans need only the pattern: they are built on host
+            // threads while this thread runs the factorisation on the device
+            const int64_t cap = sweep_tile_cap(ftype == M4G_F64 ? M4G_F64 : M4G_F32);
+            PlanPairJob plans(lo, up, g, cap, ftype, &lo_levels);
+            factorize_levels(*P, lo, *A, lo_levels);
+            mark("factor");
+            plans.finish(P->lower, P->upper, P->fstore.ptr);
             mark("plan+upload");
             P->tmp_bytes = 0;
         }
@@ -1164,19 +1190,21 @@ int m4g_pc_refactor(m4g_pc P, m4g_matrix A)
         } else if (P->kind == M4G_PC_IC0 || P->kind == M4G_PC_ILU0) {
             SweepRows lo, up;
             if (P->kind == M4G_PC_IC0) {
-                std::vector<int64_t> US, UI, apos;
+                PlanVec64 US, UI;
+                PlanVec32 apos;
                 M4G_REQUIRE(upper_pattern(n, A->h_starts.data(), A->h_indices.data(), US, UI, apos) && US == P->fS &&
                                 UI == P->fI,
                             M4G_ERR_INVALID, "m4g_pc_refactor: pattern differs");
                 M4G_REQUIRE(ic0_rows(n, P->fS, P->fI, lo, up), M4G_ERR_MISSING_DIAGONAL, "missing diagonal");
                 const int64_t nu = P->fS[n];
                 if (nu)
-                    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nu)), dim3(256), 0, s, nu, P->a_pos.ptr,
+                    hipLaunchKernelGGL((k_gather<double, int32_t>), dim3(grid_for(nu)), dim3(256), 0, s, nu, P->a_pos.ptr,
                                        A->values.ptr, P->fstore.ptr, 0.0);
             } else {
-                M4G_REQUIRE(A->h_starts == P->fS && A->h_indices == P->fI, M4G_ERR_INVALID,
-                            "m4g_pc_refactor: pattern differs");
-                std::vector<int64_t> dpos;
+                M4G_REQUIRE(std::equal(A->h_starts.begin(), A->h_starts.end(), P->fS.begin(), P->fS.end()) &&
+                                std::equal(A->h_indices.begin(), A->h_indices.end(), P->fI.begin(), P->fI.end()),
+                            M4G_ERR_INVALID, "m4g_pc_refactor: pattern differs");
+                PlanVec64 dpos;
                 M4G_REQUIRE(ilu0_rows(n, P->fS.data(), P->fI.data(), lo, up, dpos), M4G_ERR_MISSING_DIAGONAL,
                             "missing diagonal");
                 if (A->nnz)
@@ -1323,8 +1351,8 @@ int m4g_pc_download_factor(m4g_pc P, int which, int64_t* starts, int64_t* indice
         std::vector<int64_t> S(n + 1, 0), I;
         std::vector<double> V;
         if (P->kind == M4G_PC_IC0 && which == 1) {
-            S = P->fS;
-            I = P->fI;
+            S.assign(P->fS.begin(), P->fS.end());
+            I.assign(P->fI.begin(), P->fI.end());
             V = f;
         } else if (P->kind == M4G_PC_IC0) {
             // L = trans(U), rows sorted

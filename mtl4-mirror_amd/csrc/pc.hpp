@@ -53,10 +53,10 @@ struct m4g_pc_s {
     int64_t n = 0;
     m4g::GridInfo grid;
     // factor storage in double: IC(0) -> upper(A) CSR; ILU(0) -> A's CSR
-    std::vector<int64_t> fS, fI;          // pattern of the storage
-    std::vector<int64_t> diag_pos;        // ILU(0): storage index of each diagonal
+    m4g::PlanVec64 fS, fI;                // pattern of the storage
+    m4g::PlanVec64 diag_pos;              // ILU(0): storage index of each diagonal
     m4g::DevBuf<double> fstore;
-    m4g::DevBuf<int64_t> a_pos;           // IC(0): CSR index in A of each upper(A) entry
+    m4g::DevBuf<int32_t> a_pos;           // IC(0): CSR index in A of each upper(A) entry
     m4g::DevBuf<double> inv_diag;         // diagonal PC
     m4g::SweepDev lower, upper;           // solve: lower then upper
     m4g::SweepDev adj_lower, adj_upper;   // ILU(0) adjoint_solve

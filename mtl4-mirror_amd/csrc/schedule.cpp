@@ -16,6 +16,8 @@
 // consecutive rows.  Neither choice changes any arithmetic: the operation
 // order per row is the reference's (lower_trisolve.hpp:191-209 / 172-189,
 // upper_trisolve.hpp:159-176).
+#include <sys/mman.h>
+
 #include "schedule.hpp"
 
 #include <thread>
@@ -30,6 +32,18 @@
 #include <stdexcept>
 
 namespace m4g {
+
+void* large_block_alloc(size_t bytes)
+{
+    constexpr size_t huge = size_t(2) << 20;
+    const size_t rounded = (bytes + huge - 1) / huge * huge;
+    void* p = std::aligned_alloc(huge, rounded);
+    if (!p) throw std::bad_alloc();
+    madvise(p, rounded, MADV_HUGEPAGE);   // advice only: ignored where huge pages are off
+    return p;
+}
+
+void large_block_free(void* p) noexcept { std::free(p); }
 
 // Host threads a sweep plan may use: M4G_PLAN_THREADS, else half the
 // hardware threads (the two directions of a preconditioner are planned at
@@ -80,11 +94,44 @@ static int64_t grid_score_rows(int64_t n, const int64_t* S, const int64_t* I, in
     int64_t off[27];
     for (int code = 0; code < 27; code++)
         off[code] = (code % 3 - 1) + nx * ((code / 3) % 3 - 1) + nxy * (code / 9 - 1);
+    // offset -> code table (open addressing); an offset that two codes share
+    // (nx or ny below 3) is marked ambiguous and searched code by code
+    struct Slot {
+        int64_t off;
+        int code;   // -1 empty, -2 ambiguous
+    };
+    Slot tab[64];
+    for (Slot& sl : tab) sl = Slot{0, -1};
+    auto slot_of = [&](int64_t d) -> Slot& {
+        size_t h = size_t((uint64_t(d) * 0x9E3779B97F4A7C15ull) >> 58);
+        while (tab[h].code != -1 && tab[h].off != d) h = (h + 1) & 63;
+        return tab[h];
+    };
+    for (int code = 0; code < 27; code++) {
+        Slot& sl = slot_of(off[code]);
+        if (sl.code == -1) sl = Slot{off[code], code};
+        else sl.code = -2;
+    }
+    int64_t ri = r_begin % nx, rj = (r_begin / nx) % ny, rk = r_begin / nxy;
     for (int64_t r = r_begin; r < r_end; r += stride) {
-        const int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
+        if (stride != 1) {
+            ri = r % nx;
+            rj = (r / nx) % ny;
+            rk = r / nxy;
+        }
         int guess = 0;
         for (int64_t p = S[r]; p < S[r + 1]; p++) {
             const int64_t d = I[p] - r;
+            const Slot& sl = slot_of(d);
+            if (sl.code == -1) return -1;   // no neighbour has this offset
+            if (sl.code >= 0) {
+                const int code = sl.code;
+                const int64_t ci = ri + code % 3 - 1, cj = rj + (code / 3) % 3 - 1, ck = rk + code / 9 - 1;
+                if (!(ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz)) return -1;
+                seen |= uint32_t(1) << code;
+                guess = code + 1;
+                continue;
+            }
             int found = -1;
             for (int t = 0; t < 27 && found < 0; t++) {
                 const int code = (guess + t) % 27;
@@ -95,6 +142,13 @@ static int64_t grid_score_rows(int64_t n, const int64_t* S, const int64_t* I, in
             if (found < 0) return -1;
             seen |= uint32_t(1) << found;
             guess = found + 1;   // columns ascend: the next entry's offset is larger
+        }
+        if (stride == 1 && ++ri == nx) {   // next row's coordinates, stepped
+            ri = 0;
+            if (++rj == ny) {
+                rj = 0;
+                ++rk;
+            }
         }
     }
     return int64_t(seen);
@@ -131,10 +185,15 @@ GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
         std::vector<std::vector<int64_t>> seen{size_t(nt)};
         parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int th) {
             std::vector<int64_t>& sv = seen[size_t(th)];
+            int64_t recent[64];   // direct-mapped memo of offsets already listed
+            std::fill(recent, recent + 64, int64_t(0));
             for (int64_t r = r0; r < r1 && sv.size() <= 64; r++)
                 for (int64_t p = S[r]; p < S[r + 1]; p++)
                     if (I[p] < r) {
                         const int64_t d = r - I[p];
+                        int64_t& m = recent[(uint64_t(d) * 0x9E3779B97F4A7C15ull) >> 58];
+                        if (m == d) continue;
+                        m = d;
                         if (std::find(sv.begin(), sv.end(), d) == sv.end()) sv.push_back(d);
                     }
         });
@@ -150,8 +209,14 @@ GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
             int64_t nx = d + e;
             if (nx >= 2 && nx <= n && n % nx == 0) nxs.insert(nx);
         }
+    // every candidate scored on a sample of rows; then the full check, best
+    // candidate first: fewest distinct offsets (the tightest stencil), then
+    // the larger nx, then the later candidate (the larger nx*ny)
     const int64_t sample = std::max<int64_t>(1, n / 4096);
-    int64_t best_score = -1;
+    struct Cand {
+        int64_t score, nx, ny, nz, order;
+    };
+    std::vector<Cand> cands;
     for (int64_t nx : nxs) {
         std::set<int64_t> nxys;
         for (int64_t d : offs) {
@@ -163,19 +228,23 @@ GridInfo infer_grid(int64_t n, const int64_t* S, const int64_t* I)
         }
         nxys.insert(n);   // 2D (or 1D when nx == n)
         for (int64_t nxy : nxys) {
-            int64_t ny = nxy / nx, nz = n / nxy;
-            int64_t sc = grid_score(n, S, I, nx, ny, sample);
-            if (sc < 0) continue;
-            // fewest distinct offsets wins (the tightest stencil); ties keep the larger nx
-            if (best_score < 0 || sc < best_score || (sc == best_score && nx >= best.nx)) {
-                if (grid_score(n, S, I, nx, ny, 1) < 0) continue;
-                best_score = sc;
-                best.nx = nx;
-                best.ny = ny;
-                best.nz = nz;
-            }
+            const int64_t ny = nxy / nx, nz = n / nxy;
+            const int64_t sc = grid_score(n, S, I, nx, ny, sample);
+            if (sc >= 0) cands.push_back({sc, nx, ny, nz, int64_t(cands.size())});
         }
     }
+    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
+        if (a.score != b.score) return a.score < b.score;
+        if (a.nx != b.nx) return a.nx > b.nx;
+        return a.order > b.order;
+    });
+    for (const Cand& c : cands)
+        if (grid_score(n, S, I, c.nx, c.ny, 1) >= 0) {
+            best.nx = c.nx;
+            best.ny = c.ny;
+            best.nz = c.nz;
+            break;
+        }
     return best;
 }
 
@@ -216,14 +285,37 @@ static Coef skew_coefficients(const SweepRows& s, const GridInfo& g)
     std::vector<std::vector<std::array<int, 3>>> part{size_t(nt)};
     parallel_chunks(s.n, nt, [&](int64_t r0, int64_t r1, int th) {
         std::vector<std::array<int, 3>>& dl = part[size_t(th)];
+        // the offset c - r of an entry repeats from row to row: a small table
+        // maps it to the (di, dj, dk) last seen with it, which is that entry's
+        // as soon as it lands inside the grid from r (digits in range make
+        // the mixed-radix form c - r = di + nx*dj + nxy*dk unique)
+        struct Slot {
+            int64_t off;
+            int d[3];
+            bool used;
+        };
+        Slot tab[64] = {};
+        int64_t ri = r0 % nx, rj = (r0 / nx) % ny, rk = r0 / nxy;
         for (int64_t r = r0; r < r1; r++) {
-            const int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / nxy;
             for (int64_t p = s.ptr[r]; p < s.ptr[r + 1]; p++) {
-                const int64_t c = s.col[p];
+                const int64_t c = s.col[p], off = c - r;
+                Slot& sl = tab[(uint64_t(off) * 0x9E3779B97F4A7C15ull) >> 58];
+                if (sl.used && sl.off == off) {
+                    const int64_t ci = ri + sl.d[0], cj = rj + sl.d[1], ck = rk + sl.d[2];
+                    if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < g.nz) continue;
+                }
                 int di = int(c % nx - ri), dj = int((c / nx) % ny - rj), dk = int(c / nxy - rk);
+                sl = Slot{off, {di, dj, dk}, true};
                 if (!s.lower) { di = -di; dj = -dj; dk = -dk; }
                 const std::array<int, 3> d = {di, dj, dk};
                 if (std::find(dl.begin(), dl.end(), d) == dl.end()) dl.push_back(d);
+            }
+            if (++ri == nx) {
+                ri = 0;
+                if (++rj == ny) {
+                    rj = 0;
+                    ++rk;
+                }
             }
         }
     });
@@ -388,20 +480,31 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     // 1. tile of every row: linear box index, then tickets = the occupied
     //    boxes in topological order (ascending box index for the lower
     //    sweep, descending for the upper)
-    std::vector<int32_t> tile(n);
+    PlanVec32 tile(n);
     int64_t nbox = 1;
-    std::vector<int64_t> lin(n);
+    PlanVec64 lin(n);
     if (shape.grid) {
         const int64_t N1 = (g.nz - 1) / shape.e1 + 1;
         const int64_t N2 = ((g.ny - 1) + shape.a * (g.nz - 1)) / shape.e2 + 1;
         const int64_t N3 = ((g.nx - 1) + shape.b * (g.ny - 1) + shape.c * (g.nz - 1)) / shape.e3 + 1;
         nbox = N1 * N2 * N3;
         const int64_t nxy = g.nx * g.ny;
+        // along a grid line (fixed j, k) t3 grows by one per row: its box
+        // index is stepped, not divided
         parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
-            for (int64_t r = r0; r < r1; r++) {
+            for (int64_t r = r0; r < r1;) {
                 const int64_t k = r / nxy, j = (r - k * nxy) / g.nx, i = r - k * nxy - j * g.nx;
                 const int64_t t1 = k, t2 = j + shape.a * k, t3 = i + shape.b * j + shape.c * k;
-                lin[r] = ((t1 / shape.e1) * N2 + t2 / shape.e2) * N3 + t3 / shape.e3;
+                const int64_t head = ((t1 / shape.e1) * N2 + t2 / shape.e2) * N3;
+                int64_t q3 = t3 / shape.e3, m3 = t3 - q3 * shape.e3;
+                const int64_t line_end = std::min(r1, r + (g.nx - i));
+                for (; r < line_end; r++) {
+                    lin[r] = head + q3;
+                    if (++m3 == shape.e3) {
+                        m3 = 0;
+                        ++q3;
+                    }
+                }
             }
         });
     } else {
@@ -425,13 +528,13 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
             for (int64_t r = r0; r < r1; r++) tile[r] = rank[size_t(lin[r])];
         });
     }
-    std::vector<int64_t>().swap(lin);
+    PlanVec64().swap(lin);
     const int64_t T = p.ntiles;
 
     // 2. rows of each tile in natural order: per-chunk counts per tile, so
     //    every chunk writes its rows at fixed positions
     p.tile_row_begin.assign(T + 1, 0);
-    p.rows.assign(n, 0);
+    p.rows.resize(n);
     {
         const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(nt, n / 4096));
         std::vector<std::vector<int32_t>> cnt(size_t(nc), std::vector<int32_t>(size_t(T), 0));
@@ -479,7 +582,7 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     };
     std::vector<std::vector<Piece>> tpieces{size_t(T)};
     std::vector<std::vector<int32_t>> tgroups{size_t(T)};   // tile-local first piece of each group
-    std::vector<int32_t> llev(n, 0);   // local level of the row's last piece
+    PlanVec32 llev(n);   // local level of the row's last piece (written before it is read)
     std::vector<char> exported(n, 0);
     parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
         std::vector<Piece> raw;
@@ -604,9 +707,7 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
     p.pc_diag_pos.resize(p.npieces);
     p.pc_src.resize(size_t(p.npieces) * W);
     p.pc_val_pos.resize(size_t(p.npieces) * W);
-    int too_far = 0;   // a factor position needs more than 32 bits
     parallel_chunks(T, nt, [&](int64_t ta, int64_t tb, int) {
-        int64_t far = 0;
         for (int64_t t = ta; t < tb; t++) {
             const std::vector<Piece>& tp = tpieces[t];
             const int64_t base = tpiece0[t];
@@ -624,7 +725,7 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
                 p.pc_slot[q] = slot[r];
                 p.pc_len[q] = pc.len;
                 p.pc_export[q] = last ? xslot[r] : -1;
-                p.pc_diag_pos[q] = last ? int32_t(s.diag_pos[r]) : -1;
+                p.pc_diag_pos[q] = last ? s.diag_pos[r] : -1;
                 for (int32_t k = 0; k < W; k++) {
                     if (k >= pc.len) {
                         p.pc_src[q * W + k] = -1;   // padding: the zero slot, set below
@@ -633,15 +734,11 @@ SweepPlan build_sweep_plan(const SweepRows& s, const GridInfo& g, const TileShap
                     }
                     const int64_t e = s.ptr[r] + pc.k0 + k, c = s.col[e];
                     p.pc_src[q * W + k] = tile[c] == t ? slot[c] : -(xslot[c] + 1);
-                    p.pc_val_pos[q * W + k] = int32_t(s.val_pos[e]);
-                    far |= s.val_pos[e];
+                    p.pc_val_pos[q * W + k] = s.val_pos[e];
                 }
-                if (last && s.diag_pos[r] > 0) far |= s.diag_pos[r];   // -1: unit diagonal
             }
         }
-        if (far >> 31) __atomic_store_n(&too_far, 1, __ATOMIC_RELAXED);
     }, 1);
-    if (too_far) throw std::runtime_error("factor storage too large for a sweep plan (2^31 entries)");
     std::vector<std::vector<Piece>>().swap(tpieces);
 
     // 6. imports: the values each tile reads from other tiles, numbered per
@@ -822,13 +919,21 @@ template void emulate_sweep<float, float>(const SweepPlan&, const float*, const 
 
 namespace m4g {
 
-bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<int64_t>& US, std::vector<int64_t>& UI,
-                   std::vector<int64_t>& a_pos)
+// Rows and factor-storage indices of the sweeps are 32-bit.
+static void check_32bit(int64_t n, int64_t entries)
+{
+    if (n >= (int64_t(1) << 31) || entries >= (int64_t(1) << 31))
+        throw std::length_error("matrix too large for the sweeps: 2^31 or more rows or stored entries");
+}
+
+bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, PlanVec64& US, PlanVec64& UI, PlanVec32& a_pos)
 {
     // two passes over the rows (count, then fill), each split over the
     // planning threads; a U row must start with its diagonal
+    check_32bit(n, n ? S[n] : 0);
     const int nt = n >= 65536 ? plan_threads() : 1;
-    US.assign(n + 1, 0);
+    US.resize(n + 1);
+    US[0] = 0;
     int bad = 0;
     parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
         for (int64_t r = r0; r < r1; r++) {
@@ -851,16 +956,16 @@ bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<in
             const int64_t first = S[r + 1] - (US[r + 1] - US[r]);
             for (int64_t k = 0; k < US[r + 1] - US[r]; k++) {
                 UI[US[r] + k] = I[first + k];
-                a_pos[US[r] + k] = first + k;
+                a_pos[US[r] + k] = int32_t(first + k);
             }
         }
     });
     return true;
 }
 
-bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64_t>& UI, SweepRows& lower,
-              SweepRows& upper)
+bool ic0_rows(int64_t n, const PlanVec64& US, const PlanVec64& UI, SweepRows& lower, SweepRows& upper)
 {
+    check_32bit(n, n ? US[n] : 0);
     // upper sweep on U: diagonal first, then entries in column order (on a
     // second host thread; the lower rows below do not depend on it)
     bool upper_ok = true;
@@ -882,10 +987,10 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
             for (int64_t r = r0; r < r1; r++) {
                 upper.ptr[r] = US[r] - r;
                 if (r == n) continue;
-                upper.diag_pos[r] = US[r];
+                upper.diag_pos[r] = int32_t(US[r]);
                 for (int64_t p = US[r] + 1; p < US[r + 1]; p++) {
-                    upper.col[p - r - 1] = UI[p];
-                    upper.val_pos[p - r - 1] = p;
+                    upper.col[p - r - 1] = int32_t(UI[p]);
+                    upper.val_pos[p - r - 1] = int32_t(p);
                 }
             }
         });
@@ -901,29 +1006,36 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
     // the same order as by one pass over all U rows.
     lower.n = n;
     lower.lower = true;
-    lower.ptr.assign(n + 1, 0);
+    lower.ptr.resize(n + 1);
     lower.diag_pos.resize(n);
     int64_t band = 0;   // largest column - row of U
     for (int64_t r = 0; r < n; r++)
         if (US[r + 1] - US[r] > 1) band = std::max(band, UI[US[r + 1] - 1] - r);
     const int nt = n >= 65536 && band * plan_threads() <= n ? std::max(1, plan_threads() - 1) : 1;
-    std::vector<int64_t> counts(n, 0);
-    for (int64_t r = 0; r < n; r++)
-        for (int64_t p = US[r] + 1; p < US[r + 1]; p++) counts[UI[p]]++;
-    for (int64_t r = 0; r < n; r++) lower.ptr[r + 1] = lower.ptr[r] + counts[r];
+    // entries per L row (U's column counts), over the same row ranges
+    lower.ptr[0] = 0;
+    parallel_chunks(n, nt, [&](int64_t c0, int64_t c1, int) {
+        std::fill(lower.ptr.begin() + c0 + 1, lower.ptr.begin() + c1 + 1, 0);
+        for (int64_t r = std::max<int64_t>(0, c0 - band); r < c1; r++) {
+            const int64_t* b = UI.data() + US[r] + 1;
+            const int64_t* e = UI.data() + US[r + 1];
+            for (const int64_t* q = std::lower_bound(b, e, c0); q < e && *q < c1; q++) lower.ptr[*q + 1]++;
+        }
+    });
+    for (int64_t r = 0; r < n; r++) lower.ptr[r + 1] += lower.ptr[r];
     lower.col.resize(size_t(lower.ptr[n]));
     lower.val_pos.resize(size_t(lower.ptr[n]));
     parallel_chunks(n, nt, [&](int64_t c0, int64_t c1, int) {
         std::vector<int64_t> fill(lower.ptr.begin() + c0, lower.ptr.begin() + c1);
         for (int64_t r = std::max<int64_t>(0, c0 - band); r < c1; r++) {
-            if (r >= c0) lower.diag_pos[r] = US[r];
+            if (r >= c0) lower.diag_pos[r] = int32_t(US[r]);
             // entries of U row r with a column in [c0, c1) (columns ascend)
             const int64_t* b = UI.data() + US[r] + 1;
             const int64_t* e = UI.data() + US[r + 1];
             for (const int64_t* q = std::lower_bound(b, e, c0); q < e && *q < c1; q++) {
                 const int64_t d = fill[*q - c0]++;
-                lower.col[d] = r;
-                lower.val_pos[d] = q - UI.data();
+                lower.col[d] = int32_t(r);
+                lower.val_pos[d] = int32_t(q - UI.data());
             }
         }
     });
@@ -932,34 +1044,65 @@ bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64
 }
 
 bool ilu0_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper,
-               std::vector<int64_t>& diag_pos)
+               PlanVec64& diag_pos)
 {
+    // L: the strictly lower entries of each row, U: the strictly upper ones
+    // (diagonal apart); counted, then filled, over the planning threads
+    check_32bit(n, n ? S[n] : 0);
     lower.n = upper.n = n;
     lower.lower = true;
     upper.lower = false;
-    lower.ptr.assign(n + 1, 0);
-    upper.ptr.assign(n + 1, 0);
-    lower.col.clear(); lower.val_pos.clear();
-    upper.col.clear(); upper.val_pos.clear();
+    lower.ptr.resize(n + 1);
+    upper.ptr.resize(n + 1);
     lower.diag_pos.assign(n, -1);
-    upper.diag_pos.assign(n, -1);
-    diag_pos.assign(n, -1);
-    for (int64_t r = 0; r < n; r++) {
-        for (int64_t p = S[r]; p < S[r + 1]; p++) {
-            if (I[p] < r) { lower.col.push_back(I[p]); lower.val_pos.push_back(p); }
-            else if (I[p] == r) diag_pos[r] = p;
-            else { upper.col.push_back(I[p]); upper.val_pos.push_back(p); }
+    upper.diag_pos.resize(n);
+    diag_pos.resize(n);
+    lower.ptr[0] = upper.ptr[0] = 0;
+    const int nt = n >= 65536 ? plan_threads() : 1;
+    int bad = 0;
+    parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+        for (int64_t r = r0; r < r1; r++) {
+            const int64_t* b = I + S[r];
+            const int64_t* e = I + S[r + 1];
+            const int64_t* f = std::lower_bound(b, e, r);   // columns ascend
+            if (f == e || *f != r) {
+                __atomic_store_n(&bad, 1, __ATOMIC_RELAXED);
+                return;
+            }
+            diag_pos[r] = f - I;
+            upper.diag_pos[r] = int32_t(f - I);
+            lower.ptr[r + 1] = f - b;
+            upper.ptr[r + 1] = e - f - 1;
         }
-        if (diag_pos[r] < 0) return false;
-        upper.diag_pos[r] = diag_pos[r];
-        lower.ptr[r + 1] = int64_t(lower.col.size());
-        upper.ptr[r + 1] = int64_t(upper.col.size());
+    });
+    if (bad) return false;
+    for (int64_t r = 0; r < n; r++) {
+        lower.ptr[r + 1] += lower.ptr[r];
+        upper.ptr[r + 1] += upper.ptr[r];
     }
+    lower.col.resize(size_t(lower.ptr[n]));
+    lower.val_pos.resize(size_t(lower.ptr[n]));
+    upper.col.resize(size_t(upper.ptr[n]));
+    upper.val_pos.resize(size_t(upper.ptr[n]));
+    parallel_chunks(n, nt, [&](int64_t r0, int64_t r1, int) {
+        for (int64_t r = r0; r < r1; r++) {
+            int64_t lq = lower.ptr[r], uq = upper.ptr[r];
+            for (int64_t p = S[r]; p < diag_pos[r]; p++, lq++) {
+                lower.col[lq] = int32_t(I[p]);
+                lower.val_pos[lq] = int32_t(p);
+            }
+            for (int64_t p = diag_pos[r] + 1; p < S[r + 1]; p++, uq++) {
+                upper.col[uq] = int32_t(I[p]);
+                upper.val_pos[uq] = int32_t(p);
+            }
+        }
+    });
     return true;
 }
 
 bool ilu0_adjoint_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper)
 {
+    check_32bit(n, n ? S[n] : 0);
     std::vector<int64_t> dpos(n, -1);
     lower.n = upper.n = n;
     lower.lower = true;

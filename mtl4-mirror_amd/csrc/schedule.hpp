@@ -9,6 +9,13 @@
 
 namespace m4g {
 
+// Large host blocks (at least kLargeBlock bytes) on 2 MiB boundaries with
+// transparent huge pages requested: the planning arrays are touched once,
+// and first-touch faults of 4 KiB pages cost more than filling them.
+constexpr size_t kLargeBlock = size_t(4) << 20;
+void* large_block_alloc(size_t bytes);
+void large_block_free(void* p) noexcept;
+
 // Allocator whose value-initialisation leaves trivial elements uninitialised:
 // the plan's large arrays are written in parallel, and each page is first
 // touched by the thread that fills it.
@@ -21,6 +28,16 @@ struct NoInitAlloc : std::allocator<T> {
     NoInitAlloc() = default;
     template <typename U>
     NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    T* allocate(size_t n)
+    {
+        if (n * sizeof(T) < kLargeBlock) return std::allocator<T>::allocate(n);
+        return static_cast<T*>(large_block_alloc(n * sizeof(T)));
+    }
+    void deallocate(T* p, size_t n) noexcept
+    {
+        if (n * sizeof(T) < kLargeBlock) std::allocator<T>::deallocate(p, n);
+        else large_block_free(p);
+    }
     template <typename U>
     void construct(U* p) noexcept
     {
@@ -33,6 +50,7 @@ struct NoInitAlloc : std::allocator<T> {
     }
 };
 using PlanVec32 = std::vector<int32_t, NoInitAlloc<int32_t>>;
+using PlanVec64 = std::vector<int64_t, NoInitAlloc<int64_t>>;
 
 // Natural (lexicographic) grid inferred from a matrix pattern: row
 // r = i + nx*(j + ny*k) and every stored entry couples cells at Chebyshev
@@ -44,13 +62,15 @@ GridInfo infer_grid(int64_t n, const int64_t* starts, const int64_t* indices);
 
 // One row of a sweep: entries in the order they are subtracted, with the
 // factor-storage index of each value and of the diagonal (-1: unit diagonal).
+// Rows and factor-storage indices are 32-bit: the builders below reject a
+// matrix of 2^31 or more rows or stored entries.
 struct SweepRows {
     int64_t n = 0;
     bool lower = true;                    // dependencies point to smaller rows
-    std::vector<int64_t> ptr;             // [n+1]
-    std::vector<int64_t> col;             // dependency row of each entry
-    std::vector<int64_t> val_pos;         // factor storage index of each entry
-    std::vector<int64_t> diag_pos;        // [n] factor storage index of the diagonal, -1 = unit
+    PlanVec64 ptr;                        // [n+1]
+    PlanVec32 col;                        // dependency row of each entry
+    PlanVec32 val_pos;                    // factor storage index of each entry
+    PlanVec32 diag_pos;                   // [n] factor storage index of the diagonal, -1 = unit
 };
 
 // Tile shape in the skewed grid coordinates t1 = k, t2 = j + a*k,
@@ -79,8 +99,8 @@ struct SweepPlan {
     int64_t max_tile_rows = 0, max_steps = 0, levels = 0;
     TileShape shape;
     std::vector<int32_t> tile_row_begin;  // [ntiles+1] into rows
-    std::vector<int32_t> rows;            // [n] rows of each tile, in natural order
-    std::vector<int32_t> row_slot;        // [n] LDS slot of each entry of rows
+    PlanVec32 rows;                       // [n] rows of each tile, in natural order
+    PlanVec32 row_slot;                   // [n] LDS slot of each entry of rows
     std::vector<int32_t> tile_group_begin;// [ntiles+1] into groups
     std::vector<int32_t> tile_export_begin;// [ntiles+1] export slots owned by each tile
     std::vector<int32_t> xlist;           // [nexport] LDS slot of the row behind each export slot
@@ -88,9 +108,9 @@ struct SweepPlan {
     int64_t max_tile_exports = 0;
     std::vector<int32_t> group_piece_begin;// [ngroups+1] into the piece arrays
     std::vector<int32_t> group_need;      // [ngroups] imports of the tile needed before the group runs
-    std::vector<int32_t> pc_slot;         // [npieces] LDS slot of the piece's row in its tile
-    std::vector<int32_t> pc_len;          // [npieces] entries of the piece (<= width)
-    std::vector<int32_t> pc_export;       // [npieces] export slot (last piece of an exported row) or -1
+    PlanVec32 pc_slot;                    // [npieces] LDS slot of the piece's row in its tile
+    PlanVec32 pc_len;                     // [npieces] entries of the piece (<= width)
+    PlanVec32 pc_export;                  // [npieces] export slot (last piece of an exported row) or -1
     PlanVec32 pc_diag_pos;                // [npieces] factor index of the diagonal; -1: none
     PlanVec32 pc_src;                     // [npieces*width] LDS index of each entry's value:
                                           //   slot (< max_tile_rows) or max_tile_rows + tile-local import
@@ -142,13 +162,12 @@ namespace m4g {
 // int64 indices; *_pos index the factor storage:
 //  - IC(0): storage = upper(A) in CSR (inverted diagonal first per row);
 //  - ILU(0): storage = A's CSR (L strictly lower, U upper, diagonal inverted).
-// All return false on a missing diagonal.
-bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, std::vector<int64_t>& US,
-                   std::vector<int64_t>& UI, std::vector<int64_t>& a_pos);
-bool ic0_rows(int64_t n, const std::vector<int64_t>& US, const std::vector<int64_t>& UI, SweepRows& lower,
-              SweepRows& upper);
+// All return false on a missing diagonal, and throw std::length_error for a
+// matrix of 2^31 or more rows or stored entries.
+bool upper_pattern(int64_t n, const int64_t* S, const int64_t* I, PlanVec64& US, PlanVec64& UI, PlanVec32& a_pos);
+bool ic0_rows(int64_t n, const PlanVec64& US, const PlanVec64& UI, SweepRows& lower, SweepRows& upper);
 bool ilu0_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper,
-               std::vector<int64_t>& diag_pos);
+               PlanVec64& diag_pos);
 // Sweeps of ilu::adjoint_solve (ilu.hpp:105-115): lower on U^T (inverse
 // diagonal, ascending rows) then unit upper on L^T (descending rows).
 bool ilu0_adjoint_rows(int64_t n, const int64_t* S, const int64_t* I, SweepRows& lower, SweepRows& upper);
