@@ -96,6 +96,72 @@ __global__ __launch_bounds__(256) void k_ilu0_level(int64_t count, const int32_t
     LU[dp] = inv;
 }
 
+// The same row sequence with one wavefront per row (rows of at most 64
+// entries): lane q holds entry q of row i in a register.  For each lower entry
+// k (ascending) the lane of k scales it, and every later lane subtracts a_ik
+// times its column's entry in row k's U part (0 where row k has none), found
+// among that U part spread over the lanes; the next k's U part is loaded while
+// the current one is applied.  Each entry sees the same operations in the
+// same order as in k_ilu0_level.
+__global__ __launch_bounds__(256) void k_ilu0_level_wave(int64_t count, const int32_t* __restrict__ level_rows,
+                                                         const int32_t* __restrict__ S,
+                                                         const int32_t* __restrict__ I,
+                                                         const int64_t* __restrict__ diag_pos,
+                                                         double* __restrict__ LU, double* __restrict__ invdiag)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t t = blockIdx.x * int64_t(4) + (threadIdx.x >> 6);
+    if (t >= count) return;   // the whole wavefront
+    const int32_t i = level_rows[t];
+    const int32_t s0 = S[i], len = S[i + 1] - s0;
+    const int32_t dl = int32_t(diag_pos[i]) - s0;   // lower entries before the diagonal
+    const bool mine = lane < len;
+    const int32_t col = mine ? I[s0 + lane] : -1;
+    double v = mine ? LU[s0 + lane] : 0.0;
+    // lane q < dl: bounds of row col_q's U part and its inverted diagonal
+    int32_t kb = 0, ke = 0;
+    double kinv = 0.0;
+    if (lane < dl) {
+        kb = int32_t(diag_pos[col]) + 1;
+        ke = S[col + 1];
+        kinv = invdiag[col];
+    }
+    int32_t cn = -1;
+    double un = 0.0;
+    int mn = 0;
+    auto load_u = [&](int kk) {
+        const int32_t b = __shfl(kb, kk, 64), e = __shfl(ke, kk, 64);
+        mn = e - b;
+        cn = lane < mn ? I[b + lane] : -1;
+        un = lane < mn ? LU[b + lane] : 0.0;
+    };
+    if (dl > 0) load_u(0);
+    for (int kk = 0; kk < dl; kk++) {
+        const int32_t ck = cn;
+        const double uk = un;
+        const int m_n = mn;
+        if (kk + 1 < dl) load_u(kk + 1);
+        const double kinv_k = __shfl(kinv, kk, 64);
+        if (lane == kk) v = v * kinv_k;                         // ilu_0.hpp:70
+        const double aik = __shfl(v, kk, 64);
+        double lukj = 0.0;
+        for (int m = 0; m < m_n; m++) {
+            const int32_t cm = __shfl(ck, m, 64);
+            const double um = __shfl(uk, m, 64);
+            if (cm == col) lukj = um;
+        }
+        if (lane > kk && mine) v = v - aik * lukj;             // ilu_0.hpp:74
+    }
+    if (mine) {
+        if (lane == dl) {
+            const double inv = 1.0 / v;                         // ilu_0.hpp:77 and invert_diagonal
+            invdiag[i] = inv;
+            v = inv;
+        }
+        LU[s0 + lane] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // sweep kernel
 // ---------------------------------------------------------------------------
@@ -1046,10 +1112,18 @@ static void factorize_levels(m4g_pc_s& P, const SweepRows& lower_dag, const m4g_
     } else {
         DevBuf<int64_t> d_diag(std::max<int64_t>(P.n, 1));
         d_diag.upload(P.diag_pos.data(), P.n);
+        int64_t max_len = 0;
+        for (int64_t r = 0; r < P.n; r++) max_len = std::max(max_len, A.h_starts[r + 1] - A.h_starts[r]);
         for (int64_t l = 0; l < nlev; l++) {
             int64_t cnt = lptr[l + 1] - lptr[l];
-            hipLaunchKernelGGL(k_ilu0_level, dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, d_rows.ptr + lptr[l],
-                               A.starts.ptr, A.indices.ptr, d_diag.ptr, P.fstore.ptr, invdiag.ptr);
+            if (max_len <= 64)   // a wavefront per row
+                hipLaunchKernelGGL(k_ilu0_level_wave, dim3((cnt + 3) / 4), dim3(256), 0, s, cnt,
+                                   d_rows.ptr + lptr[l], A.starts.ptr, A.indices.ptr, d_diag.ptr, P.fstore.ptr,
+                                   invdiag.ptr);
+            else
+                hipLaunchKernelGGL(k_ilu0_level, dim3((cnt + 255) / 256), dim3(256), 0, s, cnt,
+                                   d_rows.ptr + lptr[l], A.starts.ptr, A.indices.ptr, d_diag.ptr, P.fstore.ptr,
+                                   invdiag.ptr);
         }
         M4G_HIP(hipGetLastError());
         M4G_HIP(hipStreamSynchronize(s));

@@ -297,6 +297,30 @@ def test_ilu0_factor_and_solves(g, name):
     assert relerr(yv.numpy(), o.ilu0_solve(L, U, x)) <= 1e-12
 
 
+@pytest.mark.parametrize("n,extra,long_rows", [(240, 12, False), (300, 45, False), (300, 70, True)])
+def test_ilu0_factor_general_patterns(g, n, extra, long_rows):
+    # Random patterns as T-itl/ilu_0_test.cpp builds them: rows of up to 64
+    # entries take the wavefront-per-row factorisation, longer ones the
+    # thread-per-row one; both apply the same operations in the same order
+    # as the oracle (ilu_0.hpp:63-78), so the factors match it exactly.
+    from test_oracle import _rand_sparse
+    say(f"ilu0 random n={n} extra={extra}")
+    A, _ = _rand_sparse(n, seed=n + extra, extra=extra)
+    assert (max(np.diff(A.starts)) > 64) == long_rows
+    st, L, U = o.ilu0_factor(A)
+    assert st == o.OK
+    P = g.PC(g.Matrix(A.starts, A.indices, A.data), g.PC_ILU0)
+    for which, ref in ((0, L), (1, U)):
+        S, I, V = P.factor(which)
+        np.testing.assert_array_equal(S, ref.starts)
+        np.testing.assert_array_equal(I, ref.indices)
+        np.testing.assert_array_equal(V, ref.data)
+    x = g.rng_uniform(8, 1, A.n)
+    xv, yv = g.Vector(data=x), g.Vector(A.n)
+    P.solve(xv, yv)
+    np.testing.assert_array_equal(yv.numpy(), o.ilu0_solve(L, U, x))
+
+
 def test_ic0_golden_float_factors(g):
     # T-itl/ic_0_test.cpp:22-36 on the GPU: ic_0<compressed2D<double>, float>
     S, I, V = g.laplacian2d(3, 3)
