@@ -173,14 +173,6 @@ __device__ inline void rhs_sums(double v, double* red, int R)
 
 typedef __attribute__((address_space(3))) double lds_double;   // LDS-qualified, so accesses are ds_* ops
 
-// Barrier for values exchanged through LDS only: waits for this wave's LDS
-// writes, not for its outstanding global loads and stores.
-__device__ inline void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // The inputs of one (position, rhs) pair of a sweep that do not depend on
 // earlier levels: the row, its dependencies (positions in the sweep's level

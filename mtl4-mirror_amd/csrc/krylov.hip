@@ -285,6 +285,264 @@ __global__ __launch_bounds__(kNT) void k_cg_update(int64_t n, const T* __restric
 }
 
 // ---------------------------------------------------------------------------
+// IC(0)-PCG of a small 2D grid system in one workgroup
+// ---------------------------------------------------------------------------
+// For a system of at most kGridMaxN rows on an nx x ny grid (ny <= 64) whose
+// IC(0) factor has the 5-point pattern, a whole batch of CG iterations runs
+// in one workgroup with no launch between the steps.  The factor (U's east
+// and north entries and the inverted diagonal) and the sweep vector stay in
+// LDS; x, r and p stay in registers (rows tid, tid + 512, ...).  One
+// wavefront runs both sweeps with lane j on grid line j: at step s it
+// finishes cell i = s - j, whose two dependencies are its own previous result
+// (the cell before it on the line) and lane j-1's previous one, passed by a
+// DPP lane shift -- a step is the dependent arithmetic of the row and no
+// memory round trip.  Each row applies its entries in the order of the
+// multi-kernel sweeps (L = trans(U): north then west, times the inverted
+// diagonal; U: east then north), absent boundary entries as 0 * (+0) with
+// both factors +0, which leaves every value unchanged.  Scalars, convergence
+// test and history follow the multi-kernel CG (cg.hpp:75-112); only the
+// order in which the dots are summed differs.
+constexpr int kGridNT = 512;
+constexpr int kGridRows = 8;   // rows per thread
+constexpr int64_t kGridMaxN = int64_t(kGridNT) * kGridRows;
+
+struct GridCgArgs {
+    int32_t n, nx, ny, iters;
+    const int32_t* S;      // CSR of A, whose rows have the 5-point pattern
+    const double* A;
+    const double* east;    // [n] U[r][r+1], 0 where absent
+    const double* north;   // [n] U[r][r+nx], 0 where absent
+    const double* dinv;    // [n] inverted diagonal of U
+    double *x, *r, *p;
+    KState* st;
+};
+
+// LDS doubles of the kernel: sweep vector, east (one leading 0), north (nx
+// leading 0s), inverted diagonal, one dump slot per lane.
+inline int64_t grid_cg_lds_doubles(int64_t n, int64_t nx) { return 4 * n + nx + 1 + kWave; }
+
+// Lane l receives lane l-1's value (lane 0: +0); lane l receives lane l+1's
+// (lane 63: +0).  DPP wave_shr:1 / wave_shl:1 on the two halves.
+__device__ inline double lane_from_prev(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x138, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ inline double lane_from_next(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x130, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Both sweeps in place over V, lane j on grid line j (j < ny), step s on cell
+// i = s - j.  Lower sweep on L = trans(U) (lower_trisolve.hpp:191-209):
+//   t_r = ((v_r - U[r-nx][r] t_{r-nx}) - U[r-1][r] t_{r-1}) * dinv_r,
+// the first operand from lane j-1, the second the lane's own last result.
+// Upper sweep on U, steps backwards (upper_trisolve.hpp:159-176):
+//   z_r = ((t_r - U[r][r+1] z_{r+1}) - U[r][r+nx] z_{r+nx}) * dinv_r.
+// Eight steps a turn: the operands of the turn's eight steps are loaded
+// together, the eight steps run on registers only (the dependent chain is the
+// row's arithmetic and one lane shift), and their results are stored
+// together.  A lane runs every step: before its line's first cell (and on a
+// lane past the last line) its operands are all +0, so its result stays +0
+// -- exactly what a missing neighbour contributes -- and past its line's
+// last cell it computes on clamped operands a value no lane reads again.
+// Only the steps on the line's cells store to V; the others store into the
+// lane's dump slot.  The step count is rounded up to a multiple of eight.
+constexpr int kSweepTurn = 8;
+
+template <bool LOWER>
+__device__ inline void grid_sweep(double* V, const double* E, const double* N, const double* D, double* dump, int nx,
+                                  int ny, int j)
+{
+    const int steps = nx + ny - 1;
+    const bool line = j < ny;
+    const int base = nx * (line ? j : 0);
+    const int ds = LOWER ? 1 : -1;
+    double y = 0.0;   // the lane's last result
+    for (int s0 = LOWER ? 0 : steps - 1, k = 0; k < steps; k += kSweepTurn, s0 += kSweepTurn * ds) {
+        double v[kSweepTurn], f1[kSweepTurn], f2[kSweepTurn], d[kSweepTurn];
+        double* at[kSweepTurn];
+#pragma unroll
+        for (int u = 0; u < kSweepTurn; u++) {
+            const int i = s0 + u * ds - j;
+            const int r = (i < 0 ? 0 : i >= nx ? nx - 1 : i) + base;   // clamped: a valid cell
+            const bool before = !line || (LOWER ? i < 0 : i >= nx);
+            v[u] = before ? 0.0 : V[r];
+            f1[u] = before ? 0.0 : LOWER ? N[r] : E[1 + r];    // U[r-nx][r] / U[r][r+1]
+            f2[u] = before ? 0.0 : LOWER ? E[r] : N[nx + r];   // U[r-1][r]  / U[r][r+nx]
+            d[u] = before ? 0.0 : D[r];
+            at[u] = line && i >= 0 && i < nx ? V + r : dump + j;
+        }
+        __builtin_amdgcn_sched_barrier(0);   // all loads first: one wait per turn
+        double out[kSweepTurn];
+#pragma unroll
+        for (int u = 0; u < kSweepTurn; u++) {
+            const double nb = LOWER ? lane_from_prev(y) : lane_from_next(y);
+            double acc = LOWER ? v[u] - f1[u] * nb : v[u] - f1[u] * y;
+            acc = LOWER ? acc - f2[u] * y : acc - f2[u] * nb;
+            y = acc * d[u];
+            out[u] = y;
+        }
+        __builtin_amdgcn_sched_barrier(0);   // the stores after the chain
+#pragma unroll
+        for (int u = 0; u < kSweepTurn; u++) *at[u] = out[u];
+    }
+}
+
+// Sum over the workgroup in a fixed order; every thread gets the total.
+__device__ inline double grid_cg_sum(double v, double* red)
+{
+    v = wave_sum(v);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) red[wave] = v;
+    lds_barrier();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kGridNT / kWave; w++) t += red[w];
+    lds_barrier();   // red is reused by the next sum
+    return t;
+}
+
+__global__ __launch_bounds__(kGridNT) void k_cg_ic0_grid(GridCgArgs a)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[kGridNT / kWave];
+    __shared__ KState ks;
+    const int n = a.n, nx = a.nx, tid = threadIdx.x;
+    double* V = lds;
+    double* E = V + n;        // E[1 + r] = U[r][r+1]
+    double* N = E + n + 1;    // N[nx + r] = U[r][r+nx]
+    double* D = N + n + nx;
+    double* dump = D + n;
+    for (int k = tid; k < n; k += kGridNT) {
+        E[1 + k] = a.east[k];
+        N[nx + k] = a.north[k];
+        D[k] = a.dinv[k];
+    }
+    if (tid == 0) E[0] = 0.0;
+    for (int k = tid; k < nx; k += kGridNT) N[k] = 0.0;
+    if (tid == 0) ks = *a.st;
+    double x[kGridRows], r[kGridRows], p[kGridRows], q[kGridRows];
+    int gi[kGridRows], gj[kGridRows], ge[kGridRows];   // grid position and first entry of each row
+#pragma unroll
+    for (int u = 0; u < kGridRows; u++) {
+        const int row = tid + u * kGridNT;
+        const bool in = row < n;
+        x[u] = in ? a.x[row] : 0.0;
+        r[u] = in ? a.r[row] : 0.0;
+        p[u] = in ? a.p[row] : 0.0;
+        gi[u] = in ? row % nx : 0;
+        gj[u] = in ? row / nx : 0;
+        ge[u] = in ? a.S[row] : 0;
+    }
+    lds_barrier();
+    for (int it = 0; it < a.iters && !ks.done; it++) {
+        // z = solve(P, r)   (cg.hpp:97)
+#pragma unroll
+        for (int u = 0; u < kGridRows; u++)
+            if (tid + u * kGridNT < n) V[tid + u * kGridNT] = r[u];
+        lds_barrier();
+        if (tid < kWave) {
+            grid_sweep<true>(V, E, N, D, dump, nx, a.ny, tid);
+            grid_sweep<false>(V, E, N, D, dump, nx, a.ny, tid);
+        }
+        lds_barrier();
+        // A's entries for q = A p, loaded now so they arrive while rho and p
+        // are formed: south (r-nx), west, centre, east, north (r+nx), 0 where
+        // the grid has no such neighbour
+        double cs[kGridRows], cw[kGridRows], cc[kGridRows], ce[kGridRows], cn[kGridRows];
+#pragma unroll
+        for (int u = 0; u < kGridRows; u++) {
+            const int row = tid + u * kGridNT;
+            const bool in = row < n, hs = in && gj[u] > 0, hw = in && gi[u] > 0, he = in && gi[u] < nx - 1,
+                       hn = in && gj[u] < a.ny - 1;
+            const int es = ge[u], ew = es + hs, ec = ew + hw, ee = ec + 1, en = ee + he;
+            cs[u] = hs ? a.A[es] : 0.0;
+            cw[u] = hw ? a.A[ew] : 0.0;
+            cc[u] = in ? a.A[ec] : 0.0;
+            ce[u] = he ? a.A[ee] : 0.0;
+            cn[u] = hn ? a.A[en] : 0.0;
+        }
+        // rho = dot(r, z)
+        double acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < kGridRows; u++)
+            if (tid + u * kGridNT < n) acc += r[u] * V[tid + u * kGridNT];
+        const double rho = grid_cg_sum(acc, red);
+        // p = z (first) or z + (rho / rho_1) p   (cg.hpp:99-102)
+        const bool first = ks.i <= 1;
+        const double beta = first ? 0.0 : rho / ks.rho_1;
+#pragma unroll
+        for (int u = 0; u < kGridRows; u++) {
+            const int row = tid + u * kGridNT;
+            if (row < n) {
+                const double z = V[row];
+                p[u] = first ? z : z + beta * p[u];
+                V[row] = p[u];
+            }
+        }
+        lds_barrier();
+        // q = A p; alpha = rho / dot(p, q)   (cg.hpp:104-105)
+        acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < kGridRows; u++) {
+            const int row = tid + u * kGridNT;
+            q[u] = 0.0;
+            if (row < n) {   // the row's entries in stored order (mat_vec_mult.hpp:662-687)
+                double sum = 0.0;
+                if (gj[u] > 0) sum += cs[u] * V[row - nx];
+                if (gi[u] > 0) sum += cw[u] * V[row - 1];
+                sum += cc[u] * V[row];
+                if (gi[u] < nx - 1) sum += ce[u] * V[row + 1];
+                if (gj[u] < a.ny - 1) sum += cn[u] * V[row + nx];
+                q[u] = sum;
+                acc += p[u] * sum;
+            }
+        }
+        const double alpha_1 = grid_cg_sum(acc, red);
+        const double alpha = rho / alpha_1;
+        // x += alpha p; r -= alpha q; rr = dot(r, r)   (cg.hpp:107-109)
+        acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < kGridRows; u++)
+            if (tid + u * kGridNT < n) {
+                x[u] = x[u] + alpha * p[u];
+                const double ri = r[u] - alpha * q[u];
+                r[u] = ri;
+                acc += ri * ri;
+            }
+        const double rr = grid_cg_sum(acc, red);
+        if (tid == 0) {   // on a private copy: the shared one is written with LDS stores only
+            KState t = ks;
+            t.rho = rho;
+            t.alpha_1 = alpha_1;
+            t.alpha = alpha;
+            t.rho_1 = rho;
+            t.rr = rr;
+            top_check(&t, sqrt(fabs(rr)));   // finished(r) or ++iter   (cg.hpp:95)
+            ks = t;
+        }
+        lds_barrier();
+    }
+#pragma unroll
+    for (int u = 0; u < kGridRows; u++) {
+        const int row = tid + u * kGridNT;
+        if (row < n) {
+            a.x[row] = x[u];
+            a.r[row] = r[u];
+            a.p[row] = p[u];
+        }
+    }
+    if (tid == 0) *a.st = ks;
+}
+
+// ---------------------------------------------------------------------------
 // BiCGStab kernels (bicgstab.hpp:40-76)
 // ---------------------------------------------------------------------------
 // Loop-top: finished(two_norm(r)); ++iter; rho_1 = dot(rtilde, r) with its
@@ -539,6 +797,34 @@ static void spmv_geom(const m4g_matrix_s& A, int64_t& chunk, int64_t& grid)
     grid = std::max<int64_t>(1, chunk * kXcds);
 }
 
+// True when every row r = i + nx*j of A holds exactly its 5-point neighbours on
+// the (2D) grid g, columns ascending: r-nx, r-1, r, r+1, r+nx where inside.
+static bool grid5_pattern(const m4g_matrix_s& A, const GridInfo& g)
+{
+    const int64_t n = A.nrows, nx = g.nx, ny = g.ny;
+    if (nx <= 0 || g.nz != 1 || nx * ny != n || n > kGridMaxN) return false;
+    for (int64_t r = 0; r < n; r++) {
+        const int64_t i = r % nx, j = r / nx;
+        int64_t e = A.h_starts[r];
+        const int64_t want[5] = {j > 0 ? r - nx : -1, i > 0 ? r - 1 : -1, r, i < nx - 1 ? r + 1 : -1,
+                                 j < ny - 1 ? r + nx : -1};
+        for (int64_t c : want)
+            if (c >= 0 && (e >= A.h_starts[r + 1] || A.h_indices[e++] != c)) return false;
+        if (e != A.h_starts[r + 1]) return false;
+    }
+    return true;
+}
+
+// Dynamic LDS a workgroup may request on this device.
+static size_t grid_cg_lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 65536;
+    return size_t(std::min(v, 160 * 1024)) - 1024;   // the kernel's static shared data
+}
+
 static int batch_size()
 {
     static int b = [] {
@@ -561,16 +847,17 @@ static void init_state(KState& h, const m4g_iteration* it, double* hist_dev)
     h.history_cap = it->history ? it->history_capacity : 0;
 }
 
-// Runs `body(k)` for batches of iterations until the device state reports
-// finished; polls the flag of batch b while batch b+1 is queued.
+// Runs `body()` in batches (calls_per_poll calls, default one per iteration
+// of a batch) until the device state reports finished; polls the flag of
+// batch b while batch b+1 is queued.
 template <typename Body>
-static void run_batches(KrylovWs& ws, const m4g_iteration* it, Body&& body)
+static void run_batches(KrylovWs& ws, const m4g_iteration* it, Body&& body, int calls_per_poll = 0)
 {
     hipStream_t s = current_stream();
     hipEvent_t ev[2];
     M4G_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     M4G_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    const int B = batch_size();
+    const int B = calls_per_poll > 0 ? calls_per_poll : batch_size();
     int slot = 0, pending = -1;
     int64_t launched = 0;
     bool finished = false;
@@ -690,7 +977,34 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
         FIN(1, FIN_CG_UPDATE, rg);
         M4G_HIP(hipGetLastError());
     };
-    run_batches(ws, it, body);
+    // a small 2D grid with an FP64 IC(0) factor: whole batches of iterations
+    // in one workgroup (k_cg_ic0_grid)
+    bool grid = false;
+    if constexpr (sizeof(T) == 8) {
+        DevBuf<double> east, north, dinv;
+        const char* env = std::getenv("M4G_GRID_CG");   // M4G_GRID_CG=0: the multi-kernel path
+        const bool enabled = !(env && std::atoi(env) == 0);
+        const size_t lds = P && enabled ? size_t(grid_cg_lds_doubles(n, P->grid.nx)) * sizeof(double) : 0;
+        if (P && enabled && lds <= grid_cg_lds_limit() && grid5_pattern(A, P->grid) &&
+            pc_grid_factor(*P, kGridMaxN, east, north, dinv, st)) {
+            grid = true;
+            static size_t granted = 0;
+            if (lds > 65536 && lds > granted) {
+                M4G_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cg_ic0_grid),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+                granted = lds;
+            }
+            // four batches' worth of iterations per launch (the launch and the
+            // LDS fill cost more than an iteration here)
+            GridCgArgs ga{int32_t(n), int32_t(P->grid.nx), int32_t(P->grid.ny), 4 * batch_size(), A.starts.ptr,
+                          A.values.ptr, east.ptr, north.ptr, dinv.ptr, (double*)x, (double*)r.ptr, (double*)p.ptr, S};
+            run_batches(ws, it, [&] {
+                hipLaunchKernelGGL(k_cg_ic0_grid, dim3(1), dim3(kGridNT), lds, st, ga);
+                M4G_HIP(hipGetLastError());
+            }, 1);
+        }
+    }
+    if (!grid) run_batches(ws, it, body);
 #undef FIN
     int fault = 0;
     if (P) fault = check_pc_faults(*P);

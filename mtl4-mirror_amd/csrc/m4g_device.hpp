@@ -9,6 +9,15 @@ namespace m4g {
 constexpr int kWave = 64;
 constexpr int kXcds = 8;    // MI355X: workgroup i runs on XCD i % 8
 
+// Barrier for values exchanged through LDS only: waits for this wave's LDS
+// writes, not for its outstanding global loads and stores.
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.
 // The combination order is fixed, so results are deterministic.
 __device__ inline double wave_sum(double v)

@@ -654,6 +654,18 @@ __global__ __launch_bounds__(64) void k_build_recs(const int4* __restrict__ grou
     for (int c = 0; c < C; c++) recs[int64_t(first) * C + int64_t(c) * m + q] = u.c[c];
 }
 
+// out_k[r] = f[pos[3r + k]] (0 where the position is -1), k = 0, 1, 2
+__global__ void k_gather_strided(int64_t n, const int32_t* __restrict__ pos, const double* __restrict__ f,
+                                 double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2)
+{
+    for (int64_t r = blockIdx.x * int64_t(256) + threadIdx.x; r < n; r += int64_t(gridDim.x) * 256) {
+        const int32_t a = pos[3 * r], b = pos[3 * r + 1], c = pos[3 * r + 2];
+        o0[r] = a >= 0 ? f[a] : 0.0;
+        o1[r] = b >= 0 ? f[b] : 0.0;
+        o2[r] = c >= 0 ? f[c] : 0.0;
+    }
+}
+
 template <typename T, typename IT>
 __global__ void k_gather(int64_t count, const IT* __restrict__ pos, const double* __restrict__ f,
                          T* __restrict__ out, T absent)
@@ -910,6 +922,52 @@ void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp,
         launch_sweep_t<float, float>(first, second, (const float*)v, (float*)tmp, skip, s);
         launch_sweep_t<float, float>(second, first, (const float*)tmp, (float*)y, skip, s);
     }
+}
+
+bool pc_grid_factor(m4g_pc_s& P, int64_t max_n, DevBuf<double>& east, DevBuf<double>& north, DevBuf<double>& dinv,
+                    hipStream_t s)
+{
+    const int64_t n = P.n, nx = P.grid.nx, ny = P.grid.ny;
+    if (P.grid5 == 0) {
+        P.grid5 = -1;
+        if (P.kind == M4G_PC_IC0 && P.factor_type == M4G_F64 && n > 0 && n <= max_n && nx > 0 && P.grid.nz == 1 &&
+            ny <= 64) {
+            // row r = i + nx*j of U holds r, then r+1 (i < nx-1), then r+nx (j < ny-1)
+            std::vector<int32_t> pos(size_t(3 * n));
+            bool ok = true;
+            for (int64_t r = 0; r < n && ok; r++) {
+                const int64_t i = r % nx, j = r / nx;
+                int64_t q = P.fS[r];
+                const int64_t len = 1 + (i < nx - 1) + (j < ny - 1);
+                ok = P.fS[r + 1] - q == len && P.fI[q] == r;
+                pos[3 * r] = int32_t(q++);
+                pos[3 * r + 1] = pos[3 * r + 2] = -1;
+                if (ok && i < nx - 1) {
+                    ok = P.fI[q] == r + 1;
+                    pos[3 * r + 1] = int32_t(q++);
+                }
+                if (ok && j < ny - 1) {
+                    ok = P.fI[q] == r + nx;
+                    pos[3 * r + 2] = int32_t(q);
+                }
+            }
+            if (ok) {
+                P.grid5_pos.reset(pos.size());
+                P.grid5_pos.upload(pos.data(), pos.size());
+                P.grid5 = 1;
+            }
+        }
+    }
+    if (P.grid5 != 1) return false;
+    east.reset(size_t(n));
+    north.reset(size_t(n));
+    dinv.reset(size_t(n));
+    // strided gathers of the three positions of each row (values as the
+    // factorisation left them: the diagonal already inverted)
+    hipLaunchKernelGGL(k_gather_strided, dim3(grid_for(n)), dim3(256), 0, s, n, P.grid5_pos.ptr, P.fstore.ptr,
+                       dinv.ptr, east.ptr, north.ptr);
+    M4G_HIP(hipGetLastError());
+    return true;
 }
 
 // Reads the sweep's fault flag and clears it: a wait that gave up spoils the

@@ -64,6 +64,10 @@ struct m4g_pc_s {
     m4g::DevBuf<unsigned char> tmp;       // intermediate vector of a PC application
     size_t tmp_bytes = 0;
     int64_t factor_levels = 0;
+    // IC(0) of a 2D 5-point grid: storage positions of each row's diagonal,
+    // east and north entries (-1: absent), for the one-workgroup CG
+    int grid5 = 0;                        // 0 not checked, 1 applies, -1 does not
+    m4g::DevBuf<int32_t> grid5_pos;       // [3n]
     ~m4g_pc_s();
 };
 
@@ -74,4 +78,10 @@ void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipS
 // Reads and clears the fault flags of every sweep of P (nonzero: a bounded
 // wait of a sweep launched since the last check gave up).
 int check_pc_faults(m4g_pc_s& P);
+// The IC(0) factor of a 2D 5-point grid of at most max_n rows and at most 64
+// lines as the one-workgroup CG reads it: U's east and north entries (0 where
+// absent) and the inverted diagonal per row.  False (nothing written) when P
+// is not such a factor.
+bool pc_grid_factor(m4g_pc_s& P, int64_t max_n, DevBuf<double>& east, DevBuf<double>& north, DevBuf<double>& dinv,
+                    hipStream_t s);
 }  // namespace m4g
