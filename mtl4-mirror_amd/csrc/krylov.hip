@@ -23,6 +23,7 @@ namespace m4g {
 
 constexpr int kNT = 256;
 constexpr int kRedBlocks = 1024;
+constexpr int kFoldMaxBlocks = 256;   // the CG folds its finalizes at this many reduction workgroups or fewer
 
 // ---------------------------------------------------------------------------
 // iteration control on the device
@@ -282,6 +283,72 @@ __global__ __launch_bounds__(kNT) void k_cg_update(int64_t n, const T* __restric
     }
     Sums<1> s{{double(acc)}};
     store_partials<kNT, 1>(s, partials);
+}
+
+// ---------------------------------------------------------------------------
+// CG with the rho and alpha finalizes folded into their consumers (small n)
+// ---------------------------------------------------------------------------
+// The total that k_fin<1, OP> forms from `nblocks` partials -- the same loop
+// and the same tree over 256 threads, hence the same value -- in every thread
+// of a 256-thread workgroup.  Each consumer workgroup forms it itself, which
+// saves a launch when the partials are few.
+__device__ inline double fin_total(const double* __restrict__ partials, int nblocks, double* scratch, double* bcast)
+{
+    double acc = 0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) acc += partials[b];
+    const double t = block_sum<256>(acc, scratch);
+    if (threadIdx.x == 0) *bcast = t;
+    __syncthreads();
+    return *bcast;
+}
+
+// rho = dot(r, z) from k_cg_rho's partials (FIN_CG_RHO), then
+// p = z (first) or z + (rho / rho_1) p   (cg.hpp:97-102)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_p_fold(int64_t n, const T* __restrict__ z, T* __restrict__ p, KState* st,
+                                                    const double* __restrict__ rho_partials, int nblocks)
+{
+    if (st->done) return;
+    __shared__ double scratch[kNT / kWave];
+    __shared__ double bcast;
+    const double rho = fin_total(rho_partials, nblocks, scratch, &bcast);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rho = rho;
+    if (st->i <= 1) {                                    // iter.first()
+        GRID_STRIDE(i, n) p[i] = z[i];
+    } else {
+        const T sc = T(rho / st->rho_1);
+        GRID_STRIDE(i, n) p[i] = z[i] + sc * p[i];
+    }
+}
+
+// alpha_1 = dot(p, q) from the SpMV's partials and alpha = rho / alpha_1
+// (FIN_CG_ALPHA), then k_cg_update's x, r and r.r partials (into out_partials)
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_cg_update_fold(int64_t n, const T* __restrict__ p, const T* __restrict__ q,
+                                                         T* __restrict__ x, T* __restrict__ r, KState* st,
+                                                         const double* __restrict__ pq_partials, int nblocks,
+                                                         double* out_partials)
+{
+    if (st->done) return;
+    __shared__ double scratch[kNT / kWave];
+    __shared__ double bcast;
+    const double alpha_1 = fin_total(pq_partials, nblocks, scratch, &bcast);
+    const double alpha_d = st->rho / alpha_1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->alpha_1 = alpha_1;
+        st->alpha = alpha_d;
+    }
+    const T alpha = T(alpha_d);
+    T acc = 0;
+    GRID_STRIDE(i, n)
+    {
+        x[i] = x[i] + alpha * p[i];
+        T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    Sums<1> sm{{double(acc)}};
+    store_partials<kNT, 1>(sm, out_partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -946,35 +1013,54 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
     hipLaunchKernelGGL(k_cg_start<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, S, part, cnt);
     FIN(1, FIN_CG_START, rg);
     M4G_HIP(hipGetLastError());
+    // with few reduction workgroups the rho and alpha finalizes are formed by
+    // every workgroup of their consumer (same sums, one launch less each);
+    // the r.r partials of the folded update go to the second half of `part`
+    const char* fold_env = std::getenv("M4G_CG_FOLD");   // M4G_CG_FOLD=0: separate finalizes always
+    const bool fold = rg <= kFoldMaxBlocks && !(fold_env && std::atoi(fold_env) == 0);
+    double* part2 = part + std::max<int64_t>(std::max<int64_t>(sgrid, pgrid), kRedBlocks);
     auto body = [&] {
         const T* zz = r.ptr;
+        bool rho_partials = false;
         if (!identity) {
             if (P->kind == M4G_PC_DIAGONAL) {
                 hipLaunchKernelGGL(k_cg_diag_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const double*)P->inv_diag.ptr,
                                    (const T*)r.ptr, z.ptr, S, part, cnt);
-                FIN(1, FIN_CG_RHO, rg);
             } else {
                 pc_apply_skip(*P, r.ptr, z.ptr, vt, &S->done, st);
                 hipLaunchKernelGGL(k_cg_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)z.ptr, S,
                                    part, cnt);
-                FIN(1, FIN_CG_RHO, rg);
             }
+            if (!fold) FIN(1, FIN_CG_RHO, rg);
+            rho_partials = true;
             zz = z.ptr;
         }
-        hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
+        if (fold && rho_partials)
+            hipLaunchKernelGGL(k_cg_p_fold<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, S, (const double*)part, rg);
+        else
+            hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
+        int pq_blocks;
         if (pw) {
             M4G_PIPE_LAUNCH(k_cg_spmv_pipe, T, pw, dim3(pgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices,
                             A.sell.padded, A.sell.slice_ptr.ptr, A.sell.idx.ptr, vals, (const T*)p.ptr, q.ptr, S, part);
-            FIN(1, FIN_CG_ALPHA, pgrid);
+            pq_blocks = pgrid;
         } else {
             hipLaunchKernelGGL(k_cg_spmv_dot<T>, dim3(sgrid), dim3(kSpmvNT), 0, st, n, A.sell.nslices, chunk,
                                A.sell.slice_ptr.ptr, A.sell.slice_idx.ptr, A.sell.idx.ptr, vals, (const T*)p.ptr,
                                q.ptr, S, part, cnt);
-            FIN(1, FIN_CG_ALPHA, sgrid);
+            pq_blocks = int(sgrid);
         }
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr, x, r.ptr,
-                           S, int(identity), part, cnt);
-        FIN(1, FIN_CG_UPDATE, rg);
+        if (fold) {
+            hipLaunchKernelGGL(k_cg_update_fold<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr,
+                               x, r.ptr, S, (const double*)part, pq_blocks, part2);
+            hipLaunchKernelGGL((k_fin<1, FIN_CG_UPDATE>), dim3(1), dim3(256), 0, st, (const double*)part2, int(rg), S,
+                               idpc);
+        } else {
+            FIN(1, FIN_CG_ALPHA, pq_blocks);
+            hipLaunchKernelGGL(k_cg_update<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)p.ptr, (const T*)q.ptr, x,
+                               r.ptr, S, int(identity), part, cnt);
+            FIN(1, FIN_CG_UPDATE, rg);
+        }
         M4G_HIP(hipGetLastError());
     };
     // a small 2D grid with an FP64 IC(0) factor: whole batches of iterations

@@ -112,3 +112,31 @@ def test_systems_outside_the_grid_path(g, case):
     r = solve(g, S, I, V, b, 1e-10)
     assert r["status"] == ref["status"] == o.OK
     assert abs(r["iterations"] - ref["iterations"]) <= 1
+
+
+@pytest.mark.parametrize("pc", ["ic0", "diag", "none"])
+def test_folded_finalizes_are_bit_identical(g, pc, monkeypatch):
+    # below 256 reduction workgroups the CG forms rho and alpha inside their
+    # consumer kernels; the scalars, hence x and the history, must be those
+    # of the separate finalize kernels bit for bit (M4G_CG_FOLD=0)
+    S, I, V = g.varcoef5(200, 150, seed=9)   # n = 30,000: 30 reduction workgroups
+    A = o.CSR(S, I, V)
+    b = o.spmv(A, g.rng_uniform(8, 1, A.n))
+    kind = {"ic0": g.PC_IC0, "diag": g.PC_DIAGONAL, "none": g.PC_IDENTITY}[pc]
+
+    def run():
+        Ag = g.Matrix(S, I, V)
+        P = g.PC(Ag, kind)
+        x = g.Vector(A.n)
+        x.fill(0.0)
+        it, hist = g.make_iteration(3000, 1e-10, 0.0, o.two_norm(b))
+        st = g.cg(Ag, x, g.Vector(data=b), P, it)
+        return st, it.iterations, x.numpy(), hist[: it.iterations + 1].copy()
+
+    folded = run()
+    monkeypatch.setenv("M4G_CG_FOLD", "0")
+    separate = run()
+    say(f"  {pc}: {folded[1]} iterations folded, {separate[1]} separate")
+    assert folded[0] == separate[0] == 0 and folded[1] == separate[1]
+    np.testing.assert_array_equal(folded[2], separate[2])
+    np.testing.assert_array_equal(folded[3], separate[3])
