@@ -252,13 +252,6 @@ __host__ __device__ inline SweepLds sweep_lds_layout(int rows, int imports, int 
     return l;
 }
 
-__device__ inline int wave_min(int v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
-    return v;
-}
-
 // Native 4 x 32-bit vector: copies of it stay in registers (HIP's int4 is a
 // struct, whose copies from global memory become private-memory memcpys).
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -448,42 +441,53 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         // Polls the next imports of the tile, in first-use order, straight
         // from the exchange buffer into LDS and publishes the prefix that has
         // arrived: a window of 64 * kImportDepth after progress, 64 while
-        // waiting (fewer loads on memory that is being written).
+        // waiting (fewer loads on memory that is being written).  A poll
+        // that finds nothing new touches no LDS (the window's slots stay in
+        // registers, the first missing import comes from ballots), so it
+        // does not queue behind the compute wavefront's LDS traffic.
         const int32_t* slot_of = reinterpret_cast<const int32_t*>(imp);
         int head = 0;
         int64_t polls = 0;
         bool wide = true;
-        while (head < ni) {
-            int32_t xs[kImportDepth];
-            VT val[kImportDepth];
+        int32_t xs[kImportDepth];   // exchange slot of import head + lane + 64u; -1: not polled
+        auto read_slots = [&]() {
 #pragma unroll
             for (int u = 0; u < kImportDepth; u++) {
                 const int idx = head + lane + kWave * u;
                 xs[u] = idx < ni && (u == 0 || wide) ? slot_of[idx * (sizeof(VT) / 4)] : -1;
             }
+        };
+        read_slots();
+        while (head < ni) {
+            VT val[kImportDepth];
 #pragma unroll
             for (int u = 0; u < kImportDepth; u++)
                 val[u] = xs[u] >= 0 ? __hip_atomic_load(xbuf + xs[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                     : VT(0);
-            int first_missing = ni;   // smallest index of the window that has not arrived
+            // smallest index of the window that has not arrived (uniform)
+            int first_missing = min(ni, head + kWave * (wide ? kImportDepth : 1));
 #pragma unroll
-            for (int u = 0; u < kImportDepth; u++) {
-                const int idx = head + lane + kWave * u;
-                if (xs[u] >= 0 && Sentinel<VT>::is(val[u])) first_missing = min(first_missing, idx);
+            for (int u = kImportDepth - 1; u >= 0; u--) {
+                const unsigned long long miss = __ballot(xs[u] >= 0 && Sentinel<VT>::is(val[u]));
+                if (miss) first_missing = head + kWave * u + int(__builtin_ctzll(miss));
             }
-            const int window_end = min(ni, head + kWave * (wide ? kImportDepth : 1));
-            first_missing = wave_min(min(first_missing, window_end));
 #pragma unroll
             for (int u = 0; u < kImportDepth; u++) {
                 const int idx = head + lane + kWave * u;
                 if (xs[u] >= 0 && idx < first_missing) imp[idx] = val[u];
             }
-            wide = first_missing > head;
-            if (wide) {
+            if (first_missing > head) {
                 head = first_missing;
+                wide = true;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 if (lane == 0) __hip_atomic_store(&progress, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                read_slots();   // the values written above lie below head
                 continue;
+            }
+            if (wide) {   // nothing new: poll the first 64 only
+                wide = false;
+#pragma unroll
+                for (int u = 1; u < kImportDepth; u++) xs[u] = -1;
             }
             ++polls;
             if (polls > a.spin_limit ||
