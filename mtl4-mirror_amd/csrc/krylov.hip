@@ -1017,8 +1017,18 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
     // every workgroup of their consumer (same sums, one launch less each);
     // the r.r partials of the folded update go to the second half of `part`
     const char* fold_env = std::getenv("M4G_CG_FOLD");   // M4G_CG_FOLD=0: separate finalizes always
-    const bool fold = rg <= kFoldMaxBlocks && !(fold_env && std::atoi(fold_env) == 0);
-    double* part2 = part + std::max<int64_t>(std::max<int64_t>(sgrid, pgrid), kRedBlocks);
+    const bool fold_ok = !(fold_env && std::atoi(fold_env) == 0);
+    const bool fold = rg <= kFoldMaxBlocks && fold_ok;
+    const int64_t part_half = std::max<int64_t>(std::max<int64_t>(sgrid, pgrid), kRedBlocks);
+    double* part2 = part + part_half;
+    // with a sweep preconditioner, rho = r.z comes from the last sweep: one
+    // partial per tile, written as the tile stores its rows of z
+    // (M4G_CG_SWEEP_DOT=0: a separate dot kernel)
+    const char* sdot_env = std::getenv("M4G_CG_SWEEP_DOT");
+    const int64_t dot_tiles = P && !(sdot_env && std::atoi(sdot_env) == 0) ? pc_dot_tiles(*P) : 0;
+    const bool sweep_dot = dot_tiles > 0 && dot_tiles <= part_half;
+    const int rho_blocks = sweep_dot ? int(dot_tiles) : rg;
+    const bool fold_rho = rho_blocks <= kFoldMaxBlocks && fold_ok;
     auto body = [&] {
         const T* zz = r.ptr;
         bool rho_partials = false;
@@ -1026,17 +1036,20 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
             if (P->kind == M4G_PC_DIAGONAL) {
                 hipLaunchKernelGGL(k_cg_diag_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const double*)P->inv_diag.ptr,
                                    (const T*)r.ptr, z.ptr, S, part, cnt);
+            } else if (sweep_dot) {
+                pc_apply_skip_dot(*P, r.ptr, z.ptr, vt, &S->done, st, r.ptr, part);
             } else {
                 pc_apply_skip(*P, r.ptr, z.ptr, vt, &S->done, st);
                 hipLaunchKernelGGL(k_cg_rho<T>, dim3(rg), dim3(kNT), 0, st, n, (const T*)r.ptr, (const T*)z.ptr, S,
                                    part, cnt);
             }
-            if (!fold) FIN(1, FIN_CG_RHO, rg);
+            if (!fold_rho) FIN(1, FIN_CG_RHO, rho_blocks);
             rho_partials = true;
             zz = z.ptr;
         }
-        if (fold && rho_partials)
-            hipLaunchKernelGGL(k_cg_p_fold<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, S, (const double*)part, rg);
+        if (fold_rho && rho_partials)
+            hipLaunchKernelGGL(k_cg_p_fold<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, S, (const double*)part,
+                               rho_blocks);
         else
             hipLaunchKernelGGL(k_cg_p<T>, dim3(rg), dim3(kNT), 0, st, n, zz, p.ptr, (const KState*)S);
         int pq_blocks;

@@ -41,4 +41,13 @@ void launch_residual_t(const m4g_matrix_s& A, const T* x, const T* b, T* r, hipS
 // PC application that is skipped on the device when *skip != 0 (pc.hip)
 void pc_apply_skip(m4g_pc_s& P, const void* x, void* y, int vtype, const int* skip, hipStream_t s);
 
+// Tiles of the PC's last sweep: the number of partials pc_apply_skip_dot
+// leaves; 0 when the PC has no sweeps.
+int64_t pc_dot_tiles(const m4g_pc_s& P);
+
+// pc_apply_skip of an IC(0)/ILU(0) PC that also stores, per tile of its last
+// sweep, the partial sum of w_r * y_r over the tile's rows into partials.
+void pc_apply_skip_dot(m4g_pc_s& P, const void* x, void* y, int vtype, const int* skip, hipStream_t s,
+                       const void* w, double* partials);
+
 }  // namespace m4g

@@ -222,6 +222,8 @@ struct SweepArgs {
     int* fault;                       // set when a bounded wait runs out
     int64_t other_nexport;            // size of the other direction's exchange buffer
     int64_t spin_limit;
+    const void* dot_with;             // w (vector type), or null: no dot
+    double* dot_partials;             // [ntiles] per tile: sum of w_r * y_r over its rows
 };
 
 // Workgroup roles: wavefront 0 imports, 1 computes, 2 and 3 load records,
@@ -604,6 +606,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
+    // write the tile's rows back; with dot_with, also the tile's partial of
+    // dot(w, y): per thread in the vector type, then across the workgroup
+    // in double (as the dot kernels sum their grid-stride partials)
+    const VT* w = static_cast<const VT*>(a.dot_with);
+    VT dacc = 0;
     for (int32_t i0 = rb + tid; i0 < re; i0 += kFillUnroll * kSweepThreads) {
         int32_t r[kFillUnroll], sl[kFillUnroll];
 #pragma unroll
@@ -612,9 +619,28 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
             r[u] = i < re ? a.rows[i] : -1;
             sl[u] = i < re ? a.row_slot[i] : 0;
         }
+        if (w) {
+            VT wv[kFillUnroll];
 #pragma unroll
-        for (int u = 0; u < kFillUnroll; u++)
-            if (r[u] >= 0) y[r[u]] = lds[sl[u]];
+            for (int u = 0; u < kFillUnroll; u++) wv[u] = r[u] >= 0 ? w[r[u]] : VT(0);
+#pragma unroll
+            for (int u = 0; u < kFillUnroll; u++)
+                if (r[u] >= 0) {
+                    const VT yv = lds[sl[u]];
+                    y[r[u]] = yv;
+                    dacc += wv[u] * yv;
+                }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kFillUnroll; u++)
+                if (r[u] >= 0) y[r[u]] = lds[sl[u]];
+        }
+    }
+    if (w) {
+        // the record ring is free now: its first words hold the wave sums
+        double* dscratch = reinterpret_cast<double*>(lds_raw + L.ring);
+        const double tot = block_sum<kSweepThreads>(double(dacc), dscratch);
+        if (tid == 0) a.dot_partials[t] = tot;
     }
     if (tid == 0) {
         unsigned done = atomicAdd(&a.tickets[1], 1u);
@@ -867,7 +893,8 @@ static void launch_sweep_w(SweepDev& d, const SweepArgs& a, const VT* v, VT* y, 
 }
 
 template <typename FV, typename VT>
-static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, const int* skip, hipStream_t s)
+static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, const int* skip, hipStream_t s,
+                           const VT* dot_with = nullptr, double* dot_partials = nullptr)
 {
     const SweepPlan& p = d.plan_meta;
     TraceRange tr(!p.lower ? kTraceUpperTrisolve : p.unit ? kTraceLowerUnitTrisolve : kTraceLowerTrisolve);
@@ -892,6 +919,8 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
     a.fault = d.fault.ptr;
     a.other_nexport = other.plan_meta.nexport;
     a.spin_limit = spin_limit();
+    a.dot_with = dot_with;
+    a.dot_partials = dot_partials;
     VT* xo = (VT*)other.xbuf.ptr;
     auto go = [&](auto unit_tag) {
         constexpr bool U = decltype(unit_tag)::value;
@@ -910,7 +939,7 @@ static void launch_sweep_t(SweepDev& d, SweepDev& other, const VT* v, VT* y, con
 
 
 void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp, void* y, int vtype,
-                    const int* skip, hipStream_t s)
+                    const int* skip, hipStream_t s, const void* dot_with, double* dot_partials)
 {
     M4G_REQUIRE(first.factor_type == second.factor_type, M4G_ERR_INVALID, "sweep factor types differ");
     first.ensure_exchange(vtype);
@@ -918,13 +947,16 @@ void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp,
     if (first.factor_type == M4G_F64) {
         M4G_REQUIRE(vtype == M4G_F64, M4G_ERR_INVALID, "double factors need double vectors");
         launch_sweep_t<double, double>(first, second, (const double*)v, (double*)tmp, skip, s);
-        launch_sweep_t<double, double>(second, first, (const double*)tmp, (double*)y, skip, s);
+        launch_sweep_t<double, double>(second, first, (const double*)tmp, (double*)y, skip, s,
+                                       (const double*)dot_with, dot_partials);
     } else if (vtype == M4G_F64) {
         launch_sweep_t<float, double>(first, second, (const double*)v, (double*)tmp, skip, s);
-        launch_sweep_t<float, double>(second, first, (const double*)tmp, (double*)y, skip, s);
+        launch_sweep_t<float, double>(second, first, (const double*)tmp, (double*)y, skip, s,
+                                      (const double*)dot_with, dot_partials);
     } else {
         launch_sweep_t<float, float>(first, second, (const float*)v, (float*)tmp, skip, s);
-        launch_sweep_t<float, float>(second, first, (const float*)tmp, (float*)y, skip, s);
+        launch_sweep_t<float, float>(second, first, (const float*)tmp, (float*)y, skip, s,
+                                     (const float*)dot_with, dot_partials);
     }
 }
 
@@ -1418,6 +1450,20 @@ void pc_apply(m4g_pc_s& P, const void* x, void* y, int vtype, bool adjoint, hipS
 void pc_apply_skip(m4g_pc_s& P, const void* x, void* y, int vtype, const int* skip, hipStream_t s)
 {
     pc_apply(P, x, y, vtype, false, s, skip);
+}
+
+int64_t pc_dot_tiles(const m4g_pc_s& P)
+{
+    if ((P.kind != M4G_PC_IC0 && P.kind != M4G_PC_ILU0) || P.n == 0) return 0;
+    return P.upper.plan_meta.ntiles;
+}
+
+void pc_apply_skip_dot(m4g_pc_s& P, const void* x, void* y, int vtype, const int* skip, hipStream_t s,
+                       const void* w, double* partials)
+{
+    M4G_REQUIRE(pc_dot_tiles(P) > 0, M4G_ERR_LOGIC, "pc_apply_skip_dot: no sweeps");
+    TraceRange tr(pc_event(P.kind, false));
+    run_sweep_pair(P.lower, P.upper, x, pc_tmp(P, vtype), y, vtype, skip, s, w, partials);
 }
 
 }  // namespace m4g

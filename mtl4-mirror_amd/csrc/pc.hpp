@@ -32,8 +32,11 @@ struct SweepDev {
     void ensure_exchange(int vtype);
 };
 
+// Both sweeps, v -> tmp -> y.  With dot_with (a vector of the same type), the
+// second sweep also stores each tile's partial of dot(dot_with, y) into
+// dot_partials[tile] (second.plan_meta.ntiles entries).
 void run_sweep_pair(SweepDev& first, SweepDev& second, const void* v, void* tmp, void* y, int vtype,
-                    const int* skip, hipStream_t s);
+                    const int* skip, hipStream_t s, const void* dot_with = nullptr, double* dot_partials = nullptr);
 int check_sweep_fault(SweepDev& d);
 int64_t sweep_tile_cap(int vtype);
 SweepPlan plan_sweep(const SweepRows& rows, const GridInfo& g, int64_t cap, int ftype);

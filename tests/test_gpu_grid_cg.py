@@ -10,7 +10,12 @@ path (M4G_GRID_CG=0) on the same inputs:
   * the iteration protocol: status, max_iter stop, absolute tolerance, a
     nonzero start vector and a caller-given norm_r0;
   * systems the path does not take (more than 64 lines, a 7-point pattern)
-    still solve through the multi-kernel path."""
+    still solve through the multi-kernel path.
+The multi-kernel CG's scalars are checked the same way: its folded
+finalizes against separate finalize kernels (bit for bit), and rho taken
+from the last sweep's per-tile partials against a separate dot kernel (to
+rounding, and the oracle's iteration count).
+"""
 import numpy as np
 import pytest
 
@@ -140,3 +145,42 @@ def test_folded_finalizes_are_bit_identical(g, pc, monkeypatch):
     assert folded[0] == separate[0] == 0 and folded[1] == separate[1]
     np.testing.assert_array_equal(folded[2], separate[2])
     np.testing.assert_array_equal(folded[3], separate[3])
+
+
+@pytest.mark.parametrize("case", ["p3d_ic0", "vc5_ilu0"])
+def test_rho_from_the_last_sweep(g, case, monkeypatch):
+    # with an IC(0)/ILU(0) PC the CG takes rho = r.z from the partials the
+    # last sweep stores per tile; a separate dot kernel (M4G_CG_SWEEP_DOT=0)
+    # sums the same products in another order, so the two solves agree to
+    # rounding: same iteration count (or one apart), x and history close
+    if case == "p3d_ic0":
+        S, I, V = g.poisson3d(40)          # 64,000 rows: several tiles per sweep
+        kind = g.PC_IC0
+    else:
+        S, I, V = g.varcoef5(300, 220, seed=3)
+        kind = g.PC_ILU0
+    A = o.CSR(S, I, V)
+    b = o.spmv(A, g.rng_uniform(9, 1, A.n))
+
+    def run():
+        Ag = g.Matrix(S, I, V)
+        P = g.PC(Ag, kind)
+        x = g.Vector(A.n)
+        x.fill(0.0)
+        it, hist = g.make_iteration(3000, 1e-10, 0.0, o.two_norm(b))
+        st = g.cg(Ag, x, g.Vector(data=b), P, it)
+        tiles = P.stats()["tiles_upper"]
+        return st, it.iterations, x.numpy(), hist[: it.iterations + 1].copy(), tiles
+
+    fused = run()
+    monkeypatch.setenv("M4G_CG_SWEEP_DOT", "0")
+    separate = run()
+    say(f"  {case}: {fused[1]} iterations with the sweep's partials ({fused[4]} tiles), {separate[1]} with a dot kernel")
+    assert fused[4] > 1
+    assert fused[0] == separate[0] == 0 and abs(fused[1] - separate[1]) <= 1
+    scale = max(1.0, np.max(np.abs(separate[2])))
+    assert np.max(np.abs(fused[2] - separate[2])) <= 1e-8 * scale
+    k = min(len(fused[3]), len(separate[3]), 40)
+    np.testing.assert_allclose(fused[3][:k], separate[3][:k], rtol=1e-7)
+    ref = o.solve("cg", A, b, pc=o.PC_IC0 if kind == g.PC_IC0 else o.PC_ILU0, max_iter=3000, rtol=1e-10)
+    assert abs(fused[1] - ref["iterations"]) <= 1
