@@ -993,7 +993,11 @@ static int cg_t(m4g_matrix_s& A, void* xp, const void* bp, m4g_pc_s* P, m4g_iter
     int64_t chunk, sgrid;
     spmv_geom(A, chunk, sgrid);
     const int pw = sell_pipe_width(A);
-    const int pgrid = sell_pipe_grid();
+    // the persistent SpMV grid, but no more workgroups than one slice per
+    // wavefront needs: on small systems its p.q partials then stay few
+    // enough for k_cg_update_fold to form alpha itself
+    const int pgrid = int(std::min<int64_t>(sell_pipe_grid(), std::max<int64_t>(1, (A.sell.nslices + kSliceWaves - 1) /
+                                                                                          kSliceWaves)));
     ws.prepare(std::max<int64_t>(sgrid, pgrid), it->history ? it->history_capacity : 0);
     KState h;
     init_state(h, it, ws.history.ptr);
