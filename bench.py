@@ -270,8 +270,9 @@ def _oracle():
 def krylov_family(g, N=64, iters=20, cpu=True):
     """The other ITL solvers on the N^3 7-point Poisson system with IC(0)
     (IDR(s): no preconditioner, as in the reference): time per iteration of a
-    fixed `iters`-iteration run on the GPU (scalars on the host, one read-back
-    per reduction) and of the oracle's restatement on one host core."""
+    fixed `iters`-iteration run on the GPU (scalars on the host, each
+    reduction posted to pinned host memory) and of the oracle's restatement
+    on one host core."""
     S, I, V = g.poisson3d(N, N, N)
     n = len(S) - 1
     A = g.Matrix(S, I, V)

@@ -83,18 +83,44 @@ __global__ __launch_bounds__(kNT) void k_convert(int64_t n, const S* __restrict_
         dst[i] = static_cast<D>(src[i]);
 }
 
-__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partials, int nblocks, double* out)
+// One thread writes count scalars into host-coherent pinned memory and then
+// the sequence number, both at system scope (the number released after the
+// values), so a host that sees the number sees the values.
+__device__ inline void post_scalars(HostPost* box, const double* v, int count, unsigned seq)
+{
+    for (int i = 0; i < count; i++) __hip_atomic_store(&box->val[i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&box->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// total of the partials into *out and, with a mailbox, posted to the host
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partials, int nblocks, double* out,
+                                                      HostPost* box, unsigned seq)
 {
     __shared__ double scratch[256 / kWave];
     double acc = 0;
     for (int b = threadIdx.x; b < nblocks; b += 256) acc += partials[b];
     acc = block_sum<256>(acc, scratch);
-    if (threadIdx.x == 0) *out = acc;
+    if (threadIdx.x == 0) {
+        *out = acc;
+        if (box) post_scalars(box, &acc, 1, seq);
+    }
+}
+
+__global__ void __launch_bounds__(64) k_post(const double* __restrict__ src, int count, HostPost* box, unsigned seq)
+{
+    if (threadIdx.x == 0) post_scalars(box, src, count, seq);
 }
 
 void launch_sum_partials(const double* partials, int nblocks, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, nblocks, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, nblocks, out, nullptr, 0u);
+}
+
+void launch_post(const double* src, int count, HostPost* box, unsigned seq, hipStream_t s)
+{
+    M4G_REQUIRE(count >= 0 && count <= kPostMax, M4G_ERR_INVALID, "launch_post: too many scalars");
+    hipLaunchKernelGGL(k_post, dim3(1), dim3(64), 0, s, src, count, box, seq);
+    M4G_HIP(hipGetLastError());
 }
 
 static int stream_grid(int64_t n)
@@ -120,12 +146,31 @@ void launch_dot(int64_t n, const double* x, const double* y, double* out, hipStr
     launch_dot_with(n, x, y, reduce_ws().partials, out, s);
 }
 
+void launch_dot_post(int64_t n, const double* x, const double* y, double* out, HostPost* box, unsigned seq,
+                     hipStream_t s)
+{
+    TraceRange tr(kTraceDot);
+    ReduceWs& ws = reduce_ws();
+    hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, y, ws.partials, nullptr, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, ws.partials, reduce_grid(n), out, box, seq);
+    M4G_HIP(hipGetLastError());
+}
+
 void launch_sumsq(int64_t n, const double* x, double* out, hipStream_t s)
 {
     TraceRange tr(kTraceUnaryDot);
     ReduceWs& ws = reduce_ws();
     hipLaunchKernelGGL(k_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, ws.partials, ws.counters, out);
     launch_sum_partials(ws.partials, reduce_grid(n), out, s);
+    M4G_HIP(hipGetLastError());
+}
+
+void launch_sumsq_post(int64_t n, const double* x, double* out, HostPost* box, unsigned seq, hipStream_t s)
+{
+    TraceRange tr(kTraceUnaryDot);
+    ReduceWs& ws = reduce_ws();
+    hipLaunchKernelGGL(k_sumsq, dim3(reduce_grid(n)), dim3(kNT), 0, s, n, x, ws.partials, ws.counters, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, ws.partials, reduce_grid(n), out, box, seq);
     M4G_HIP(hipGetLastError());
 }
 

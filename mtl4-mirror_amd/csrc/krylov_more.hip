@@ -3,8 +3,9 @@
 //
 // Unlike cg / bicgstab (krylov.hip), whose scalars stay on the device, these
 // solvers run host-driven: each dot or norm is one deterministic two-stage
-// reduction read back to the host, where the scalar recurrences and
-// basic_iteration's test (basic_iteration.hpp:52-93) run.  Every vector
+// reduction whose last kernel posts the value into pinned host memory, where
+// the host spins for it and runs the scalar recurrences and basic_iteration's
+// test (basic_iteration.hpp:52-93).  Every vector
 // update is one kernel that evaluates the reference's expression element by
 // element in its operation order (e.g. p = u + beta*(q + beta*p)), so the
 // iterates differ from the CPU only through the reduction order of dots.
@@ -102,11 +103,16 @@ int lin_grid(int64_t n)
     return int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)));
 }
 
-// The vector algebra of the host-driven solvers on one stream.
+// The vector algebra of the host-driven solvers on one stream.  Dots and
+// norms come back through a mailbox: the reduction's last kernel posts the
+// value into pinned host memory and the host spins on it.
 struct Ops {
     int64_t n;
     hipStream_t s;
     double* red;   // one double of scratch for reductions
+    std::unique_ptr<Mailbox> mb;
+
+    Ops(int64_t n_, hipStream_t s_, double* red_) : n(n_), s(s_), red(red_), mb(std::make_unique<Mailbox>(s_)) {}
 
     void lin1(const double* a, double sc, const double* b, double* y) const   // y = a + sc*b
     {
@@ -126,13 +132,15 @@ struct Ops {
     }
     double dot(const double* a, const double* b) const
     {
-        launch_dot(n, a, b, red, s);
-        return read_scalar(red);
+        const unsigned q = mb->arm();
+        launch_dot_post(n, a, b, red, mb->box(), q, s);
+        return mb->wait(q)[0];
     }
     double norm(const double* a) const
     {
-        launch_sumsq(n, a, red, s);
-        return std::sqrt(read_scalar(red));
+        const unsigned q = mb->arm();
+        launch_sumsq_post(n, a, red, mb->box(), q, s);
+        return std::sqrt(mb->wait(q)[0]);
     }
     void scale(double sc, const double* a, double* y) const   // y = sc*a
     {
@@ -175,8 +183,13 @@ struct Ops {
         std::vector<double> w(v.size());
         if (v.empty()) return w;
         for (size_t i = 0; i < v.size(); i++) launch_dot(n, v[i], a, scratch + i, s);
-        M4G_HIP(hipMemcpyAsync(w.data(), scratch, v.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        M4G_HIP(hipStreamSynchronize(s));
+        for (size_t i0 = 0; i0 < v.size(); i0 += kPostMax) {
+            const int count = int(std::min<size_t>(kPostMax, v.size() - i0));
+            const unsigned q = mb->arm();
+            launch_post(scratch + i0, count, mb->box(), q, s);
+            const double* got = mb->wait(q);
+            std::copy(got, got + count, w.begin() + std::ptrdiff_t(i0));
+        }
         return w;
     }
 };
@@ -271,16 +284,15 @@ static int cgs_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* P, m4g_i
     return status_of(it, sweep_fault(P));
 }
 
-// itl::bicg (bicg.hpp:27-68); adjoint(A) * v runs as A^T (built once, rows in
-// A's row order, so each entry sums in the order of the transposed traversal)
+// itl::bicg (bicg.hpp:27-68); adjoint(A) * v runs as A^T (pattern kept with A,
+// rows in A's row order, so each entry sums in the order of the transposed
+// traversal; values gathered from A at the start of each solve)
 static int bicg_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* P, m4g_iteration* it)
 {
     const int64_t n = A.nrows;
     DevBuf<double> red(1);
     Ops o{n, current_stream(), red.ptr};
-    m4g_matrix At = nullptr;
-    M4G_REQUIRE(m4g_matrix_transpose(&A, &At) == M4G_OK, M4G_ERR_HIP, m4g_last_error());
-    std::unique_ptr<m4g_matrix_s, int (*)(m4g_matrix)> hold(At, m4g_matrix_destroy);
+    const m4g_matrix_s& At = transpose_of(A, o.s);
     Vecs V(8, n);
     double *r = V[0], *z = V[1], *p = V[2], *q = V[3], *r_t = V[4], *z_t = V[5], *p_t = V[6], *q_t = V[7];
     double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
@@ -304,7 +316,7 @@ static int bicg_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* P, m4g_
             o.lin1(z_t, beta, p_t, p_t);  // p_tilde = z_tilde + conj(beta)*p_tilde
         }
         launch_spmv(A, p, q, M4G_ASSIGN, nullptr, o.s);
-        launch_spmv(*At, p_t, q_t, M4G_ASSIGN, nullptr, o.s);
+        launch_spmv(At, p_t, q_t, M4G_ASSIGN, nullptr, o.s);
         alpha = rho_1 / o.dot(p_t, q);
         o.lin1(x, alpha, p, x);
         o.minus(r, alpha, q, r);
@@ -391,7 +403,7 @@ static int tfqmr_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* L, m4g
 // ---- QMR, GMRES, BiCGStab(l), IDR(s) -----------------------------------------
 
 // itl::qmr (qmr.hpp:29-109) with left L and right R (null: identity);
-// trans(A) * q runs on an explicit transpose.  The loop tests rho = ||y||,
+// trans(A) * q runs on A's kept transpose.  The loop tests rho = ||y||,
 // the norm of the preconditioned Lanczos vector, as the reference does;
 // breakdowns record their codes 1..6.
 static int qmr_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* L, m4g_pc_s* R, m4g_iteration* it)
@@ -400,9 +412,7 @@ static int qmr_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* L, m4g_p
     M4G_REQUIRE(n > 0, M4G_ERR_LOGIC, "empty rhs vector");
     DevBuf<double> red(1);
     Ops o{n, current_stream(), red.ptr};
-    m4g_matrix At = nullptr;
-    M4G_REQUIRE(m4g_matrix_transpose(&A, &At) == M4G_OK, M4G_ERR_HIP, m4g_last_error());
-    std::unique_ptr<m4g_matrix_s, int (*)(m4g_matrix)> hold(At, m4g_matrix_destroy);
+    const m4g_matrix_s& At = transpose_of(A, o.s);
     Vecs V(14, n);
     double *r = V[0], *v_tld = V[1], *y = V[2], *w_tld = V[3], *z = V[4], *v = V[5], *w = V[6], *y_tld = V[7],
            *z_tld = V[8], *p = V[9], *q = V[10], *p_tld = V[11], *d = V[12], *sv = V[13];
@@ -450,7 +460,7 @@ static int qmr_t(m4g_matrix_s& A, double* x, const double* b, m4g_pc_s* L, m4g_p
         apply_pc(L, v_tld, y, false, o);
         rho_1 = rho;
         rho = o.norm(y);
-        launch_spmv(*At, q, w_tld, M4G_ASSIGN, nullptr, o.s);
+        launch_spmv(At, q, w_tld, M4G_ASSIGN, nullptr, o.s);
         o.minus(w_tld, beta, w, w_tld);               // w_tld = trans(A)*q - beta*w
         apply_pc(R, w_tld, z, true, o);
         xi = o.norm(z);

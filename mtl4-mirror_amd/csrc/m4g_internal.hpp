@@ -11,6 +11,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mtl4gpu.h"
@@ -63,6 +64,7 @@ int guard(F&& f)
 // ---------------------------------------------------------------------------
 // device and stream state
 // ---------------------------------------------------------------------------
+struct HostPost;
 struct DeviceState {
     int device = 0;
     hipStream_t stream = nullptr;      // library stream (default priority)
@@ -71,6 +73,8 @@ struct DeviceState {
     std::multimap<size_t, void*> pool; // freed buffers by size, reused exactly
     double* pinned = nullptr;          // small pinned host staging buffer (scalars)
     double* noalloc_scratch = nullptr; // this device's copy of m4g_dot_noalloc's scratch array
+    std::mutex mail_mutex;
+    std::vector<std::pair<HostPost*, unsigned>> mailboxes;   // idle mailboxes, next sequence number
 };
 
 DeviceState& device_state();          // state of the calling thread's device (created on first use)
@@ -168,6 +172,10 @@ struct m4g_matrix_s {
     // reduction workspace of the fused SpMV + dot (one partial per workgroup)
     m4g::DevBuf<double> red_partials;
     m4g::DevBuf<unsigned> red_counter;
+    // trans(A) for the adjoint products of bicg / qmr: pattern, SELL layout and
+    // the map to A's entries built on first use, values gathered at each use
+    std::unique_ptr<m4g_matrix_s> tr;
+    m4g::DevBuf<int64_t> tr_pos;
 };
 
 struct m4g_pc_s;   // defined in pc.hip
@@ -178,6 +186,9 @@ namespace m4g {
 // kernel launchers shared between translation units
 // ---------------------------------------------------------------------------
 void sell_build(m4g_matrix_s& A);                   // spmv.hip
+// trans(A) with A's current values (runtime.hip): the pattern is built on the
+// first call and kept with A, the values are gathered from A on stream s
+const m4g_matrix_s& transpose_of(m4g_matrix_s& A, hipStream_t s);
 void sell_refresh_values(m4g_matrix_s& A);          // spmv.hip
 void sell_build_f32(m4g_matrix_s& A);               // spmv.hip
 
@@ -218,5 +229,44 @@ constexpr int kCounters = 64;
 
 // Host read of a device double (synchronises the stream).
 double read_scalar(const double* d);
+
+// Scalars that a kernel posts straight into pinned, host-coherent memory,
+// followed by a sequence number.  The host spins on the number instead of
+// copying the value back and synchronising the stream: about 6.5 us from a
+// small kernel's launch to its value on the host, against 14 us for a copy
+// and a stream synchronisation.
+constexpr int kPostMax = 64;
+struct HostPost {
+    double val[kPostMax];
+    unsigned seq;
+};
+
+// One of the device's mailboxes, leased for the lifetime of a solve on one
+// stream.  arm() gives the number the next post carries; wait() returns the
+// posted values once that number has arrived (after 50 ms of spinning it
+// synchronises the stream instead, which also reports a failed kernel).  A
+// mailbox goes back to the pool only once every armed post has arrived.
+class Mailbox {
+public:
+    explicit Mailbox(hipStream_t s);
+    ~Mailbox();
+    Mailbox(const Mailbox&) = delete;
+    Mailbox& operator=(const Mailbox&) = delete;
+    HostPost* box() const { return box_; }
+    unsigned arm() { return ++armed_; }
+    const double* wait(unsigned seq);
+
+private:
+    HostPost* box_ = nullptr;
+    unsigned armed_ = 0, seen_ = 0;
+    hipStream_t s_;
+};
+
+// dot / squared norm into *out, the result also posted into box[0] with seq
+void launch_dot_post(int64_t n, const double* x, const double* y, double* out, HostPost* box, unsigned seq,
+                     hipStream_t s);
+void launch_sumsq_post(int64_t n, const double* x, double* out, HostPost* box, unsigned seq, hipStream_t s);
+// posts src[0..count) (count <= kPostMax, device memory) into box with seq
+void launch_post(const double* src, int count, HostPost* box, unsigned seq, hipStream_t s);
 
 }  // namespace m4g

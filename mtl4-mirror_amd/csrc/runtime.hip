@@ -2,6 +2,7 @@
 #include "m4g_internal.hpp"
 
 #include <algorithm>
+#include <chrono>
 
 namespace m4g {
 
@@ -110,6 +111,98 @@ double read_scalar(const double* d)
     M4G_HIP(hipMemcpyAsync(st.pinned, d, sizeof(double), hipMemcpyDeviceToHost, s));
     M4G_HIP(hipStreamSynchronize(s));
     return st.pinned[0];
+}
+
+Mailbox::Mailbox(hipStream_t s) : s_(s)
+{
+    DeviceState& st = device_state();
+    {
+        std::lock_guard<std::mutex> lock(st.mail_mutex);
+        if (!st.mailboxes.empty()) {
+            box_ = st.mailboxes.back().first;
+            armed_ = seen_ = st.mailboxes.back().second;
+            st.mailboxes.pop_back();
+        }
+    }
+    if (!box_) {
+        M4G_HIP(hipHostMalloc(reinterpret_cast<void**>(&box_), sizeof(HostPost), hipHostMallocCoherent));
+        std::memset(static_cast<void*>(box_), 0, sizeof(HostPost));
+    }
+}
+
+Mailbox::~Mailbox()
+{
+    // a post still in flight (a solve left by an exception) must land before
+    // the mailbox serves another solve
+    if (seen_ != armed_ && hipStreamSynchronize(s_) != hipSuccess) return;   // a failed stream: keep it out of the pool
+    DeviceState& st = device_state();
+    std::lock_guard<std::mutex> lock(st.mail_mutex);
+    st.mailboxes.emplace_back(box_, armed_);
+}
+
+const double* Mailbox::wait(unsigned seq)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 1;; k++) {
+        if (__atomic_load_n(&box_->seq, __ATOMIC_ACQUIRE) == seq) {
+            seen_ = seq;
+            return box_->val;
+        }
+        if ((k & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+        __builtin_ia32_pause();
+    }
+    // a long kernel ahead of the post, or a failed one: the stream's own wait
+    M4G_HIP(hipStreamSynchronize(s_));
+    M4G_REQUIRE(__atomic_load_n(&box_->seq, __ATOMIC_ACQUIRE) == seq, M4G_ERR_HIP,
+                "a posted scalar did not arrive although its stream completed");
+    seen_ = seq;
+    return box_->val;
+}
+
+// The pattern of trans(A) (rows of A's columns, entries in A's row order, so
+// each row of the transpose sums in the order of the transposed traversal),
+// uploaded, and pos[d] = the position in A of the transpose's entry d.
+std::unique_ptr<m4g_matrix_s> transpose_pattern(const m4g_matrix_s& A, DevBuf<int64_t>& pos_dev)
+{
+    auto T = std::make_unique<m4g_matrix_s>();
+    const int64_t m = A.nrows, n = A.ncols, nnz = A.nnz;
+    T->nrows = n;
+    T->ncols = m;
+    T->nnz = nnz;
+    T->h_starts.assign(n + 1, 0);
+    T->h_indices.assign(nnz, 0);
+    std::vector<int64_t> pos(nnz);
+    for (int64_t p = 0; p < nnz; p++) T->h_starts[A.h_indices[p] + 1]++;
+    for (int64_t r = 0; r < n; r++) T->h_starts[r + 1] += T->h_starts[r];
+    std::vector<int64_t> fill(T->h_starts.begin(), T->h_starts.end() - 1);
+    for (int64_t r = 0; r < m; r++)
+        for (int64_t p = A.h_starts[r]; p < A.h_starts[r + 1]; p++) {
+            int64_t d = fill[A.h_indices[p]]++;
+            T->h_indices[d] = r;
+            pos[d] = p;
+        }
+    std::vector<int32_t> s32(T->h_starts.begin(), T->h_starts.end()), i32(T->h_indices.begin(), T->h_indices.end());
+    T->starts.reset(n + 1);
+    T->indices.reset(std::max<int64_t>(nnz, 1));
+    T->values.reset(std::max<int64_t>(nnz, 1));
+    T->starts.upload(s32.data(), n + 1);
+    T->indices.upload(i32.data(), nnz);
+    pos_dev.reset(std::max<int64_t>(nnz, 1));
+    pos_dev.upload(pos.data(), nnz);
+    return T;
+}
+
+const m4g_matrix_s& transpose_of(m4g_matrix_s& A, hipStream_t s)
+{
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    const bool first = !A.tr;
+    if (first) A.tr = transpose_pattern(A, A.tr_pos);
+    m4g_matrix_s& T = *A.tr;
+    launch_gather_values(A.nnz, A.tr_pos.ptr, A.values.ptr, T.values.ptr, s);
+    if (first) sell_build(T);   // lays out the values just gathered
+    else sell_refresh_values(T);
+    return T;
 }
 
 ReduceWs& reduce_ws()
@@ -395,32 +488,9 @@ int m4g_matrix_transpose(m4g_matrix A, m4g_matrix* out)
 {
     return guard([&] {
         M4G_REQUIRE(A && out, M4G_ERR_INVALID, "m4g_matrix_transpose: null argument");
-        auto T = std::make_unique<m4g_matrix_s>();
-        const int64_t m = A->nrows, n = A->ncols, nnz = A->nnz;
-        T->nrows = n;
-        T->ncols = m;
-        T->nnz = nnz;
-        T->h_starts.assign(n + 1, 0);
-        T->h_indices.assign(nnz, 0);
-        std::vector<int64_t> pos(nnz);
-        for (int64_t p = 0; p < nnz; p++) T->h_starts[A->h_indices[p] + 1]++;
-        for (int64_t r = 0; r < n; r++) T->h_starts[r + 1] += T->h_starts[r];
-        std::vector<int64_t> fill(T->h_starts.begin(), T->h_starts.end() - 1);
-        for (int64_t r = 0; r < m; r++)
-            for (int64_t p = A->h_starts[r]; p < A->h_starts[r + 1]; p++) {
-                int64_t d = fill[A->h_indices[p]]++;
-                T->h_indices[d] = r;
-                pos[d] = p;
-            }
-        std::vector<int32_t> s32(T->h_starts.begin(), T->h_starts.end()), i32(T->h_indices.begin(), T->h_indices.end());
-        T->starts.reset(n + 1);
-        T->indices.reset(std::max<int64_t>(nnz, 1));
-        T->values.reset(std::max<int64_t>(nnz, 1));
-        T->starts.upload(s32.data(), n + 1);
-        T->indices.upload(i32.data(), nnz);
-        DevBuf<int64_t> dpos(std::max<int64_t>(nnz, 1));
-        dpos.upload(pos.data(), nnz);
-        launch_gather_values(nnz, dpos.ptr, A->values.ptr, T->values.ptr, current_stream());
+        DevBuf<int64_t> pos;
+        auto T = transpose_pattern(*A, pos);
+        launch_gather_values(A->nnz, pos.ptr, A->values.ptr, T->values.ptr, current_stream());
         sell_build(*T);
         M4G_HIP(hipStreamSynchronize(current_stream()));
         *out = T.release();

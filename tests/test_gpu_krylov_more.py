@@ -64,6 +64,53 @@ def test_solver_matches_oracle(g, method, case):
         assert np.max(np.abs(xs - ref["x"])) <= 1e-6 * max(1.0, np.max(np.abs(ref["x"])))
 
 
+def test_adjoint_products_follow_new_values(g):
+    """bicg and qmr keep trans(A)'s pattern with A and gather its values at
+    each solve: after set_values both follow the oracle on the new system (and
+    on the first one again once its values are put back).  bicg converges
+    within the oracle's own iteration range over 1e-14 changes of b (66-70
+    here: the residual wanders near the tolerance) +- 2 %; MTL4's qmr tests
+    the Lanczos vector's norm, which
+    here stays above 1e-8 * ||r0||, so it runs its 60 iterations on both
+    sides and x is compared (the oracle's own spread is about 1e-8)."""
+    S, I, V = g.convdiff27(9, 8, 7)
+    S2, I2 = np.asarray(S), np.asarray(I)
+    rows = np.repeat(np.arange(len(S2) - 1), np.diff(S2))
+    V1 = np.asarray(V, dtype=np.float64)
+    # a nonsymmetric change of the off-diagonal entries: trans(A) changes too
+    V2 = np.where(rows != I2, V1 * (1.0 + 0.3 * np.sin(rows + 2.0 * I2)), V1)
+    n = len(S2) - 1
+    b = g.rng_uniform(5, 1, n)
+    nb = o.two_norm(b)
+    rtol = 1e-8
+    Ag = g.Matrix(S, I, V1)
+    P = g.PC(Ag, g.PC_IDENTITY)
+    for vals in (V1, V2, V1):
+        Ag.set_values(vals)
+        A = o.CSR(S, I, vals)
+        for method, max_iter in (("bicg", 400), ("qmr", 60)):
+            print(f"{method} after set_values", flush=True)
+            ref = o.solve(method, A, b, pc=0, max_iter=max_iter, rtol=rtol, history=False)
+            x = g.Vector(n)
+            x.fill(0.0)
+            it, _ = g.make_iteration(max_iter, rtol, 0.0, nb, history=False)
+            bv = g.Vector(data=b)
+            st = g.bicg(Ag, x, bv, P, it) if method == "bicg" else g.qmr(Ag, x, bv, P, it)
+            xs = x.numpy()
+            if method == "bicg":
+                assert st == 0 and ref["status"] == 0, (st, ref["status"])
+                spread = [ref["iterations"]]
+                for k in range(1, 12):
+                    bb = b * (1.0 + 1e-14 * np.random.default_rng(k).standard_normal(n))
+                    spread.append(o.solve(method, A, bb, pc=0, max_iter=max_iter, rtol=rtol, history=False)["iterations"])
+                slack = max(1, int(np.ceil(0.02 * max(spread))))
+                assert min(spread) - slack <= it.iterations <= max(spread) + slack, (it.iterations, spread)
+                assert o.two_norm(o.residual(A, xs, b)) <= 20 * rtol * nb
+            else:
+                assert st == g.MAX_ITER and ref["status"] == g.MAX_ITER and it.iterations == max_iter
+            assert np.max(np.abs(xs - ref["x"])) <= 1e-6 * max(1.0, np.max(np.abs(ref["x"])))
+
+
 def test_zero_rhs_and_max_iter(g):
     S, I, V = g.laplacian2d(12, 12)
     Ag = g.Matrix(S, I, V)
