@@ -914,9 +914,69 @@ static void init_state(KState& h, const m4g_iteration* it, double* hist_dev)
     h.history_cap = it->history ? it->history_capacity : 0;
 }
 
+// M4G_KRYLOV_GRAPH=1: replay the iteration batches as a hipGraph.  Off by
+// default: the GPU time of a solve does not improve (the eager launches
+// already run back to back; 128^3 IC(0)-PCG 43.36 ms eager, 43.63 ms
+// replayed), only the host's work per batch shrinks to one launch.
+static bool graph_batches()
+{
+    const char* e = std::getenv("M4G_KRYLOV_GRAPH");
+    return e && std::atoi(e) != 0;
+}
+
+// A batch of body() calls captured once as a hipGraph and replayed.  The
+// first batch runs eagerly (it makes the lazy allocations, which a capture
+// cannot), the second is captured; a stream that cannot be captured (the
+// null stream, a capture already running) keeps the eager launches.  The
+// iteration kernels take only device pointers and read their scalars from
+// the device state, so every replay is the same work as an eager batch, and
+// no memset is recorded (the sweep kernels reset their own tickets).
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    template <typename Body>
+    bool capture(hipStream_t s, int calls, Body& body)
+    {
+        if (!s) return false;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        try {
+            for (int k = 0; k < calls; k++) body();
+        } catch (...) {
+            // nothing recorded has run: the caller launches the batch eagerly,
+            // where an error that was not the capture's shows again
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return false;
+        }
+        if (hipStreamEndCapture(s, &graph) != hipSuccess || !graph) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            exec = nullptr;
+            return false;
+        }
+        return true;
+    }
+};
+
 // Runs `body()` in batches (calls_per_poll calls, default one per iteration
 // of a batch) until the device state reports finished; polls the flag of
-// batch b while batch b+1 is queued.
+// batch b while batch b+1 is queued.  With graph_batches(), a batch from the
+// second on is one hipGraph launch.
 template <typename Body>
 static void run_batches(KrylovWs& ws, const m4g_iteration* it, Body&& body, int calls_per_poll = 0)
 {
@@ -930,9 +990,21 @@ static void run_batches(KrylovWs& ws, const m4g_iteration* it, Body&& body, int 
     bool finished = false;
     // guard: never queue more batches than max_iter allows (+1 for the check)
     const int64_t cap = int64_t(std::max(it->max_iter - it->iterations, 0)) + B + 1;
+    BatchGraph bg;
+    bool use_graph = false, tried = !graph_batches();
     try {
         while (!finished) {
-            for (int k = 0; k < B; k++) body();
+            if (use_graph) {
+                M4G_HIP(hipGraphLaunch(bg.exec, s));
+            } else if (launched > 0 && !tried) {
+                tried = true;
+                use_graph = bg.capture(s, B, body);
+                if (use_graph) M4G_HIP(hipGraphLaunch(bg.exec, s));
+                else
+                    for (int k = 0; k < B; k++) body();
+            } else {
+                for (int k = 0; k < B; k++) body();
+            }
             launched += B;
             M4G_HIP(hipMemcpyAsync(ws.host_state + slot, ws.state.ptr, sizeof(KState), hipMemcpyDeviceToHost, s));
             M4G_HIP(hipEventRecord(ev[slot], s));

@@ -227,7 +227,8 @@ ReduceWs& reduce_ws();
 constexpr int kReduceBlocks = 1024;  // blocks of the first reduction stage
 constexpr int kCounters = 64;
 
-// Host read of a device double (synchronises the stream).
+// Host read of a device double once the stream's work before it is done
+// (posted through a mailbox, below).
 double read_scalar(const double* d);
 
 // Scalars that a kernel posts straight into pinned, host-coherent memory,

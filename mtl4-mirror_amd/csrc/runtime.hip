@@ -106,11 +106,11 @@ void pool_free(void* p, size_t bytes)
 
 double read_scalar(const double* d)
 {
-    DeviceState& st = device_state();
     hipStream_t s = current_stream();
-    M4G_HIP(hipMemcpyAsync(st.pinned, d, sizeof(double), hipMemcpyDeviceToHost, s));
-    M4G_HIP(hipStreamSynchronize(s));
-    return st.pinned[0];
+    Mailbox mb(s);
+    const unsigned q = mb.arm();
+    launch_post(d, 1, mb.box(), q, s);
+    return mb.wait(q)[0];
 }
 
 Mailbox::Mailbox(hipStream_t s) : s_(s)
