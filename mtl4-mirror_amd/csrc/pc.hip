@@ -355,8 +355,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
     // the role of a wavefront is uniform: keep its branches scalar, so the
     // roles' code is not linearised into one another
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    // Tiles go out in ticket order, so a tile's dependencies are held by
+    // workgroups that have started.  Workgroup 0 is dispatched first and
+    // takes tile 0 without a ticket: the first tile, whose start-up is on the
+    // critical path, then issues its loads without waiting for an atomic
+    // that all workgroups contend for.
     if (tid == 0) {
-        tile_sh = int(atomicAdd(&a.tickets[0], 1u));
+        if (blockIdx.x != 0) tile_sh = 1 + int(atomicAdd(&a.tickets[0], 1u));
         progress = 0;
         consumed = 0;
         finished = 0;
@@ -364,8 +369,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
         half_batch[1] = -1;
         lds[a.max_tile_rows + a.max_tile_imports] = VT(0);
     }
-    __syncthreads();
-    const int t = tile_sh;
+    if (blockIdx.x != 0) __syncthreads();
+    const int t = blockIdx.x == 0 ? 0 : tile_sh;
     // the other direction's exchange buffer is free: reset it for its next use
     for (int64_t x = int64_t(t) * kSweepThreads + tid; x < a.other_nexport; x += a.ntiles * int64_t(kSweepThreads))
         xbuf_other[x] = Sentinel<VT>::value();
@@ -644,7 +649,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a, const int4
     }
     if (tid == 0) {
         unsigned done = atomicAdd(&a.tickets[1], 1u);
-        if (done == unsigned(a.ntiles) - 1) {   // every tile holds a ticket: reset for the next launch
+        if (done == unsigned(a.ntiles) - 1) {   // every tile has started: reset for the next launch
             a.tickets[0] = 0;
             a.tickets[1] = 0;
         }
