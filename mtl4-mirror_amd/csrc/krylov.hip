@@ -384,25 +384,27 @@ struct GridCgArgs {
     KState* st;
 };
 
-// LDS doubles of the kernel: sweep vector, east (one leading 0), north (nx
-// leading 0s), inverted diagonal, one dump slot per lane.
-inline int64_t grid_cg_lds_doubles(int64_t n, int64_t nx) { return 4 * n + nx + 1 + kWave; }
+// LDS doubles of the kernel: 8 of slack, sweep vector, east (one leading 0),
+// north (nx leading 0s), inverted diagonal, one dump slot per lane, 8 zeros.
+constexpr int kGridSlack = 8;
+inline int64_t grid_cg_lds_doubles(int64_t n, int64_t nx) { return kGridSlack + 4 * n + nx + 1 + kWave + 8; }
 
 // Lane l receives lane l-1's value (lane 0: +0); lane l receives lane l+1's
-// (lane 63: +0).  DPP wave_shr:1 / wave_shl:1 on the two halves.
+// (lane 63: +0).  DPP wave_shr:1 / wave_shl:1 on the two halves, lanes
+// without a source reading 0 (bound_ctrl).
 __device__ inline double lane_from_prev(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, int(b), 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x138, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0x138, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 __device__ inline double lane_from_next(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, int(b), 0x130, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x130, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0x130, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
@@ -412,53 +414,74 @@ __device__ inline double lane_from_next(double v)
 // the first operand from lane j-1, the second the lane's own last result.
 // Upper sweep on U, steps backwards (upper_trisolve.hpp:159-176):
 //   z_r = ((t_r - U[r][r+1] z_{r+1}) - U[r][r+nx] z_{r+nx}) * dinv_r.
-// Eight steps a turn: the operands of the turn's eight steps are loaded
-// together, the eight steps run on registers only (the dependent chain is the
-// row's arithmetic and one lane shift), and their results are stored
-// together.  A lane runs every step: before its line's first cell (and on a
-// lane past the last line) its operands are all +0, so its result stays +0
-// -- exactly what a missing neighbour contributes -- and past its line's
-// last cell it computes on clamped operands a value no lane reads again.
-// Only the steps on the line's cells store to V; the others store into the
-// lane's dump slot.  The step count is rounded up to a multiple of eight.
+// Eight steps a turn.  Lane j reads step s's operands at element
+// nx*j + s - j of each array, so a turn's are eight consecutive elements
+// from a per-lane pointer that moves by eight a turn; the steps run on
+// registers (the dependent chain is the row's arithmetic and one lane
+// shift), with the next turn's loads issued before them.  Steps outside a
+// lane's line read neighbouring lines or the slack around the arrays (at
+// most 7 elements before V and 6 past D): a step before the line starts
+// gives +0 (the value of a missing neighbour), a step after it ends gives a
+// value no lane reads, and neither is stored (the dump slot takes it).
+// Lanes past the last line read eight zeros Z and give +0.  The step count
+// is rounded up to a multiple of eight.
 constexpr int kSweepTurn = 8;
 
 template <bool LOWER>
-__device__ inline void grid_sweep(double* V, const double* E, const double* N, const double* D, double* dump, int nx,
-                                  int ny, int j)
+__device__ inline void grid_sweep(double* V, const double* E, const double* N, const double* D, double* dump,
+                                  const double* Z, int nx, int ny, int j)
 {
     const int steps = nx + ny - 1;
+    const int turns = (steps + kSweepTurn - 1) / kSweepTurn;
     const bool line = j < ny;
-    const int base = nx * (line ? j : 0);
-    const int ds = LOWER ? 1 : -1;
+    const int S = steps - 1;
+    // element of step s: nx*j + s - j; lower s = 8k + u, upper s = S - 8k - u
+    const int q0 = LOWER ? nx * j - j : nx * j - j + S - (kSweepTurn - 1);
+    const int stride = line ? (LOWER ? kSweepTurn : -kSweepTurn) : 0;
+    const double* pv = line ? V + q0 : Z;
+    const double* pf1 = line ? (LOWER ? N + q0 : E + 1 + q0) : Z;   // U[r-nx][r] / U[r][r+1]
+    const double* pf2 = line ? (LOWER ? E + q0 : N + nx + q0) : Z;  // U[r-1][r]  / U[r][r+nx]
+    const double* pd = line ? D + q0 : Z;
     double y = 0.0;   // the lane's last result
-    for (int s0 = LOWER ? 0 : steps - 1, k = 0; k < steps; k += kSweepTurn, s0 += kSweepTurn * ds) {
-        double v[kSweepTurn], f1[kSweepTurn], f2[kSweepTurn], d[kSweepTurn];
-        double* at[kSweepTurn];
+    auto load = [&](int k, double (&v)[kSweepTurn], double (&f1)[kSweepTurn], double (&f2)[kSweepTurn],
+                    double (&d)[kSweepTurn]) {
+        const int o = k * stride;
 #pragma unroll
         for (int u = 0; u < kSweepTurn; u++) {
-            const int i = s0 + u * ds - j;
-            const int r = (i < 0 ? 0 : i >= nx ? nx - 1 : i) + base;   // clamped: a valid cell
-            const bool before = !line || (LOWER ? i < 0 : i >= nx);
-            v[u] = before ? 0.0 : V[r];
-            f1[u] = before ? 0.0 : LOWER ? N[r] : E[1 + r];    // U[r-nx][r] / U[r][r+1]
-            f2[u] = before ? 0.0 : LOWER ? E[r] : N[nx + r];   // U[r-1][r]  / U[r][r+nx]
-            d[u] = before ? 0.0 : D[r];
-            at[u] = line && i >= 0 && i < nx ? V + r : dump + j;
+            const int e = o + (LOWER ? u : kSweepTurn - 1 - u);
+            v[u] = pv[e];
+            f1[u] = pf1[e];
+            f2[u] = pf2[e];
+            d[u] = pd[e];
         }
-        __builtin_amdgcn_sched_barrier(0);   // all loads first: one wait per turn
-        double out[kSweepTurn];
+    };
+    auto run = [&](int k, const double (&v)[kSweepTurn], const double (&f1)[kSweepTurn],
+                   const double (&f2)[kSweepTurn], const double (&d)[kSweepTurn]) {
+        const int o = k * stride;
+        const int i0 = LOWER ? kSweepTurn * k - j : S - kSweepTurn * k - j;   // cell of step 0
 #pragma unroll
         for (int u = 0; u < kSweepTurn; u++) {
+            const int i = LOWER ? i0 + u : i0 - u;
             const double nb = LOWER ? lane_from_prev(y) : lane_from_next(y);
             double acc = LOWER ? v[u] - f1[u] * nb : v[u] - f1[u] * y;
             acc = LOWER ? acc - f2[u] * y : acc - f2[u] * nb;
-            y = acc * d[u];
-            out[u] = y;
+            const double yn = acc * d[u];
+            y = (LOWER ? i < 0 : i >= nx) ? 0.0 : yn;
+            double* at = line && i >= 0 && i < nx ? V + (q0 + o + (LOWER ? u : kSweepTurn - 1 - u)) : dump + j;
+            *at = y;
         }
-        __builtin_amdgcn_sched_barrier(0);   // the stores after the chain
-#pragma unroll
-        for (int u = 0; u < kSweepTurn; u++) *at[u] = out[u];
+    };
+    // the next turn's operands are loaded before a turn's steps (two register
+    // sets, no copies)
+    double cv[kSweepTurn], c1[kSweepTurn], c2[kSweepTurn], cd[kSweepTurn];
+    double nv[kSweepTurn], n1[kSweepTurn], n2[kSweepTurn], nd[kSweepTurn];
+    load(0, cv, c1, c2, cd);
+    for (int k = 0; k < turns; k += 2) {
+        if (k + 1 < turns) load(k + 1, nv, n1, n2, nd);
+        run(k, cv, c1, c2, cd);
+        if (k + 1 >= turns) break;
+        if (k + 2 < turns) load(k + 2, cv, c1, c2, cd);
+        run(k + 1, nv, n1, n2, nd);
     }
 }
 
@@ -482,11 +505,12 @@ __global__ __launch_bounds__(kGridNT) void k_cg_ic0_grid(GridCgArgs a)
     __shared__ double red[kGridNT / kWave];
     __shared__ KState ks;
     const int n = a.n, nx = a.nx, tid = threadIdx.x;
-    double* V = lds;
+    double* V = lds + kGridSlack;
     double* E = V + n;        // E[1 + r] = U[r][r+1]
     double* N = E + n + 1;    // N[nx + r] = U[r][r+nx]
     double* D = N + n + nx;
     double* dump = D + n;
+    double* Z = dump + kWave;   // eight zeros
     for (int k = tid; k < n; k += kGridNT) {
         E[1 + k] = a.east[k];
         N[nx + k] = a.north[k];
@@ -494,6 +518,8 @@ __global__ __launch_bounds__(kGridNT) void k_cg_ic0_grid(GridCgArgs a)
     }
     if (tid == 0) E[0] = 0.0;
     for (int k = tid; k < nx; k += kGridNT) N[k] = 0.0;
+    if (tid < kGridSlack) lds[tid] = 0.0;
+    if (tid < 8) Z[tid] = 0.0;
     if (tid == 0) ks = *a.st;
     double x[kGridRows], r[kGridRows], p[kGridRows], q[kGridRows];
     int gi[kGridRows], gj[kGridRows], ge[kGridRows];   // grid position and first entry of each row
@@ -516,8 +542,8 @@ __global__ __launch_bounds__(kGridNT) void k_cg_ic0_grid(GridCgArgs a)
             if (tid + u * kGridNT < n) V[tid + u * kGridNT] = r[u];
         lds_barrier();
         if (tid < kWave) {
-            grid_sweep<true>(V, E, N, D, dump, nx, a.ny, tid);
-            grid_sweep<false>(V, E, N, D, dump, nx, a.ny, tid);
+            grid_sweep<true>(V, E, N, D, dump, Z, nx, a.ny, tid);
+            grid_sweep<false>(V, E, N, D, dump, Z, nx, a.ny, tid);
         }
         lds_barrier();
         // A's entries for q = A p, loaded now so they arrive while rho and p
