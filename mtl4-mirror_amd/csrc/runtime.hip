@@ -527,8 +527,11 @@ int m4g_dot(m4g_vector x, m4g_vector y, double* result)
         M4G_REQUIRE(x->n == y->n, M4G_ERR_SIZE, "m4g_dot: incompatible size");
         ReduceWs& ws = reduce_ws();
         double* out = ws.partials + kReduceBlocks * 16;
-        launch_dot(x->n, (const double*)x->ptr, (const double*)y->ptr, out, current_stream());
-        *result = read_scalar(out);
+        hipStream_t s = current_stream();
+        Mailbox mb(s);
+        const unsigned q = mb.arm();
+        launch_dot_post(x->n, (const double*)x->ptr, (const double*)y->ptr, out, mb.box(), q, s);
+        *result = mb.wait(q)[0];
         return M4G_OK;
     });
 }
@@ -566,8 +569,11 @@ int m4g_unary_dot(m4g_vector x, double* result)
         check_f64(x, "m4g_unary_dot");
         ReduceWs& ws = reduce_ws();
         double* out = ws.partials + kReduceBlocks * 16;
-        launch_sumsq(x->n, (const double*)x->ptr, out, current_stream());
-        *result = read_scalar(out);
+        hipStream_t s = current_stream();
+        Mailbox mb(s);
+        const unsigned q = mb.arm();
+        launch_sumsq_post(x->n, (const double*)x->ptr, out, mb.box(), q, s);
+        *result = mb.wait(q)[0];
         return M4G_OK;
     });
 }
